@@ -1,0 +1,139 @@
+"""Minimal FITS image reader for beams and maps (host-side ingestion).
+
+Reads the subset of FITS that the reference's beam and map readers consume
+(src/beam.cc:295-380, src/fitsData.cc:56-315): 2880-byte blocks of 80-column
+header cards, IMAGE HDUs with BITPIX 8/16/32/64/-32/-64, BZERO/BSCALE, and
+named extensions (EXTNAME, matched case-insensitively like the reference's
+"image"/"IMAGE" and "mask"/"MASK" lookups).  Pixels come back in FITS storage
+order (NAXIS1 fastest), which is the order the reference fills its arrays in.
+"""
+import numpy as np
+
+_BLOCK = 2880
+_DTYPES = {8: ">u1", 16: ">i2", 32: ">i4", 64: ">i8", -32: ">f4", -64: ">f8"}
+
+
+def _parse_value(raw):
+    v = raw.split("/", 1)[0].strip() if not raw.strip().startswith("'") \
+        else raw.strip()
+    if v.startswith("'"):
+        end = v.find("'", 1)
+        while end != -1 and end + 1 < len(v) and v[end + 1] == "'":
+            end = v.find("'", end + 2)
+        return v[1:end].replace("''", "'").rstrip()
+    if v in ("T", "F"):
+        return v == "T"
+    try:
+        return int(v)
+    except ValueError:
+        pass
+    try:
+        return float(v.replace("D", "E"))
+    except ValueError:
+        return v
+
+
+class HDU:
+    def __init__(self, header, data):
+        self.header = header
+        self.data = data
+
+    @property
+    def name(self):
+        return str(self.header.get("EXTNAME", "")).strip()
+
+
+def read_hdus(path):
+    """Return all HDUs of a FITS file as a list of HDU(header, data)."""
+    with open(path, "rb") as fh:
+        buf = fh.read()
+    hdus = []
+    pos = 0
+    while pos + _BLOCK <= len(buf):
+        header = {}
+        done = False
+        while not done:
+            block = buf[pos:pos + _BLOCK]
+            pos += _BLOCK
+            if len(block) < _BLOCK:
+                raise ValueError("truncated FITS header in %s" % path)
+            for i in range(0, _BLOCK, 80):
+                card = block[i:i + 80].decode("ascii", "replace")
+                key = card[:8].strip()
+                if key == "END":
+                    done = True
+                    break
+                if card[8:10] == "= ":
+                    header[key] = _parse_value(card[10:])
+        bitpix = int(header.get("BITPIX", 8))
+        naxis = int(header.get("NAXIS", 0))
+        dims = [int(header.get("NAXIS%d" % (k + 1), 0)) for k in range(naxis)]
+        gcount = int(header.get("GCOUNT", 1))
+        pcount = int(header.get("PCOUNT", 0))
+        nelem = int(np.prod(dims)) if naxis > 0 else 0
+        nbytes = abs(bitpix) // 8 * gcount * (pcount + nelem) if naxis else 0
+        data = None
+        xt = str(header.get("XTENSION", "IMAGE")).strip().upper()
+        if nelem and xt == "IMAGE" and bitpix in _DTYPES:
+            raw = np.frombuffer(buf, dtype=_DTYPES[bitpix], count=nelem,
+                                offset=pos)
+            arr = raw.astype(np.float64 if bitpix < 0 else np.int64)
+            bscale = header.get("BSCALE", 1)
+            bzero = header.get("BZERO", 0)
+            if bscale != 1 or bzero != 0:
+                arr = arr * bscale + bzero
+            data = arr.reshape(dims[::-1])
+        pos += (nbytes + _BLOCK - 1) // _BLOCK * _BLOCK
+        hdus.append(HDU(header, data))
+    return hdus
+
+
+def _split_ext(path):
+    """cfitsio extended syntax 'file.fits[k]' selects HDU k (0-based)."""
+    if path.endswith("]") and "[" in path:
+        base, ext = path[:-1].rsplit("[", 1)
+        return base, int(ext)
+    return path, None
+
+
+def read_beam(path):
+    """Pixels and PIXSIZE (or PIXSCALE) of a beam image, as beam::readFile.
+
+    Returns (pixels flattened in FITS order, pixsize in arcsec)."""
+    base, ext = _split_ext(path)
+    hdus = read_hdus(base)
+    h = hdus[ext if ext is not None else 0]
+    if h.data is None or h.data.ndim != 2:
+        raise ValueError("Input BEAM is not 2D")
+    pixsize = h.header.get("PIXSIZE", h.header.get("PIXSCALE"))
+    if pixsize is None:
+        raise ValueError("Unable to find pixel scale information in " + path)
+    return np.asarray(h.data, dtype=np.float64).ravel(), float(pixsize)
+
+
+def read_map(path, ignore_mask=False):
+    """Unmasked pixels of a map, as fitsData::readData (no mean subtraction).
+
+    The image comes from the extension named image/IMAGE, else the primary
+    HDU; pixels whose MASK value is non-zero are dropped."""
+    base, ext = _split_ext(path)
+    hdus = read_hdus(base)
+    if ext is not None:
+        img = hdus[ext]
+    else:
+        named = [h for h in hdus if h.name.lower() == "image"]
+        img = named[0] if named else hdus[0]
+    if img.data is None or img.data.ndim != 2:
+        raise ValueError("Image is not 2D")
+    data = np.asarray(img.data, dtype=np.float64).ravel()
+    if not ignore_mask:
+        masks = [h for h in hdus if h.name.lower() == "mask"]
+        if masks:
+            m = masks[0].data
+            if m is None or m.shape != img.data.shape:
+                raise ValueError("Mask does not match image extent")
+            keep = np.asarray(m).ravel() == 0
+            if not keep.any():
+                raise ValueError("All data masked")
+            data = data[keep]
+    return data
