@@ -1,0 +1,160 @@
+/*
+ * pofd_affine_amd.h -- C ABI of the MI355X (gfx950) P(D) likelihood library.
+ *
+ * This is the drop-in boundary for the likelihood call that the reference's
+ * MPI slaves make (affineEnsemble::slaveSample -> pofdMCMC::getLogLike ->
+ * calcLike::getLogLike, src/affineEnsemble.cc:1490, src/pofdMCMC.cc:347-352,
+ * include/calcLike.h:275).  The reference has no foreign-function layer of its
+ * own; each entry point below names the reference interface it replaces.
+ * Plain C types only: pointers, sizes, doubles, float32 parameter vectors
+ * (paramSet stores float, include/paramSet.h:20).
+ *
+ * Threading: a context is bound to one host thread and one HIP device, like
+ * the reference's one-calcLike-per-MPI-rank model (calcLike.h:47-48 mutable
+ * state is not re-entrant).  All functions return POFD_OK (0) or a POFD_E_*
+ * code; pofd_last_error() then holds "class::method: message" in the style of
+ * affineExcept (include/affineExcept.h).
+ */
+#ifndef POFD_AFFINE_AMD_H
+#define POFD_AFFINE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, also used per walker in the batch outputs. */
+enum {
+  POFD_OK = 0,
+  POFD_E_SPLIT = 1,         /* PDFactory::findSplitPoint threw
+                               (src/PDFactory.cc:753-920) */
+  POFD_E_INVALID_MODEL = 2, /* initPD "Invalid model": a knot value is NaN
+                               (src/numberCountsKnots.cc:273-284,
+                               src/PDFactory.cc:1046-1047) */
+  POFD_E_MEAN = 3,          /* unwrapAndNormalizePD non-finite mean
+                               (src/PDFactory.cc:998-1004) */
+  POFD_E_BADLIKE = 4,       /* calcLikeSingle::bad_like returned: sigma
+                               multiplier < 0 (src/calcLike.cc:403); the
+                               walker's loglike carries 1e25 per beam set as
+                               in the reference */
+  POFD_E_ARG = 5,           /* invalid argument (reference throws) */
+  POFD_E_INTERNAL = 6,      /* HIP runtime failure or unsupported setup */
+  POFD_E_NOGPU = 7          /* no usable gfx950 device */
+};
+
+/* Options for pofd_set_option. */
+enum {
+  /* Spectral components whose Gaussian noise factor
+     exp(-sigma^2 omega^2 / 2) is below exp(-value) are treated as zero.
+     Default 44.3614195558365 (= 64 ln 2, i.e. below 2^-64 of the DC term).
+     746 reproduces exactly the components the reference's exp() already
+     underflows to 0.0 (src/PDFactory.cc:1180-1183). */
+  POFD_OPT_SPECTRAL_CUT = 1
+};
+
+typedef struct pofd_ctx pofd_ctx;
+typedef struct pofd_beam pofd_beam;
+
+/* ---- context --------------------------------------------------------- */
+int pofd_create(pofd_ctx **out, int hip_device);
+void pofd_destroy(pofd_ctx *ctx);
+const char *pofd_last_error(const pofd_ctx *ctx);
+int pofd_set_option(pofd_ctx *ctx, int option, double value);
+
+/* ---- beams: replaces beam::readFile (after the FITS read) and
+        beam::makeHistogram, src/beam.cc:295-458, 467-556.  Host code,
+        bit-exact with the reference (glibc log2, same operation order). ---- */
+int pofd_beam_create(pofd_beam **out, const double *pix, uint64_t npix,
+                     double pixsize_arcsec, double minval);
+int pofd_beam_make_histogram(pofd_beam *bm, uint32_t nbins);
+void pofd_beam_destroy(pofd_beam *bm);
+/* which: 0 pos pixels (sorted), 1 |neg| pixels (sorted), 2 pos inverse,
+   3 neg inverse, 4 pos hist weights, 5 pos hist inverse values,
+   6 neg hist weights, 7 neg hist inverse values.  *n receives the length;
+   out may be NULL to query it. */
+int pofd_beam_get(const pofd_beam *bm, int which, double *out, uint32_t *n);
+/* info[16]: npos, nneg, haspos, hasneg, pixsize, minval, is_pos_hist,
+   is_neg_hist, posnbins, negnbins, area (deg^2), area_pos, area_neg,
+   area_pix, area_sq, max positive pixel */
+int pofd_beam_info(const pofd_beam *bm, double *info);
+
+/* ---- data: replaces fitsData::applyBinning and fitsData::meanSubtract,
+        src/fitsData.cc:393-437.  Host code, bit-exact. ---- */
+int pofd_data_bin(const double *data, uint64_t n, uint32_t nbins,
+                  uint32_t *bins_out, double *bincent0, double *bindelta);
+int pofd_data_mean_subtract(double *data, uint64_t n, double *mean_out);
+
+/* ---- 1D likelihood setup: replaces calcLike::readDataFromFiles grouping
+        (src/calcLike.cc:688-758), calcLikeSingle::setSigmaBase /
+        setLikeNorm / setExpConf (src/calcLike.cc:241-308) and
+        calcLike::setKnotPositions / setFFTSize / setNInterp. ---- */
+/* knots in Jy (double, strictly increasing, > 0); nflux in
+   {32768, 65536, 131072, 262144}; ninterp >= 16 */
+int pofd1d_configure(pofd_ctx *ctx, uint32_t nflux, uint32_t ninterp,
+                     const double *knot_pos, uint32_t nknots,
+                     double exp_conf);
+/* adds a beam set (the beam is copied); *beamset_id receives its index */
+int pofd1d_add_beamset(pofd_ctx *ctx, const pofd_beam *bm, int *beamset_id);
+/* adds a dataset to a beam set.  Unbinned: pix/npix (bins NULL).
+   Binned: bins/nbins/bincent0/bindelta from pofd_data_bin and npix = the
+   number of pixels that were binned (used for like_offset). */
+int pofd1d_add_field(pofd_ctx *ctx, int beamset_id, const double *pix,
+                     uint64_t npix, const uint32_t *bins, uint32_t nbins,
+                     double bincent0, double bindelta, double sigma_base,
+                     double like_norm, int *field_id);
+/* Builds the theta-independent device tables.  Replaces calcLike::setRRanges
+   (src/calcLike.cc:827-831 -> calcLikeSingle::setRRange :346-371); the R
+   range depends only on knot positions, beam and data. */
+int pofd1d_prepare(pofd_ctx *ctx);
+/* Overrides the R range of a beam set (call after pofd1d_prepare). */
+int pofd1d_set_rrange(pofd_ctx *ctx, int beamset_id, double min_rflux,
+                      double max_rflux);
+/* out[8]: minRFlux, maxRFlux, maxsigma_base, minDataFlux, maxDataFlux,
+   nfields, exp_conf, dflux */
+int pofd1d_beamset_info(const pofd_ctx *ctx, int beamset_id, double *out);
+
+/* ---- hot calls: replace calcLike::getLogLike(const paramSet&, bool&)
+        (src/calcLike.cc:884-955) for a batch of walkers.  theta rows use the
+        pofdMCMC layout [knot log10 values..., sigma_mult, bonus...]
+        (src/pofdMCMC.cc:67-91); row stride theta_stride floats. ---- */
+/* host buffers; synchronous */
+int pofd1d_loglike_batch(pofd_ctx *ctx, const float *theta, uint32_t n_walkers,
+                         uint32_t theta_stride, double *loglike_out,
+                         int32_t *status_out);
+/* device buffers on the context's device; asynchronous on `stream`
+   (a hipStream_t, 0 = the null stream) */
+int pofd1d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
+                                uint32_t n_walkers, uint32_t theta_stride,
+                                double *d_loglike_out, int32_t *d_status_out,
+                                void *stream);
+
+/* ---- parity / debug accessors for the most recent batch ---- */
+/* The P(D) as PDFactory::getPD(sigma, pd, setLog) leaves PD::pd_ (rotated,
+   normalised, log2 when is_log > 0) plus pd.minflux / pd.dflux
+   (src/PDFactory.cc:934-1009, src/PD.cc:169-184).  is_log < 0 returns the
+   clamped c2r output before the split (PDFactory's internal pofd array). */
+int pofd1d_get_pd(pofd_ctx *ctx, uint32_t walker, int beamset_id,
+                  int field_id, int is_log, double *pd_out, double *minflux,
+                  double *dflux);
+/* info[24]: split index, peak, minidx, minval, jitter, integral (before
+   normalisation), mean shift (-minflux), sgpos, sgneg, mn, var_noi, r0,
+   kmax (last spectral index kept), nrows, field loglike (before offset),
+   status, 1/integral, minflux, P(D) stored as log2, kmax of the beam set,
+   sum P, sum j P, sum of P above the split, sigma */
+int pofd1d_get_debug(pofd_ctx *ctx, uint32_t walker, int beamset_id,
+                     int field_id, double *info);
+/* prtrans after initPD for k = 0..kmax (interleaved re, im), kmax from
+   pofd1d_get_debug; *n receives kmax+1 */
+int pofd1d_get_spectrum(pofd_ctx *ctx, uint32_t walker, int beamset_id,
+                        double *out, uint32_t capacity, uint32_t *n);
+/* R interpolation nodes and log2 R values (PDFactory::initRInterp,
+   src/PDFactory.cc:426-527) of the most recent batch */
+int pofd1d_get_rinterp(pofd_ctx *ctx, uint32_t walker, int beamset_id,
+                       double *flux_out, double *log2r_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

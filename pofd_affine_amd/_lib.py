@@ -1,0 +1,303 @@
+"""ctypes binding of libpofd_affine_amd.so (the C ABI in include/).
+
+The library is built in-tree by ``pofd_affine_amd/csrc/Makefile`` (see
+``__graft_entry__.build()``).  There is no CPU fallback: if the library or a
+gfx950 device is missing, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "_build", "libpofd_affine_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "pofd_affine_amd.h")
+
+POFD_OK = 0
+POFD_E_SPLIT = 1
+POFD_E_INVALID_MODEL = 2
+POFD_E_MEAN = 3
+POFD_E_BADLIKE = 4
+POFD_E_ARG = 5
+POFD_E_INTERNAL = 6
+POFD_E_NOGPU = 7
+POFD_OPT_SPECTRAL_CUT = 1
+
+STATUS_NAMES = {0: "ok", 1: "split", 2: "invalid_model", 3: "mean",
+                4: "bad_like", 5: "arg", 6: "internal", 7: "nogpu"}
+
+D = C.c_double
+DP = C.POINTER(C.c_double)
+U32P = C.POINTER(C.c_uint32)
+I32P = C.POINTER(C.c_int32)
+FP = C.POINTER(C.c_float)
+VP = C.c_void_p
+
+_SIGS = {
+    "pofd_create": (C.c_int, [C.POINTER(VP), C.c_int]),
+    "pofd_destroy": (None, [VP]),
+    "pofd_last_error": (C.c_char_p, [VP]),
+    "pofd_set_option": (C.c_int, [VP, C.c_int, D]),
+    "pofd_beam_create": (C.c_int, [C.POINTER(VP), DP, C.c_uint64, D, D]),
+    "pofd_beam_make_histogram": (C.c_int, [VP, C.c_uint32]),
+    "pofd_beam_destroy": (None, [VP]),
+    "pofd_beam_get": (C.c_int, [VP, C.c_int, DP, U32P]),
+    "pofd_beam_info": (C.c_int, [VP, DP]),
+    "pofd_data_bin": (C.c_int, [DP, C.c_uint64, C.c_uint32, U32P, DP, DP]),
+    "pofd_data_mean_subtract": (C.c_int, [DP, C.c_uint64, DP]),
+    "pofd1d_configure": (C.c_int, [VP, C.c_uint32, C.c_uint32, DP,
+                                   C.c_uint32, D]),
+    "pofd1d_add_beamset": (C.c_int, [VP, VP, C.POINTER(C.c_int)]),
+    "pofd1d_add_field": (C.c_int, [VP, C.c_int, DP, C.c_uint64, U32P,
+                                   C.c_uint32, D, D, D, D,
+                                   C.POINTER(C.c_int)]),
+    "pofd1d_prepare": (C.c_int, [VP]),
+    "pofd1d_set_rrange": (C.c_int, [VP, C.c_int, D, D]),
+    "pofd1d_beamset_info": (C.c_int, [VP, C.c_int, DP]),
+    "pofd1d_loglike_batch": (C.c_int, [VP, FP, C.c_uint32, C.c_uint32, DP,
+                                       I32P]),
+    "pofd1d_loglike_batch_device": (C.c_int, [VP, VP, C.c_uint32,
+                                              C.c_uint32, VP, VP, VP]),
+    "pofd1d_get_pd": (C.c_int, [VP, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                DP, DP, DP]),
+    "pofd1d_get_debug": (C.c_int, [VP, C.c_uint32, C.c_int, C.c_int, DP]),
+    "pofd1d_get_spectrum": (C.c_int, [VP, C.c_uint32, C.c_int, DP,
+                                      C.c_uint32, U32P]),
+    "pofd1d_get_rinterp": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
+}
+
+_lib = None
+
+
+class PofdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d: %s)" % (msg, code,
+                                                 STATUS_NAMES.get(code, "?")))
+        self.code = code
+
+
+def exported_symbols():
+    """Names the public header declares (parsed from include/)."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(pofd(?:1d)?_[a-z_]+)\s*\(", txt)))
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise OSError("libpofd_affine_amd.so is not built (%s); run "
+                      "__graft_entry__.build()" % LIBPATH)
+    L = C.CDLL(LIBPATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- host prep
+class Beam:
+    """beam::readFile + makeHistogram restated (host, bit-exact)."""
+
+    def __init__(self, pix, pixsize, minval=1e-5, nhist=None):
+        pix = _f64(np.asarray(pix).ravel())
+        self.ptr = VP()
+        rc = lib().pofd_beam_create(C.byref(self.ptr), _dp(pix), pix.size,
+                                    float(pixsize), float(minval))
+        if rc:
+            raise PofdError(rc, "beam::readFile failed")
+        if nhist:
+            rc = lib().pofd_beam_make_histogram(self.ptr, int(nhist))
+            if rc:
+                raise PofdError(rc, "beam::makeHistogram failed")
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.pofd_beam_destroy(self.ptr)
+            self.ptr = None
+
+    def info(self):
+        out = np.zeros(16)
+        lib().pofd_beam_info(self.ptr, _dp(out))
+        keys = ["npos", "nneg", "haspos", "hasneg", "pixsize", "minval",
+                "is_pos_hist", "is_neg_hist", "posnbins", "negnbins",
+                "area", "area_pos", "area_neg", "area_pix", "area_sq",
+                "maxpos"]
+        return dict(zip(keys, out))
+
+    def array(self, which):
+        names = ["pospix", "negpix", "posinv", "neginv", "posweights",
+                 "poshistval", "negweights", "neghistval"]
+        i = names.index(which)
+        n = C.c_uint32()
+        lib().pofd_beam_get(self.ptr, i, None, C.byref(n))
+        out = np.zeros(n.value)
+        if n.value:
+            lib().pofd_beam_get(self.ptr, i, _dp(out), C.byref(n))
+        return out
+
+
+def data_bin(d, nbins):
+    """fitsData::applyBinning restated (host, bit-exact)."""
+    d = _f64(d)
+    bins = np.zeros(nbins, dtype=np.uint32)
+    c0, dl = D(), D()
+    rc = lib().pofd_data_bin(_dp(d), d.size, nbins,
+                             bins.ctypes.data_as(U32P), C.byref(c0),
+                             C.byref(dl))
+    if rc:
+        raise PofdError(rc, "fitsData::applyBinning failed")
+    return bins, c0.value, dl.value
+
+
+def data_mean_subtract(d):
+    d = _f64(d).copy()
+    m = D()
+    rc = lib().pofd_data_mean_subtract(_dp(d), d.size, C.byref(m))
+    if rc:
+        raise PofdError(rc, "fitsData::meanSubtract failed")
+    return d, m.value
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One pofd_ctx: a 1D likelihood (calcLike) on one gfx950 device."""
+
+    def __init__(self, device=0):
+        self.ptr = VP()
+        rc = lib().pofd_create(C.byref(self.ptr), int(device))
+        if rc:
+            raise PofdError(rc, "pofd_create failed (no gfx950 device?)")
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.pofd_destroy(self.ptr)
+            self.ptr = None
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().pofd_last_error(self.ptr)
+            raise PofdError(rc, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    def set_spectral_cut(self, value):
+        self._check(lib().pofd_set_option(self.ptr, POFD_OPT_SPECTRAL_CUT,
+                                          float(value)), "pofd_set_option")
+
+    def configure(self, nflux, ninterp, knot_pos, exp_conf=0.0):
+        kp = _f64(knot_pos)
+        self.nknots = kp.size
+        self.nflux = nflux
+        self.ninterp = ninterp
+        self._check(lib().pofd1d_configure(self.ptr, nflux, ninterp, _dp(kp),
+                                           kp.size, float(exp_conf)),
+                    "pofd1d_configure")
+
+    def add_beamset(self, beam):
+        i = C.c_int()
+        self._check(lib().pofd1d_add_beamset(self.ptr, beam.ptr, C.byref(i)),
+                    "pofd1d_add_beamset")
+        return i.value
+
+    def add_field(self, b, pix=None, bins=None, bincent0=0.0, bindelta=0.0,
+                  sigma_base=0.0025, like_norm=1.0, npix=None):
+        i = C.c_int()
+        if bins is not None:
+            bb = np.ascontiguousarray(bins, dtype=np.uint32)
+            n = int(npix if npix is not None else bb.sum())
+            rc = lib().pofd1d_add_field(self.ptr, b, None, n,
+                                        bb.ctypes.data_as(U32P), bb.size,
+                                        bincent0, bindelta, sigma_base,
+                                        like_norm, C.byref(i))
+        else:
+            p = _f64(pix)
+            rc = lib().pofd1d_add_field(self.ptr, b, _dp(p), p.size, None, 0,
+                                        0.0, 0.0, sigma_base, like_norm,
+                                        C.byref(i))
+        self._check(rc, "pofd1d_add_field")
+        return i.value
+
+    def prepare(self):
+        self._check(lib().pofd1d_prepare(self.ptr), "pofd1d_prepare")
+
+    def set_rrange(self, b, mn, mx):
+        self._check(lib().pofd1d_set_rrange(self.ptr, b, mn, mx),
+                    "pofd1d_set_rrange")
+
+    def beamset_info(self, b):
+        out = np.zeros(8)
+        self._check(lib().pofd1d_beamset_info(self.ptr, b, _dp(out)),
+                    "pofd1d_beamset_info")
+        return dict(zip(["minRFlux", "maxRFlux", "maxsigma_base",
+                         "minDataFlux", "maxDataFlux", "nfields", "exp_conf",
+                         "dflux"], out))
+
+    def loglike_batch(self, thetas):
+        t = np.ascontiguousarray(np.atleast_2d(thetas), dtype=np.float32)
+        w, stride = t.shape
+        ll = np.zeros(w)
+        st = np.zeros(w, dtype=np.int32)
+        self._check(lib().pofd1d_loglike_batch(self.ptr, t.ctypes.data_as(FP),
+                                               w, stride, _dp(ll),
+                                               st.ctypes.data_as(I32P)),
+                    "pofd1d_loglike_batch")
+        return ll, st
+
+    def loglike_batch_device(self, d_theta, nwalk, stride, d_ll, d_st,
+                             stream=0):
+        """Device pointers (ints) on this context's device; asynchronous."""
+        self._check(lib().pofd1d_loglike_batch_device(
+            self.ptr, VP(d_theta), nwalk, stride, VP(d_ll), VP(d_st),
+            VP(stream)), "pofd1d_loglike_batch_device")
+
+    def get_pd(self, walker, b=0, f=0, is_log=True):
+        """is_log True/False: PD::pd_ after getPD; "raw": the clamped,
+        unrotated, unnormalised transform output."""
+        is_log = -1 if is_log == "raw" else int(bool(is_log))
+        out = np.zeros(self.nflux)
+        mf, df = D(), D()
+        self._check(lib().pofd1d_get_pd(self.ptr, walker, b, f, is_log,
+                                        _dp(out), C.byref(mf), C.byref(df)),
+                    "pofd1d_get_pd")
+        return out, mf.value, df.value
+
+    DEBUG_KEYS = ["split", "peak", "minidx", "minval", "jitter", "integral",
+                  "tmn", "sgpos", "sgneg", "mn", "var_noi", "r0", "kmax",
+                  "nrows", "field_loglike", "status", "itot", "minflux",
+                  "pofd_is_log", "kmax_beamset", "S", "SJ", "Shigh", "sigma"]
+
+    def get_debug(self, walker, b=0, f=0):
+        out = np.zeros(24)
+        self._check(lib().pofd1d_get_debug(self.ptr, walker, b, f, _dp(out)),
+                    "pofd1d_get_debug")
+        return dict(zip(self.DEBUG_KEYS, out))
+
+    def get_spectrum(self, walker, b=0):
+        n = C.c_uint32()
+        self._check(lib().pofd1d_get_spectrum(self.ptr, walker, b, None, 0,
+                                              C.byref(n)), "spectrum size")
+        out = np.zeros(2 * n.value)
+        self._check(lib().pofd1d_get_spectrum(self.ptr, walker, b, _dp(out),
+                                              n.value, C.byref(n)),
+                    "pofd1d_get_spectrum")
+        return out[0::2] + 1j * out[1::2]
+
+    def get_rinterp(self, walker, b=0):
+        x = np.zeros(self.ninterp)
+        y = np.zeros(self.ninterp)
+        self._check(lib().pofd1d_get_rinterp(self.ptr, walker, b, _dp(x),
+                                             _dp(y)), "pofd1d_get_rinterp")
+        return x, y

@@ -1,0 +1,730 @@
+// C ABI: context, theta-independent device tables, batched launches.
+//
+// The host part restates, once per fit, the theta-independent pieces of the
+// reference's PDFactory setup so that the kernels only do per-walker work:
+//   calcLikeSingle::setSigmaBase like_offset        src/calcLike.cc:283-308
+//   calcLikeSingle::setRRange                       src/calcLike.cc:346-371
+//   numberCountsKnots::getRRangeInternal            src/numberCountsKnots.cc:345-365
+//   PDFactory::initRFlux                            src/PDFactory.cc:361-414
+//   PDFactory::initRInterp node fluxes              src/PDFactory.cc:446-463
+//   PDFactory::initR fill index range               src/PDFactory.cc:562-574
+//   GSL cspline_init tridiagonal factors (the node grid is fixed)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace {
+
+using pofd::BeamsetDev;
+using pofd::FieldDev;
+using pofd::WorkDev;
+
+const double kPi = 3.141592653589793238462643383279502884197;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+struct FieldHost {
+  std::vector<double> pix;        // sorted ascending (unbinned)
+  std::vector<uint32_t> bins;     // binned
+  uint64_t npix = 0;
+  bool binned = false;
+  double bincent0 = 0, bindelta = 0;
+  double sigma_base = 0, like_norm = 1, like_offset = 0;
+  double mindata = 0, maxdata = 0;
+  std::unique_ptr<DevBuf> dpix, dbins;
+};
+
+struct Beamset {
+  pofd::Beam beam;
+  std::vector<FieldHost> fields;
+  double minRFlux = NAN, maxRFlux = NAN, maxsigma_base = NAN;
+  double minDataFlux = NAN, maxDataFlux = NAN;
+  bool prepared = false;
+  BeamsetDev dev{};
+  std::vector<std::unique_ptr<DevBuf>> tables;
+  // workspace
+  WorkDev wk{};
+  std::unique_ptr<DevBuf> wbuf;
+  uint32_t wcap = 0;
+};
+
+}  // namespace
+
+struct pofd_ctx {
+  int device = 0;
+  std::string err;
+  double spectral_cut = 64.0 * 0.69314718055994530942;
+  bool configured = false;
+  uint32_t nflux = 0, ninterp = 0;
+  std::vector<double> knots, logknots;
+  double exp_conf = 0.0;
+  std::vector<std::unique_ptr<Beamset>> sets;
+  uint32_t last_nwalk = 0;
+  std::unique_ptr<DevBuf> dtheta, dll, dst;
+  size_t cap_theta = 0, cap_out = 0;
+};
+
+namespace {
+
+int fail(pofd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(pofd_ctx* c, hipError_t e, const char* what) {
+  return fail(c, POFD_E_INTERNAL,
+              std::string("pofd::hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+int dev_alloc(pofd_ctx* c, std::unique_ptr<DevBuf>& b, size_t bytes) {
+  if (b && b->bytes >= bytes) return POFD_OK;
+  b.reset(new DevBuf());
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(&b->p, bytes);
+  if (e != hipSuccess) {
+    b.reset();
+    return hip_fail(c, e, "hipMalloc");
+  }
+  b->bytes = bytes;
+  return POFD_OK;
+}
+
+template <typename T>
+int upload(pofd_ctx* c, Beamset& s, const std::vector<T>& v, const T** out) {
+  s.tables.emplace_back(new DevBuf());
+  DevBuf& b = *s.tables.back();
+  const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 8);
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(table)");
+  b.bytes = bytes;
+  if (!v.empty()) {
+    e = hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(table)");
+  }
+  *out = static_cast<const T*>(b.p);
+  return POFD_OK;
+}
+
+// gsl_interp_bsearch over [0, n-1] with the accelerator's clamping
+size_t bsearch_idx(const double* x, size_t n, double v) {
+  if (v <= x[0]) return 0;
+  if (v >= x[n - 1]) return n - 2;
+  size_t lo = 0, hi = n - 1;
+  while (hi > lo + 1) {
+    const size_t i = (hi + lo) / 2;
+    if (x[i] > v)
+      hi = i;
+    else
+      lo = i;
+  }
+  return lo;
+}
+
+std::vector<double2> twiddles(size_t n, size_t count) {
+  std::vector<double2> w(count);
+  for (size_t s = 0; s < count; ++s) {
+    const long double a = -2.0L * static_cast<long double>(kPi) *
+                          static_cast<long double>(s) / static_cast<long double>(n);
+    w[s] = make_double2(static_cast<double>(cosl(a)), static_cast<double>(sinl(a)));
+  }
+  return w;
+}
+
+int prepare_beamset(pofd_ctx* c, Beamset& s) {
+  const pofd::Beam& bm = s.beam;
+  const uint32_t N = c->nflux, ninterp = c->ninterp;
+  const unsigned nk = static_cast<unsigned>(c->knots.size());
+  if (s.fields.empty())
+    return fail(c, POFD_E_ARG, "calcLikeSingle::readDataFromFile: No data sets");
+  if (bm.pos.empty())
+    return fail(c, POFD_E_ARG,
+                "pofd1d_prepare: the beam has no positive pixels");
+  if (!bm.neg.empty())
+    return fail(c, POFD_E_ARG,
+                "pofd1d_prepare: beams with negative pixels are not supported "
+                "by the device path");
+  // data range (src/calcLike.cc:205-230) and sigma base (:283-308)
+  s.minDataFlux = s.fields[0].mindata;
+  s.maxDataFlux = s.fields[0].maxdata;
+  s.maxsigma_base = s.fields[0].sigma_base;
+  for (auto& f : s.fields) {
+    s.minDataFlux = std::min(s.minDataFlux, f.mindata);
+    s.maxDataFlux = std::max(s.maxDataFlux, f.maxdata);
+    if (f.sigma_base > s.maxsigma_base) s.maxsigma_base = f.sigma_base;
+  }
+  for (auto& f : s.fields) {
+    const double var = f.sigma_base * f.sigma_base + c->exp_conf * c->exp_conf;
+    f.like_offset = 0.0;
+    if (var > 0) {
+      const double ndd = static_cast<double>(f.npix);
+      f.like_offset = -0.5 * ndd * (log(2.0 * kPi * var) + 1.0);
+    }
+  }
+  // setRRange
+  if (std::isnan(s.minRFlux) || std::isnan(s.maxRFlux)) {
+    const double maxknot = c->knots[nk - 1];
+    const double maxRF = maxknot * bm.pos.back();
+    const double minRF = 0.0;  // positive beam only
+    const double sigma = sqrt(c->exp_conf * c->exp_conf +
+                              s.maxsigma_base * s.maxsigma_base);
+    s.minRFlux = 1.05 * minRF;
+    s.maxRFlux = 1.05 * maxRF + 7.5 * sigma;
+    if (s.maxDataFlux > s.maxRFlux) s.maxRFlux = s.maxDataFlux;
+  }
+  // initRFlux (simple case: 0 .. maxflux)
+  double minflux = s.minRFlux, maxflux = s.maxRFlux;
+  if (maxflux < minflux) std::swap(minflux, maxflux);
+  if (!(maxflux > 0))
+    return fail(c, POFD_E_ARG, "PDFactory::initPD: Invalid (non-positive) maxflux");
+  if (minflux > 0) minflux = 0.0;
+  if (minflux != 0.0)
+    return fail(c, POFD_E_ARG,
+                "pofd1d_prepare: negative R fluxes are not supported by the "
+                "device path");
+  const double inm1 = 1.0 / static_cast<double>(N - 1);
+  const int wrapRidx = static_cast<int>(N - 1);
+  const double minflux_R = 0.0;
+  const double dflux = maxflux * inm1;
+
+  // initRInterp node fluxes (positive part)
+  const double ininterpm1 = 1.0 / static_cast<double>(ninterp - 1);
+  const double mininterpflux = c->knots[0] * 1e-5;  // subedgemult
+  double maxinterpflux = c->knots[nk - 1] * bm.pos.back();
+  const double dinterp = 0.001 * (maxinterpflux - mininterpflux) * ininterpm1;
+  maxinterpflux -= dinterp;
+  const double dinterpflux = (log2(maxinterpflux / mininterpflux)) * ininterpm1;
+  std::vector<double> xnode(ninterp);
+  for (uint32_t i = 0; i < ninterp; ++i)
+    xnode[i] = mininterpflux * exp2(static_cast<double>(i) * dinterpflux);
+  for (uint32_t i = 1; i < ninterp; ++i)
+    if (!(xnode[i] > xnode[i - 1]))
+      return fail(c, POFD_E_ARG, "PDFactory::initRInterp: interpolation "
+                                 "fluxes not increasing");
+
+  // beam entries: histogram (weights, inverse bin means) or raw inverse
+  std::vector<double> bw, bieta;
+  if (bm.is_pos_hist) {
+    bw = bm.posw;
+    bieta = bm.posv;
+  } else {
+    bw.assign(bm.posinv.size(), 1.0);
+    bieta = bm.posinv;
+  }
+  const size_t nbeam = bw.size();
+  // knot-interval table: for every (beam entry, node) the knot interval of
+  // log2(flux * inverse beam) and its offset, or -1 where getR skips it
+  const double minknot = c->knots[0], maxknot = c->knots[nk - 1];
+  const double rmax = maxknot * bm.pos.back();  // getRRangeInternal().second
+  std::vector<int8_t> rk(nbeam * ninterp, -1);
+  std::vector<double> rdelx(nbeam * ninterp, 0.0);
+  for (size_t j = 0; j < nbeam; ++j)
+    for (uint32_t i = 0; i < ninterp; ++i) {
+      const double fd = xnode[i];
+      if (!(fd > 0 && fd <= rmax)) continue;
+      const double currarg = fd * bieta[j];
+      if (currarg < minknot || currarg >= maxknot) continue;
+      const double lx = log2(currarg);
+      const size_t k = bsearch_idx(c->logknots.data(), nk, lx);
+      rk[j * ninterp + i] = static_cast<int8_t>(k);
+      rdelx[j * ninterp + i] = lx - c->logknots[k];
+    }
+  // fixed LDL^T factors of the node spline system (GSL cspline_init +
+  // gsl_linalg_solve_symm_tridiag)
+  const uint32_t sys = ninterp - 2;
+  std::vector<double> ginv(ninterp - 1), diag(sys), off(sys), lalpha(sys),
+      lgamma(sys, 0.0);
+  for (uint32_t i = 0; i + 1 < ninterp; ++i) {
+    const double h = xnode[i + 1] - xnode[i];
+    ginv[i] = (h != 0.0) ? 1.0 / h : 0.0;
+  }
+  for (uint32_t i = 0; i < sys; ++i) {
+    const double h_i = xnode[i + 1] - xnode[i];
+    const double h_ip1 = xnode[i + 2] - xnode[i + 1];
+    off[i] = h_ip1;
+    diag[i] = 2.0 * (h_ip1 + h_i);
+  }
+  lalpha[0] = diag[0];
+  lgamma[0] = off[0] / lalpha[0];
+  for (uint32_t i = 1; i + 1 < sys; ++i) {
+    lalpha[i] = diag[i] - off[i - 1] * lgamma[i - 1];
+    lgamma[i] = off[i] / lalpha[i];
+  }
+  if (sys > 1) lalpha[sys - 1] = diag[sys - 1] - off[sys - 2] * lgamma[sys - 2];
+  // initR fill range
+  const double minI = xnode[0], maxI = xnode[ninterp - 1];
+  unsigned minitidx = static_cast<unsigned>(minI / dflux + 0.9999999999999999);
+  unsigned maxitidx = static_cast<unsigned>(maxI / dflux);
+  if (maxitidx > static_cast<unsigned>(wrapRidx)) maxitidx = wrapRidx;
+  // interval of the first flux of every 8-run of the grid
+  std::vector<uint16_t> kst(N / 8);
+  for (uint32_t b = 0; b < N / 8; ++b) {
+    const double cf = static_cast<double>(8 * b) * dflux;
+    kst[b] = static_cast<uint16_t>(bsearch_idx(xnode.data(), ninterp, cf));
+  }
+  const uint32_t M = N / 2, M2 = M / pofd::kM1;
+
+  BeamsetDev& d = s.dev;
+  std::memset(&d, 0, sizeof(d));
+  d.N = static_cast<int>(N);
+  d.M = static_cast<int>(M);
+  d.M2 = static_cast<int>(M2);
+  d.ninterp = static_cast<int>(ninterp);
+  d.nknots = static_cast<int>(nk);
+  d.nbeam = static_cast<int>(nbeam);
+  d.minit = static_cast<int>(minitidx);
+  d.maxit = static_cast<int>(maxitidx);
+  d.wrapRidx = wrapRidx;
+  d.dflux = dflux;
+  d.minflux_R = minflux_R;
+  d.logfac = log2(10.0);
+  double prefac = bm.pixsize / 3600.0;
+  d.prefac = prefac * prefac;
+  d.maxsigma_base = s.maxsigma_base;
+  d.iflux = (2.0 * kPi) / (N * dflux);
+  d.spectral_cut = c->spectral_cut;
+  for (unsigned i = 0; i < nk; ++i) d.logknots[i] = c->logknots[i];
+  s.tables.clear();
+  int rc;
+  if ((rc = upload(c, s, xnode, &d.xnode))) return rc;
+  if ((rc = upload(c, s, ginv, &d.ginv))) return rc;
+  if ((rc = upload(c, s, lalpha, &d.lalpha))) return rc;
+  if ((rc = upload(c, s, lgamma, &d.lgamma))) return rc;
+  if ((rc = upload(c, s, rk, &d.rk))) return rc;
+  if ((rc = upload(c, s, rdelx, &d.rdelx))) return rc;
+  if ((rc = upload(c, s, bw, &d.bw))) return rc;
+  if ((rc = upload(c, s, bieta, &d.bieta))) return rc;
+  if ((rc = upload(c, s, kst, &d.kst))) return rc;
+  const std::vector<double2> w256 = twiddles(256, 256), wlo = twiddles(M, 256),
+                             wm2 = twiddles(M2, M2);
+  if ((rc = upload(c, s, w256, &d.W256))) return rc;
+  if ((rc = upload(c, s, wlo, &d.Wlo))) return rc;
+  if ((rc = upload(c, s, wm2, &d.Wm2))) return rc;
+  d.nfields = static_cast<int>(s.fields.size());
+  for (size_t f = 0; f < s.fields.size(); ++f) {
+    FieldHost& fh = s.fields[f];
+    FieldDev& fd = d.fields[f];
+    fd.sigma_base = fh.sigma_base;
+    fd.like_offset = fh.like_offset;
+    fd.like_norm = fh.like_norm;
+    fd.binned = fh.binned ? 1 : 0;
+    fd.nbins = static_cast<uint32_t>(fh.bins.size());
+    fd.bincent0 = fh.bincent0;
+    fd.bindelta = fh.bindelta;
+    fd.npix = fh.binned ? 0 : fh.pix.size();
+    if (fh.binned) {
+      if ((rc = dev_alloc(c, fh.dbins, fh.bins.size() * sizeof(uint32_t)))) return rc;
+      hipError_t e = hipMemcpy(fh.dbins->p, fh.bins.data(),
+                               fh.bins.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(bins)");
+      fd.bins = static_cast<const uint32_t*>(fh.dbins->p);
+      fd.pix = nullptr;
+    } else {
+      if ((rc = dev_alloc(c, fh.dpix, fh.pix.size() * sizeof(double)))) return rc;
+      hipError_t e = hipMemcpy(fh.dpix->p, fh.pix.data(),
+                               fh.pix.size() * sizeof(double),
+                               hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(pix)");
+      fd.pix = static_cast<const double*>(fh.dpix->p);
+      fd.bins = nullptr;
+    }
+  }
+  s.wcap = 0;  // workspace layout depends on the tables
+  s.prepared = true;
+  return POFD_OK;
+}
+
+int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
+  if (s.wcap >= nwalk && s.wbuf) return POFD_OK;
+  const BeamsetDev& d = s.dev;
+  WorkDev& w = s.wk;
+  const size_t nf = static_cast<size_t>(d.nfields);
+  w.coef_stride = static_cast<size_t>(d.ninterp - 1) * 4;
+  w.ynode_stride = static_cast<size_t>(d.ninterp);
+  w.zrows_stride = static_cast<size_t>(2) * d.M2 * pofd::kM1;
+  w.spec_stride = static_cast<size_t>(d.M) + 1;
+  w.pofd_stride = static_cast<size_t>(d.N);
+  w.blk_stride = static_cast<size_t>(d.N / pofd::kBlk);
+  auto r8 = [](size_t n) { return (n + 7) & ~size_t(7); };  // 64-byte units
+  const size_t n_coef = r8(w.coef_stride) * nwalk;
+  const size_t n_ynode = r8(w.ynode_stride) * nwalk;
+  const size_t n_z = 2 * r8(w.zrows_stride) * nwalk;
+  const size_t n_spec = 2 * r8(2 * w.spec_stride) * nwalk;  // prtrans + xp
+  const size_t n_pofd = r8(w.pofd_stride) * nf * nwalk;
+  const size_t n_blk = 2 * r8(w.blk_stride) * nf * nwalk;
+  const size_t n_dbg = r8(pofd::kDbg) * nf * nwalk;
+  const size_t total = n_coef + n_ynode + n_z + n_spec + n_pofd + n_blk + n_dbg;
+  s.wbuf.reset();
+  int rc = dev_alloc(c, s.wbuf, total * sizeof(double));
+  if (rc) return rc;
+  double* base = static_cast<double*>(s.wbuf->p);
+  w.coef_stride = r8(w.coef_stride);
+  w.ynode_stride = r8(w.ynode_stride);
+  w.coef = base;
+  base += n_coef;
+  w.ynode = base;
+  base += n_ynode;
+  w.zrows_stride = r8(w.zrows_stride * 2) / 2;
+  w.zrows = reinterpret_cast<double2*>(base);
+  base += n_z;
+  w.spec_stride = r8(2 * w.spec_stride) / 2;
+  w.prtrans = reinterpret_cast<double2*>(base);
+  base += n_spec / 2;
+  w.xp = reinterpret_cast<double2*>(base);
+  base += n_spec / 2;
+  w.pofd_stride = r8(w.pofd_stride);
+  w.pofd = base;
+  base += n_pofd;
+  w.blk_stride = r8(w.blk_stride);
+  w.bmin = base;
+  base += n_blk / 2;
+  w.bsum = base;
+  base += n_blk / 2;
+  w.dbg = base;
+  s.wcap = nwalk;
+  return POFD_OK;
+}
+
+int check_ready(pofd_ctx* c) {
+  if (!c) return POFD_E_ARG;
+  if (!c->configured)
+    return fail(c, POFD_E_ARG, "calcLike::getLogLike: pofd1d_configure not called");
+  if (c->sets.empty())
+    return fail(c, POFD_E_ARG, "calcLike::getLogLike: no beam sets");
+  for (auto& s : c->sets)
+    if (!s->prepared)
+      return fail(c, POFD_E_ARG,
+                  "calcLikeSingle::getNegLogLike: R range not set "
+                  "(pofd1d_prepare not called)");
+  return POFD_OK;
+}
+
+}  // namespace
+
+extern "C" int pofd_create(pofd_ctx** out, int hip_device) {
+  if (!out) return POFD_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= hip_device ||
+      hip_device < 0)
+    return POFD_E_NOGPU;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, hip_device) != hipSuccess)
+    return POFD_E_NOGPU;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return POFD_E_NOGPU;
+  if (hipSetDevice(hip_device) != hipSuccess) return POFD_E_NOGPU;
+  auto* c = new (std::nothrow) pofd_ctx();
+  if (!c) return POFD_E_INTERNAL;
+  c->device = hip_device;
+  *out = c;
+  return POFD_OK;
+}
+
+extern "C" void pofd_destroy(pofd_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  delete c;
+}
+
+extern "C" const char* pofd_last_error(const pofd_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+extern "C" int pofd_set_option(pofd_ctx* c, int option, double value) {
+  if (!c) return POFD_E_ARG;
+  if (option == POFD_OPT_SPECTRAL_CUT) {
+    if (!(value > 0.0))
+      return fail(c, POFD_E_ARG, "pofd_set_option: spectral cut must be > 0");
+    c->spectral_cut = value;
+    for (auto& s : c->sets) s->dev.spectral_cut = value;
+    return POFD_OK;
+  }
+  return fail(c, POFD_E_ARG, "pofd_set_option: unknown option");
+}
+
+extern "C" int pofd1d_configure(pofd_ctx* c, uint32_t nflux, uint32_t ninterp,
+                                const double* knot_pos, uint32_t nknots,
+                                double exp_conf) {
+  if (!c) return POFD_E_ARG;
+  if (nflux != 32768 && nflux != 65536 && nflux != 131072 && nflux != 262144)
+    return fail(c, POFD_E_ARG,
+                "pofd1d_configure: nflux must be 2^15, 2^16, 2^17 or 2^18");
+  if (ninterp < 16 || ninterp > 2048)
+    return fail(c, POFD_E_ARG, "pofd1d_configure: ninterp must be in [16, 2048]");
+  if (!knot_pos || nknots < 2 || nknots > static_cast<uint32_t>(pofd::kMaxKnots))
+    return fail(c, POFD_E_ARG, "pofd1d_configure: 2 to 64 knots required");
+  for (uint32_t i = 0; i < nknots; ++i)
+    if (!(knot_pos[i] > 0.0))
+      return fail(c, POFD_E_ARG,
+                  "numberCountsKnots::setKnots: Negative knot positions not "
+                  "allowed");
+  for (uint32_t i = 1; i < nknots; ++i)
+    if (!(knot_pos[i] > knot_pos[i - 1]))
+      return fail(c, POFD_E_ARG,
+                  "numberCountsKnots::setKnotPositions: Knot positions not "
+                  "monotonically increasing");
+  c->nflux = nflux;
+  c->ninterp = ninterp;
+  c->knots.assign(knot_pos, knot_pos + nknots);
+  c->logknots.resize(nknots);
+  for (uint32_t i = 0; i < nknots; ++i) c->logknots[i] = log2(c->knots[i]);
+  c->exp_conf = exp_conf;
+  c->configured = true;
+  for (auto& s : c->sets) s->prepared = false;
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_add_beamset(pofd_ctx* c, const pofd_beam* bm, int* id) {
+  if (!c || !bm || !id) return POFD_E_ARG;
+  std::unique_ptr<Beamset> s(new Beamset());
+  s->beam = bm->beam;
+  c->sets.push_back(std::move(s));
+  *id = static_cast<int>(c->sets.size() - 1);
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_add_field(pofd_ctx* c, int b, const double* pix,
+                                uint64_t npix, const uint32_t* bins,
+                                uint32_t nbins, double bincent0,
+                                double bindelta, double sigma_base,
+                                double like_norm, int* fid) {
+  if (!c || b < 0 || static_cast<size_t>(b) >= c->sets.size())
+    return fail(c, POFD_E_ARG, "pofd1d_add_field: bad beam set");
+  Beamset& s = *c->sets[b];
+  if (s.fields.size() >= static_cast<size_t>(pofd::kMaxFields))
+    return fail(c, POFD_E_ARG, "pofd1d_add_field: too many fields");
+  if (npix == 0)
+    return fail(c, POFD_E_ARG,
+                "calcLikeSingle::readDataFromFile: No unmasked pixels in data");
+  if (!(like_norm != 0.0))
+    return fail(c, POFD_E_ARG, "pofd1d_add_field: like_norm must be non-zero");
+  FieldHost f;
+  f.sigma_base = sigma_base;
+  f.like_norm = like_norm;
+  f.npix = npix;
+  if (bins) {
+    if (nbins == 0) return fail(c, POFD_E_ARG, "fitsData::applyBinning: no bins");
+    f.binned = true;
+    f.bins.assign(bins, bins + nbins);
+    f.bincent0 = bincent0;
+    f.bindelta = bindelta;
+    // fitsData::getMinMax binned branch src/fitsData.cc:353-357
+    f.mindata = bincent0;
+    f.maxdata = bincent0 + static_cast<double>(nbins - 1) * bindelta;
+  } else {
+    if (!pix) return fail(c, POFD_E_ARG, "pofd1d_add_field: no pixels");
+    f.pix.assign(pix, pix + npix);
+    pofd::data_minmax(f.pix.data(), f.pix.size(), f.mindata, f.maxdata);
+    std::sort(f.pix.begin(), f.pix.end());
+  }
+  if (!std::isfinite(f.mindata) || !std::isfinite(f.maxdata))
+    return fail(c, POFD_E_ARG,
+                "calcLikeSingle::readDataFromFile: Invalid data range "
+                "encountered in data file");
+  s.fields.push_back(std::move(f));
+  s.prepared = false;
+  if (fid) *fid = static_cast<int>(s.fields.size() - 1);
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_prepare(pofd_ctx* c) {
+  if (!c) return POFD_E_ARG;
+  if (!c->configured)
+    return fail(c, POFD_E_ARG, "pofd1d_prepare: pofd1d_configure not called");
+  if (c->sets.empty()) return fail(c, POFD_E_ARG, "pofd1d_prepare: no beam sets");
+  if (hipSetDevice(c->device) != hipSuccess)
+    return fail(c, POFD_E_INTERNAL, "pofd1d_prepare: hipSetDevice failed");
+  for (auto& s : c->sets) {
+    s->minRFlux = s->maxRFlux = NAN;
+    int rc = prepare_beamset(c, *s);
+    if (rc) return rc;
+  }
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_set_rrange(pofd_ctx* c, int b, double mn, double mx) {
+  if (!c || b < 0 || static_cast<size_t>(b) >= c->sets.size())
+    return fail(c, POFD_E_ARG, "pofd1d_set_rrange: bad beam set");
+  Beamset& s = *c->sets[b];
+  s.minRFlux = mn;
+  s.maxRFlux = mx;
+  return prepare_beamset(c, s);
+}
+
+extern "C" int pofd1d_beamset_info(const pofd_ctx* c, int b, double* out) {
+  if (!c || !out || b < 0 || static_cast<size_t>(b) >= c->sets.size())
+    return POFD_E_ARG;
+  const Beamset& s = *c->sets[b];
+  out[0] = s.minRFlux;
+  out[1] = s.maxRFlux;
+  out[2] = s.maxsigma_base;
+  out[3] = s.minDataFlux;
+  out[4] = s.maxDataFlux;
+  out[5] = static_cast<double>(s.fields.size());
+  out[6] = c->exp_conf;
+  out[7] = s.prepared ? s.dev.dflux : NAN;
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
+                                           uint32_t nwalk, uint32_t stride,
+                                           double* d_ll, int32_t* d_st,
+                                           void* stream) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (nwalk == 0) return POFD_OK;
+  if (!d_theta || !d_ll || !d_st)
+    return fail(c, POFD_E_ARG, "pofd1d_loglike_batch_device: null buffer");
+  if (stride < c->knots.size() + 1)
+    return fail(c, POFD_E_ARG,
+                "calcLike::getLogLike: Not enough elements in paramSet");
+  for (size_t b = 0; b < c->sets.size(); ++b) {
+    Beamset& s = *c->sets[b];
+    if ((rc = ensure_workspace(c, s, nwalk))) return rc;
+    rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta,
+                                nwalk, stride, d_ll, d_st, b > 0 ? 1 : 0,
+                                stream);
+    if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: launch failed: " +
+                                   std::string(hipGetErrorString(hipGetLastError())));
+  }
+  c->last_nwalk = nwalk;
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_loglike_batch(pofd_ctx* c, const float* theta,
+                                    uint32_t nwalk, uint32_t stride,
+                                    double* ll, int32_t* st) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (nwalk == 0) return POFD_OK;
+  if (!theta || !ll || !st)
+    return fail(c, POFD_E_ARG, "pofd1d_loglike_batch: null buffer");
+  const size_t tb = static_cast<size_t>(nwalk) * stride * sizeof(float);
+  if ((rc = dev_alloc(c, c->dtheta, tb))) return rc;
+  if ((rc = dev_alloc(c, c->dll, nwalk * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(c, c->dst, nwalk * sizeof(int32_t)))) return rc;
+  hipError_t e = hipMemcpy(c->dtheta->p, theta, tb, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(theta)");
+  rc = pofd1d_loglike_batch_device(c, static_cast<const float*>(c->dtheta->p),
+                                   nwalk, stride,
+                                   static_cast<double*>(c->dll->p),
+                                   static_cast<int32_t*>(c->dst->p), nullptr);
+  if (rc) return rc;
+  e = hipMemcpy(ll, c->dll->p, nwalk * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(loglike)");
+  e = hipMemcpy(st, c->dst->p, nwalk * sizeof(int32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(status)");
+  return POFD_OK;
+}
+
+namespace {
+int dbg_record(pofd_ctx* c, uint32_t walker, int b, int f, double* rec) {
+  if (!c || b < 0 || static_cast<size_t>(b) >= c->sets.size())
+    return fail(c, POFD_E_ARG, "debug accessor: bad beam set");
+  Beamset& s = *c->sets[b];
+  if (!s.wbuf || walker >= s.wcap || walker >= c->last_nwalk)
+    return fail(c, POFD_E_ARG, "debug accessor: no such walker in last batch");
+  if (f < 0 || f >= s.dev.nfields)
+    return fail(c, POFD_E_ARG, "debug accessor: bad field");
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(c, e, "hipDeviceSynchronize");
+  const double* src = s.wk.dbg + (static_cast<size_t>(walker) * s.dev.nfields + f) * pofd::kDbg;
+  e = hipMemcpy(rec, src, pofd::kDbg * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(debug)");
+  return POFD_OK;
+}
+}  // namespace
+
+extern "C" int pofd1d_get_debug(pofd_ctx* c, uint32_t walker, int b, int f,
+                                double* info) {
+  if (!info) return POFD_E_ARG;
+  return dbg_record(c, walker, b, f, info);
+}
+
+extern "C" int pofd1d_get_pd(pofd_ctx* c, uint32_t walker, int b, int f,
+                             int is_log, double* pd_out, double* minflux,
+                             double* dflux) {
+  double rec[pofd::kDbg];
+  int rc = dbg_record(c, walker, b, f, rec);
+  if (rc) return rc;
+  if (!pd_out) return fail(c, POFD_E_ARG, "pofd1d_get_pd: null output");
+  if (rec[15] != POFD_OK)
+    return fail(c, static_cast<int>(rec[15]),
+                "pofd1d_get_pd: the P(D) of this walker failed");
+  Beamset& s = *c->sets[b];
+  const int N = s.dev.N;
+  std::unique_ptr<DevBuf> out;
+  if ((rc = dev_alloc(c, out, N * sizeof(double)))) return rc;
+  const double* pofd = s.wk.pofd + (static_cast<size_t>(walker) * s.dev.nfields + f) * s.wk.pofd_stride;
+  const int already_log = rec[18] != 0.0;
+  if (already_log && is_log <= 0)
+    return fail(c, POFD_E_ARG,
+                "pofd1d_get_pd: unbinned fields keep only the log2 P(D)");
+  // is_log < 0: the clamped, unrotated, unnormalised transform output
+  const int split = (is_log < 0) ? 0 : static_cast<int>(rec[0]);
+  const double itot = (is_log < 0) ? 1.0 : rec[16];
+  rc = pofd::launch_materialize_pd(pofd, N, split, itot, is_log > 0 ? 1 : 0,
+                                   already_log, static_cast<double*>(out->p),
+                                   nullptr);
+  if (rc) return fail(c, rc, "pofd1d_get_pd: launch failed");
+  hipError_t e = hipMemcpy(pd_out, out->p, N * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(pd)");
+  if (minflux) *minflux = rec[17];
+  if (dflux) *dflux = s.dev.dflux;
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_get_spectrum(pofd_ctx* c, uint32_t walker, int b,
+                                   double* out, uint32_t capacity,
+                                   uint32_t* n) {
+  double rec[pofd::kDbg];
+  int rc = dbg_record(c, walker, b, 0, rec);
+  if (rc) return rc;
+  Beamset& s = *c->sets[b];
+  const uint32_t cnt = static_cast<uint32_t>(rec[19]) + 1;
+  if (n) *n = cnt;
+  if (!out) return POFD_OK;
+  if (capacity < cnt) return fail(c, POFD_E_ARG, "pofd1d_get_spectrum: capacity");
+  const double2* src = s.wk.prtrans + static_cast<size_t>(walker) * s.wk.spec_stride;
+  hipError_t e = hipMemcpy(out, src, cnt * sizeof(double2), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(spectrum)");
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_get_rinterp(pofd_ctx* c, uint32_t walker, int b,
+                                  double* flux_out, double* log2r_out) {
+  double rec[pofd::kDbg];
+  int rc = dbg_record(c, walker, b, 0, rec);
+  if (rc) return rc;
+  Beamset& s = *c->sets[b];
+  const size_t n = static_cast<size_t>(s.dev.ninterp);
+  hipError_t e;
+  if (flux_out) {
+    e = hipMemcpy(flux_out, s.dev.xnode, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(xnode)");
+  }
+  if (log2r_out) {
+    e = hipMemcpy(log2r_out, s.wk.ynode + walker * s.wk.ynode_stride,
+                  n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(ynode)");
+  }
+  return POFD_OK;
+}

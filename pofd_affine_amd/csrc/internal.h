@@ -1,0 +1,119 @@
+// Internal declarations shared by the host code and the HIP kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/pofd_affine_amd.h"
+
+namespace pofd {
+
+// beam after beam::readFile (+ makeHistogram), include/beam.h:28-64
+struct Beam {
+  double pixsize = 0.0, minval = 0.0;
+  std::vector<double> pos, neg, posinv, neginv;  // sorted |pixel| values
+  double totpos = 0.0, totpossq = 0.0, totneg = 0.0, totnegsq = 0.0;
+  uint32_t nbins = 0;
+  bool is_pos_hist = false, is_neg_hist = false;
+  std::vector<double> posw, posv, negw, negv;    // hist weights, inverse vals
+  int set(const double* pix, size_t n, double pixsize, double minval,
+          std::string& err);
+  int make_histogram(uint32_t nbins, std::string& err);
+};
+
+void data_minmax(const double* d, size_t n, double& mn, double& mx);
+
+// Limits of the fused 1D kernel.
+constexpr int kThreads = 1024;   // threads per workgroup (one walker)
+constexpr int kM1 = 256;         // column transform length
+constexpr int kChunk = 16;       // columns per LDS chunk
+constexpr int kRowsPerPass = 8;  // spectral rows accumulated per forward pass
+constexpr int kMaxKnots = 64;
+constexpr int kBlk = 32;         // P(D) values per min/sum block
+constexpr int kWarm = 64;        // warm-up steps of the truncated recurrences
+constexpr int kDbg = 24;         // debug scalars per (walker, field)
+
+// One field (dataset) of a beam set, device view.
+struct FieldDev {
+  double sigma_base;
+  double like_offset;
+  double like_norm;
+  int binned;
+  const uint32_t* bins;   // nbins
+  uint32_t nbins;
+  double bincent0, bindelta;
+  const double* pix;      // npix (sorted ascending), unbinned only
+  uint64_t npix;
+};
+
+constexpr int kMaxFields = 16;
+
+// Everything the fused kernel needs for one beam set; theta-independent.
+struct BeamsetDev {
+  int N;        // nflux
+  int M;        // N/2
+  int M2;       // M / 256 (number of columns)
+  int ninterp;
+  int nknots;
+  int nbeam;    // beam entries (histogram bins or raw pixels), positive
+  int minit, maxit;       // fill range of R indices (PDFactory.cc:569-574)
+  int wrapRidx;
+  double dflux;
+  double minflux_R;
+  double logfac;          // log2(10.0) as glibc computes it
+  double prefac;          // (pixsize/3600)^2
+  double maxsigma_base;
+  double iflux;           // two_pi / (N * dflux)
+  double spectral_cut;    // drop components with sigma^2 w^2/2 > cut
+  double logknots[kMaxKnots];
+  const double* xnode;    // ninterp interpolation fluxes
+  const double* ginv;     // ninterp-1: 1/h_i  (GSL cspline_init g_i)
+  const double* lalpha;   // ninterp-2: LDL^T diagonal of the node system
+  const double* lgamma;   // ninterp-2: LDL^T sub-diagonal multipliers
+  const int8_t* rk;       // ninterp*nbeam: knot interval or -1 (skip)
+  const double* rdelx;    // ninterp*nbeam: log2(S) - logknots[k]
+  const double* bw;       // nbeam weights (1.0 for a raw beam)
+  const double* bieta;    // nbeam inverse beam values
+  const uint16_t* kst;    // N/8: interval of the first flux of each 8-run
+  const double2* W256;    // 256: exp(-2 pi i s/256)
+  const double2* Wlo;     // 256: exp(-2 pi i s/M)
+  const double2* Wm2;     // M2: exp(-2 pi i s/M2)
+  int nfields;
+  FieldDev fields[kMaxFields];
+};
+
+// Per-walker device workspace layout (doubles), sized by the host.
+struct WorkDev {
+  double* coef;      // W * (ninterp-1) * 4 : y, b, c, d per interval
+  double* ynode;     // W * ninterp        : log2 R at the nodes
+  double2* zrows;    // W * 2 * M2 * 256   : forward rows, then noisy rows
+  double2* prtrans;  // W * (M+1)          : exp(R^ - R^0) with phase
+  double2* xp;       // W * (M+1)          : prtrans * noise factor
+  double* pofd;      // W * nfields * N    : clamped, unrotated P(D)
+  double* bmin;      // W * nfields * N/32
+  double* bsum;      // W * nfields * N/32
+  double* dbg;       // W * nfields * kDbg : debug scalars
+  size_t coef_stride, ynode_stride, zrows_stride, spec_stride, pofd_stride,
+      blk_stride;
+};
+
+}  // namespace pofd
+
+struct pofd_beam {
+  pofd::Beam beam;
+};
+
+// launcher implemented in pofd1d_kernel.hip
+namespace pofd {
+int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
+                     const float* d_theta, uint32_t nwalk, uint32_t stride,
+                     double* d_loglike, int32_t* d_status, int accumulate,
+                     void* stream);
+int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
+                          int is_log, int already_log, double* d_out,
+                          void* stream);
+}  // namespace pofd

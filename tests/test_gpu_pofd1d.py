@@ -1,0 +1,145 @@
+"""GPU parity of the fused 1D P(D) likelihood against the CPU oracle.
+
+Every call goes through the C ABI (include/pofd_affine_amd.h).  Tolerances
+are the north-star ones: log likelihood within 1e-6 absolute, P(D) within
+1e-10 of its peak; intermediate stages are held tighter where the
+arithmetic allows it.
+"""
+import numpy as np
+import pytest
+
+from pofd_affine_amd import synth, workload
+from oracle_side import build_oracle
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(npix_side=200, seed=7)
+NF_S, NI_S = 32768, 512
+
+
+def _say(msg):
+    print("[gpu-case] " + msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def small():
+    inp = workload.make_inputs(**SMALL)
+    th = synth.walker_thetas(8, seed=11)
+    out = {}
+    for binned in (True, False):
+        _say("build small binned=%s" % binned)
+        ctx = workload.build_context(inp, NF_S, NI_S, binned=binned)
+        L = build_oracle(inp, NF_S, NI_S, binned, th[0])
+        ll, st = ctx.loglike_batch(th)
+        out[binned] = (ctx, L, ll, st)
+    return inp, th, out
+
+
+@pytest.mark.parametrize("binned", [True, False])
+def test_small_loglike(small, binned):
+    inp, th, out = small
+    ctx, L, ll, st = out[binned]
+    _say("small loglike binned=%s" % binned)
+    ref, rst = L.loglike_batch(th, 8)
+    assert np.all(rst == 0) and np.all(st == 0), (st, rst)
+    np.testing.assert_allclose(ll, ref, rtol=0, atol=1e-6)
+
+
+def test_small_setup_matches(small):
+    inp, th, out = small
+    ctx, L, _, _ = out[True]
+    gi = ctx.beamset_info(0)
+    oi = L.beamset_info(0)
+    for k in ("minRFlux", "maxRFlux", "maxsigma_base", "minDataFlux",
+              "maxDataFlux"):
+        assert gi[k] == oi[k], k
+
+
+@pytest.mark.parametrize("w", [0, 5])
+def test_small_rinterp_and_spectrum(small, w):
+    inp, th, out = small
+    ctx, L, _, _ = out[True]
+    _say("small rinterp/spectrum walker %d" % w)
+    ip = L.initpd(0, th[w])
+    x, y = ctx.get_rinterp(w)
+    np.testing.assert_array_equal(x, ip["rinterp_flux"])
+    np.testing.assert_allclose(y, ip["rinterp_vals"], rtol=0, atol=1e-12)
+    dbg = ctx.get_debug(w)
+    assert dbg["mn"] == pytest.approx(ip["mn"], rel=1e-13)
+    assert dbg["var_noi"] == pytest.approx(ip["var_noi"], rel=1e-13)
+    assert dbg["r0"] == pytest.approx(ip["r0"], rel=1e-13)
+    spec = ctx.get_spectrum(w)
+    kmax = int(dbg["kmax_beamset"])
+    assert spec.size == kmax + 1
+    np.testing.assert_allclose(spec, ip["prtrans"][:kmax + 1], rtol=0,
+                               atol=1e-12)
+
+
+@pytest.mark.parametrize("w", [0, 3, 7])
+def test_small_pd(small, w):
+    inp, th, out = small
+    ctx, L, _, _ = out[True]
+    _say("small pd walker %d" % w)
+    L.initpd(0, th[w])
+    ref = L.getpd(0, 0, float(th[w][8]), setlog=False)
+    raw = ctx.get_pd(w, is_log="raw")[0]
+    peak = ref["peak"]
+    assert np.max(np.abs(raw - ref["pofd_raw"])) <= 1e-10 * peak
+    dbg = ctx.get_debug(w)
+    assert dbg["peak"] == pytest.approx(peak, rel=1e-12)
+    pdl, mf, df = ctx.get_pd(w, is_log=False)
+    assert df == ref["dflux"]
+    # compare by flux value: both arrays sit on the same grid, offset by
+    # the difference of the (continuous) mean estimates
+    assert abs(mf - ref["minflux"]) < 1e-9 * df + abs(
+        dbg["split"] - ref["split"]) * df * 1.0001
+    if int(dbg["split"]) == ref["split"]:
+        np.testing.assert_allclose(pdl, ref["pd"], rtol=0,
+                                   atol=1e-10 * ref["pd"].max())
+
+
+def test_status_codes(small):
+    inp, th, out = small
+    ctx = out[True][0]
+    _say("status codes")
+    bad = th[:3].copy()
+    bad[0, 2] = np.nan     # invalid model
+    bad[1, 8] = -1.0       # sigma multiplier < 0 -> bad_like
+    ll, st = ctx.loglike_batch(bad)
+    assert st[0] == 2 and np.isnan(ll[0])
+    assert st[1] == 4 and ll[1] == 1e25
+    assert st[2] == 0 and np.isfinite(ll[2])
+
+
+def test_spectral_cut_exact_mode(small):
+    inp, th, out = small
+    ctx, _, ll, _ = out[True]
+    _say("spectral cut 746 vs default")
+    ctx2 = workload.build_context(inp, NF_S, NI_S, binned=True,
+                                  spectral_cut=746.0)
+    ll2, st2 = ctx2.loglike_batch(th)
+    assert np.all(st2 == 0)
+    np.testing.assert_allclose(ll2, ll, rtol=0, atol=1e-9)
+    assert ctx2.get_debug(0)["kmax"] > ctx.get_debug(0)["kmax"]
+
+
+def test_deterministic(small):
+    inp, th, out = small
+    ctx, _, ll, _ = out[False]
+    _say("determinism")
+    ll2, _ = ctx.loglike_batch(th)
+    assert np.array_equal(ll, ll2)
+
+
+@pytest.mark.parametrize("binned", [True, False])
+def test_config1d_full_size(binned):
+    """configs[1] shapes: 1000x1000 map, nflux 2^17, ninterp 2048."""
+    inp = workload.make_inputs(npix_side=1000, seed=42)
+    th = synth.walker_thetas(16, seed=1234)
+    _say("full size binned=%s" % binned)
+    ctx = workload.build_context(inp, 131072, 2048, binned=binned)
+    ll, st = ctx.loglike_batch(th)
+    L = build_oracle(inp, 131072, 2048, binned, th[0])
+    ref, rst = L.loglike_batch(th, 8)
+    assert np.all(st == 0) and np.all(rst == 0), (st, rst)
+    np.testing.assert_allclose(ll, ref, rtol=0, atol=1e-6)
