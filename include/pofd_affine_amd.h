@@ -80,6 +80,19 @@ int pofd_beam_get(const pofd_beam *bm, int which, double *out, uint32_t *n);
    area_pix, area_sq, max positive pixel */
 int pofd_beam_info(const pofd_beam *bm, double *info);
 
+/* ---- FITS ingestion: the read side of beam::readFile and
+        fitsData::readData (src/beam.cc:295-380, src/fitsData.cc:56-315).
+        IMAGE HDUs with BITPIX 8/16/32/64/-32/-64 and BZERO/BSCALE; "file[k]"
+        selects HDU k; maps use the image/IMAGE extension (else the primary)
+        and drop pixels whose mask/MASK value is non-zero.  Call with a NULL
+        buffer to query the size.  pofd_fits_last_error() describes the last
+        failure of the calling thread. */
+int pofd_fits_read_beam(const char *path, double *pix, uint64_t capacity,
+                        uint64_t *npix, double *pixsize_arcsec);
+int pofd_fits_read_map(const char *path, int ignore_mask, double *data,
+                       uint64_t capacity, uint64_t *n);
+const char *pofd_fits_last_error(void);
+
 /* ---- data: replaces fitsData::applyBinning and fitsData::meanSubtract,
         src/fitsData.cc:393-437.  Host code, bit-exact. ---- */
 int pofd_data_bin(const double *data, uint64_t n, uint32_t nbins,
@@ -138,11 +151,14 @@ int pofd1d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
 int pofd1d_get_pd(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                   int field_id, int is_log, double *pd_out, double *minflux,
                   double *dflux);
-/* info[24]: split index, peak, minidx, minval, jitter, integral (before
+/* info[32]: split index, peak, minidx, minval, jitter, integral (before
    normalisation), mean shift (-minflux), sgpos, sgneg, mn, var_noi, r0,
    kmax (last spectral index kept), nrows, field loglike (before offset),
    status, 1/integral, minflux, P(D) stored as log2, kmax of the beam set,
-   sum P, sum j P, sum of P above the split, sigma */
+   sum P, sum j P, sum of P above the split, sigma, then wall-clock ticks
+   (100 MHz) of the kernel phases: setup+R nodes, node spline, forward
+   transform, spectrum, inverse transform, split search, normalise+logL,
+   total */
 int pofd1d_get_debug(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                      int field_id, double *info);
 /* prtrans after initPD for k = 0..kmax (interleaved re, im), kmax from

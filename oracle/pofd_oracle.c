@@ -678,10 +678,15 @@ typedef struct orc_fft {
   double *rtw;    /* m complex twiddles e^{-2 pi i k/n} */
   unsigned *rev;  /* bit reversal of [0, m) */
   double *work;   /* 2*m */
+  /* extended-precision variant (precise mode) */
+  long double *twl, *rtwl, *workl;
 } orc_fft;
 
 static void orc_fft_free(orc_fft *f) {
   if (!f) return;
+  free(f->twl);
+  free(f->rtwl);
+  free(f->workl);
   free(f->tw);
   free(f->rtw);
   free(f->rev);
@@ -817,6 +822,108 @@ int orc_irfft(unsigned n, const double *in, double *out) {
   orc_irfft_plan(f, in, out);
   orc_fft_free(f);
   return ORC_OK;
+}
+
+/* Extended-precision (x87 long double, 64-bit significand) versions of the
+   same transforms.  They are not the reference's arithmetic: they give the
+   exact-math value of the reference's double-precision pipeline to ~1e-19
+   of the peak, so that the intrinsic rounding noise of any double FFT (the
+   reference's FFTW included) can be measured and excluded from parity. */
+static void orc_fft_prepare_l(orc_fft *f) {
+  if (f->twl) return;
+  unsigned m = f->m, n = f->n;
+  f->twl = (long double *)malloc(sizeof(long double) * m);
+  f->rtwl = (long double *)malloc(sizeof(long double) * 2 * m);
+  f->workl = (long double *)malloc(sizeof(long double) * 2 * m);
+  for (unsigned k = 0; k < m / 2; ++k) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L *
+                    (long double)k / (long double)m;
+    f->twl[2 * k] = cosl(a);
+    f->twl[2 * k + 1] = sinl(a);
+  }
+  for (unsigned k = 0; k < m; ++k) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L *
+                    (long double)k / (long double)n;
+    f->rtwl[2 * k] = cosl(a);
+    f->rtwl[2 * k + 1] = sinl(a);
+  }
+}
+
+static void orc_cfft_l(const orc_fft *f, long double *a, int sign) {
+  unsigned m = f->m;
+  for (unsigned i = 0; i < m; ++i) {
+    unsigned r = f->rev[i];
+    if (r > i) {
+      long double t0 = a[2 * i], t1 = a[2 * i + 1];
+      a[2 * i] = a[2 * r];
+      a[2 * i + 1] = a[2 * r + 1];
+      a[2 * r] = t0;
+      a[2 * r + 1] = t1;
+    }
+  }
+  for (unsigned len = 2; len <= m; len <<= 1) {
+    unsigned half = len >> 1, step = m / len;
+    for (unsigned s = 0; s < m; s += len) {
+      for (unsigned j = 0; j < half; ++j) {
+        long double wr = f->twl[2 * j * step];
+        long double wi = sign < 0 ? f->twl[2 * j * step + 1] : -f->twl[2 * j * step + 1];
+        long double *u = a + 2 * (s + j), *v = a + 2 * (s + j + half);
+        long double tr = v[0] * wr - v[1] * wi;
+        long double ti = v[0] * wi + v[1] * wr;
+        v[0] = u[0] - tr;
+        v[1] = u[1] - ti;
+        u[0] = u[0] + tr;
+        u[1] = u[1] + ti;
+      }
+    }
+  }
+}
+
+static void orc_rfft_l(orc_fft *f, const double *in, long double *out) {
+  orc_fft_prepare_l(f);
+  unsigned m = f->m;
+  long double *z = f->workl;
+  for (unsigned i = 0; i < 2 * m; ++i) z[i] = in[i];
+  orc_cfft_l(f, z, -1);
+  for (unsigned k = 0; k <= m; ++k) {
+    unsigned kk = (k == m) ? 0 : k, mk = (k == 0) ? 0 : m - k;
+    long double zr = z[2 * kk], zi = z[2 * kk + 1];
+    long double cr = z[2 * mk], ci = -z[2 * mk + 1];
+    long double ar = 0.5L * (zr + cr), ai = 0.5L * (zi + ci);
+    long double br = 0.5L * (zi - ci), bi = -0.5L * (zr - cr);
+    long double wr, wi;
+    if (k == m) {
+      wr = -1.0L;
+      wi = 0.0L;
+    } else {
+      wr = f->rtwl[2 * k];
+      wi = f->rtwl[2 * k + 1];
+    }
+    out[2 * k] = ar + (br * wr - bi * wi);
+    out[2 * k + 1] = ai + (br * wi + bi * wr);
+  }
+}
+
+static void orc_irfft_l(orc_fft *f, const long double *in, double *out) {
+  orc_fft_prepare_l(f);
+  unsigned m = f->m;
+  long double *z = f->workl;
+  for (unsigned k = 0; k < m; ++k) {
+    long double xr = in[2 * k], xi = in[2 * k + 1];
+    long double yr = in[2 * (m - k)], yi = in[2 * (m - k) + 1];
+    if (k == 0) {
+      xi = 0.0L;
+      yi = 0.0L;
+    }
+    long double er = xr + yr, ei = xi - yi;
+    long double dr = xr - yr, di = xi + yi;
+    long double wr = f->rtwl[2 * k], wi = -f->rtwl[2 * k + 1];
+    long double or_ = dr * wr - di * wi, oi = dr * wi + di * wr;
+    z[2 * k] = er - oi;
+    z[2 * k + 1] = ei + or_;
+  }
+  orc_cfft_l(f, z, +1);
+  for (unsigned i = 0; i < 2 * m; ++i) out[i] = (double)z[i];
 }
 
 /* ===================================================================== */
@@ -1017,6 +1124,8 @@ typedef struct orc_pdfac {
   double *RinterpFlux_neg, *RinterpVals_neg, *c_neg;
   int has_interp_pos, has_interp_neg;
   orc_fft *fft;
+  int precise;
+  long double *lprtrans, *lpval;
   /* last PD */
   double *pd;
   unsigned pd_n;
@@ -1026,8 +1135,13 @@ typedef struct orc_pdfac {
   double last_peak, last_integral;
 } orc_pdfac;
 
-static orc_pdfac *orc_pdfac_new(unsigned n, unsigned ninterp) {
+static orc_pdfac *orc_pdfac_new(unsigned n, unsigned ninterp, int precise) {
   orc_pdfac *p = (orc_pdfac *)calloc(1, sizeof(orc_pdfac));
+  p->precise = precise;
+  if (precise) {
+    p->lprtrans = (long double *)malloc(sizeof(long double) * 2 * (n / 2 + 1));
+    p->lpval = (long double *)malloc(sizeof(long double) * 2 * (n / 2 + 1));
+  }
   p->currsize = n;
   p->ninterp = ninterp;
   p->rvals = (double *)malloc(sizeof(double) * n);
@@ -1048,6 +1162,8 @@ static orc_pdfac *orc_pdfac_new(unsigned n, unsigned ninterp) {
 
 static void orc_pdfac_free(orc_pdfac *p) {
   if (!p) return;
+  free(p->lprtrans);
+  free(p->lpval);
   free(p->rvals);
   free(p->pofd);
   free(p->pd);
@@ -1279,6 +1395,32 @@ static int orc_initpd(orc_pdfac *p, unsigned n, double minflux,
   p->sgpos = sqrt(p->var_noi);
   p->doshift = (p->sgpos > p->dflux) && (fabs(p->mn) > p->dflux);
   p->shift = p->doshift ? -p->mn : 0.0;
+  if (p->precise) {
+    /* extended-precision transform and exponentiation; prtrans keeps the
+       rounded values for inspection */
+    long double *lp = p->lprtrans;
+    orc_rfft_l(p->fft, p->rvals, lp);
+    long double iflux_l = (long double)orc_two_pi / ((long double)n * p->dflux);
+    long double r0l = lp[0];
+    p->r0 = (double)r0l;
+    if (p->precise == 3) {
+      /* analysis mode: exact transform, but X_k and R^_0 rounded to double
+         before the exponent difference is formed (as any double code does) */
+      for (unsigned i = 0; i < 2 * (n / 2 + 1); ++i) lp[i] = (long double)(double)lp[i];
+      r0l = (long double)(double)r0l;
+    }
+    lp[0] = 1.0L;
+    lp[1] = 0.0L;
+    long double wfac = (long double)p->shift * iflux_l;
+    for (unsigned idx = 1; idx < n / 2 + 1; ++idx) {
+      long double expfac = expl(lp[2 * idx] - r0l);
+      long double ival = lp[2 * idx + 1] - (p->doshift ? wfac * (long double)idx : 0.0L);
+      lp[2 * idx] = expfac * cosl(ival);
+      lp[2 * idx + 1] = expfac * sinl(ival);
+    }
+    for (unsigned i = 0; i < 2 * (n / 2 + 1); ++i) p->prtrans[i] = (double)lp[i];
+    return ORC_OK;
+  }
   orc_rfft_plan(p->fft, p->rvals, p->prtrans);
   double *pr = p->prtrans;
   double iflux = orc_two_pi / (n * p->dflux);
@@ -1462,7 +1604,33 @@ static int orc_unwrap_normalize(orc_pdfac *p, orc_err *err) {
 static int orc_getpd(orc_pdfac *p, double sigma, int setlog, orc_err *err) {
   unsigned n = p->currsize;
   unsigned ncplx = n / 2 + 1;
-  if (sigma > 0) {
+  if (p->precise == 2) {
+    /* analysis mode: extended forward pipeline, double inverse transform */
+    double iflux = orc_two_pi / (n * p->dflux);
+    double sigfac = -0.5 * sigma * sigma;
+    for (unsigned idx = 0; idx < ncplx; ++idx) {
+      double w = iflux * (double)idx;
+      double ef = (sigma > 0 && idx > 0) ? exp(sigfac * w * w) : 1.0;
+      p->pval[2 * idx] = ef * p->prtrans[2 * idx];
+      p->pval[2 * idx + 1] = ef * p->prtrans[2 * idx + 1];
+    }
+    orc_irfft_plan(p->fft, p->pval, p->pofd);
+  } else if (p->precise) {
+    long double *lv = p->lpval, *lp = p->lprtrans;
+    long double iflux = (long double)orc_two_pi / ((long double)n * p->dflux);
+    long double sigfac = -0.5L * (long double)sigma * (long double)sigma;
+    for (unsigned idx = 0; idx < ncplx; ++idx) {
+      long double w = iflux * (long double)idx;
+      long double ef = (sigma > 0 && idx > 0) ? expl(sigfac * w * w) : 1.0L;
+      lv[2 * idx] = ef * lp[2 * idx];
+      lv[2 * idx + 1] = ef * lp[2 * idx + 1];
+      if (p->precise == 4) { /* analysis mode: spectrum rounded to double */
+        lv[2 * idx] = (long double)(double)lv[2 * idx];
+        lv[2 * idx + 1] = (long double)(double)lv[2 * idx + 1];
+      }
+    }
+    orc_irfft_l(p->fft, lv, p->pofd);
+  } else if (sigma > 0) {
     double iflux = orc_two_pi / (n * p->dflux);
     double sigfac = -0.5 * sigma * sigma;
     p->pval[0] = 1.0;
@@ -1526,6 +1694,7 @@ struct orc_like {
   orc_err err;
   orc_pdfac *fac; /* single-threaded path workspace */
   int setup_done;
+  int precise;
 };
 
 orc_like *orc_like_new(unsigned fftsize, unsigned ninterp) {
@@ -1559,6 +1728,12 @@ void orc_like_free(orc_like *L) {
 }
 
 const orc_err *orc_like_error(const orc_like *L) { return &L->err; }
+
+void orc_like_set_precise(orc_like *L, int precise) {
+  L->precise = precise;
+  orc_pdfac_free(L->fac);
+  L->fac = NULL;
+}
 
 int orc_like_set_knot_positions(orc_like *L, unsigned n, const double *pos) {
   return orc_knots_set_positions(&L->model, n, pos, &L->err);
@@ -1732,7 +1907,7 @@ static int orc_loglike_ws(const orc_like *L, orc_knots *model, orc_pdfac *p,
 }
 
 static orc_pdfac *orc_like_fac(orc_like *L) {
-  if (!L->fac) L->fac = orc_pdfac_new(L->fftsize, L->ninterp);
+  if (!L->fac) L->fac = orc_pdfac_new(L->fftsize, L->ninterp, L->precise);
   return L->fac;
 }
 
@@ -1756,7 +1931,7 @@ typedef struct orc_job {
 
 static void *orc_worker(void *arg) {
   orc_job *j = (orc_job *)arg;
-  orc_pdfac *p = orc_pdfac_new(j->L->fftsize, j->L->ninterp);
+  orc_pdfac *p = orc_pdfac_new(j->L->fftsize, j->L->ninterp, j->L->precise);
   orc_knots model = j->L->model;
   orc_err err;
   for (unsigned w = j->tid; w < j->nwalk; w += j->nthreads) {

@@ -153,6 +153,10 @@ typedef struct orc_like orc_like;
 orc_like *orc_like_new(unsigned fftsize, unsigned ninterp);
 void orc_like_free(orc_like *L);
 const orc_err *orc_like_error(const orc_like *L);
+/* precise = 1: transforms and exponentials in x87 extended precision (the
+   exact-math value of the reference's pipeline); 0: plain double as the
+   reference computes it (default). */
+void orc_like_set_precise(orc_like *L, int precise);
 int orc_like_set_knot_positions(orc_like *L, unsigned n, const double *pos);
 /* adds a beam set (takes a copy of the beam); returns its index */
 int orc_like_add_beamset(orc_like *L, const orc_beam *bm);

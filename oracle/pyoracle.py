@@ -85,6 +85,7 @@ def lib():
         "orc_like_new": (C.c_void_p, [C.c_uint, C.c_uint]),
         "orc_like_free": (None, [C.c_void_p]),
         "orc_like_error": (C.POINTER(OrcErr), [C.c_void_p]),
+        "orc_like_set_precise": (None, [C.c_void_p, C.c_int]),
         "orc_like_set_knot_positions": (C.c_int, [C.c_void_p, C.c_uint, DP]),
         "orc_like_add_beamset": (C.c_int, [C.c_void_p, C.c_void_p]),
         "orc_like_add_field": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_size_t,
@@ -345,6 +346,12 @@ class Like:
         if rc:
             e = lib().orc_like_error(self.ptr).contents
             raise OracleError(rc, e.text())
+
+    def set_precise(self, precise=True):
+        """1 (True): extended-precision transforms; 0: double (reference
+        arithmetic); 2 and 3: analysis modes (extended forward + double
+        inverse; extended transforms with X_k, R^_0 rounded to double)."""
+        lib().orc_like_set_precise(self.ptr, int(precise))
 
     def add_beamset(self, beam):
         return lib().orc_like_add_beamset(self.ptr, beam.ptr)

@@ -277,10 +277,12 @@ class Context:
     DEBUG_KEYS = ["split", "peak", "minidx", "minval", "jitter", "integral",
                   "tmn", "sgpos", "sgneg", "mn", "var_noi", "r0", "kmax",
                   "nrows", "field_loglike", "status", "itot", "minflux",
-                  "pofd_is_log", "kmax_beamset", "S", "SJ", "Shigh", "sigma"]
+                  "pofd_is_log", "kmax_beamset", "S", "SJ", "Shigh", "sigma",
+                  "t_setup_rinterp", "t_spline", "t_forward", "t_spectral",
+                  "t_inverse", "t_split", "t_loglike", "t_total"]
 
     def get_debug(self, walker, b=0, f=0):
-        out = np.zeros(24)
+        out = np.zeros(32)
         self._check(lib().pofd1d_get_debug(self.ptr, walker, b, f, _dp(out)),
                     "pofd1d_get_debug")
         return dict(zip(self.DEBUG_KEYS, out))

@@ -35,7 +35,7 @@ constexpr int kRowsPerPass = 8;  // spectral rows accumulated per forward pass
 constexpr int kMaxKnots = 64;
 constexpr int kBlk = 32;         // P(D) values per min/sum block
 constexpr int kWarm = 64;        // warm-up steps of the truncated recurrences
-constexpr int kDbg = 24;         // debug scalars per (walker, field)
+constexpr int kDbg = 32;         // debug scalars per (walker, field)
 
 // One field (dataset) of a beam set, device view.
 struct FieldDev {

@@ -66,6 +66,7 @@ struct Smem {
   int rowof[512];               // k2 -> row position, -1 if absent
   double sc[32];                // broadcast scalars
   int isc[32];
+  unsigned long long tstamp[8]; // wall-clock phase stamps (thread 0)
 };
 
 enum {  // sc[] slots
@@ -240,6 +241,7 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
   double2* xp = wk.xp + walker * wk.spec_stride;
 
   // ------------------------------------------------------------------ setup
+  if (tid == 0) sm.tstamp[0] = wall_clock64();
   if (tid == 0) {
     int st = accumulate ? status[walker] : POFD_OK;
     sm.isc[I_STATUS] = st;
@@ -361,6 +363,7 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
     }
   }
   __syncthreads();
+  if (tid == 0) sm.tstamp[1] = wall_clock64();
 
   // ---- natural spline over the nodes: the tridiagonal matrix is fixed
   // (theta-independent LDL^T factors from the host); forward and backward
@@ -420,6 +423,7 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
 
   // ---- spectral rows needed by the widest field of this beam set ---------
   if (tid == 0) {
+    sm.tstamp[2] = wall_clock64();
     int kmaxb = 0;
     for (int f = 0; f < bs.nfields; ++f) {
       const int kf = spectral_kmax(sigmult * bs.fields[f].sigma_base, bs.iflux,
@@ -521,6 +525,7 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
       if (q == 0 && r < np) zrows[(p0 + r) * kM1 + row4] = v;
     }
   }
+  if (tid == 0) sm.tstamp[3] = wall_clock64();
   {
     const double mn = block_sum(mn_part, sm.red);
     const double var = block_sum(var_part, sm.red);
@@ -572,6 +577,7 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
   __syncthreads();
 
   // ------------------------------------------------------- datasets (fields)
+  if (tid == 0) sm.tstamp[4] = wall_clock64();
   double ll_total = 0.0;
   int st_out = POFD_OK;
   for (int f = 0; f < bs.nfields; ++f) {
@@ -734,6 +740,7 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
     }
     // make this workgroup's P(D) stores visible to its own later loads
     __threadfence_block();
+    if (tid == 0) sm.tstamp[5] = wall_clock64();
     {
       const double peak = block_max(pk, sm.red);
       block_argmin(mnv, mni, sm.red);
@@ -876,6 +883,7 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
     }
 
     // ---- rotate, normalise, mean (unwrapAndNormalizePD) -----------------
+    if (tid == 0) sm.tstamp[6] = wall_clock64();
     int st = sm.isc[I_STATUS];
     if (st == POFD_OK) {
       const int split = sm.isc[I_SPLIT];
@@ -1005,6 +1013,10 @@ __global__ __launch_bounds__(kThreads) void loglike1d_kernel(
       dbg[21] = sm.sc[S_SUMJ];
       dbg[22] = sm.sc[S_SHIGH];
       dbg[23] = sigma;
+      sm.tstamp[7] = wall_clock64();
+      for (int i = 0; i < 7; ++i)
+        dbg[24 + i] = static_cast<double>(sm.tstamp[i + 1] - sm.tstamp[i]);
+      dbg[31] = static_cast<double>(sm.tstamp[7] - sm.tstamp[0]);
       if (stf == POFD_OK) {
         ll_total += (sm.sc[S_LL] - fd.like_offset) / fd.like_norm;
       } else {

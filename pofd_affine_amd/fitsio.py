@@ -137,3 +137,60 @@ def read_map(path, ignore_mask=False):
                 raise ValueError("All data masked")
             data = data[keep]
     return data
+
+
+def _card(key, value, comment=""):
+    if isinstance(value, bool):
+        v = "%20s" % ("T" if value else "F")
+    elif isinstance(value, int):
+        v = "%20d" % value
+    elif isinstance(value, float):
+        v = "%20.17G" % value
+    else:
+        v = "'%-8s'" % str(value).replace("'", "''")
+    c = "%-8s= %s" % (key, v)
+    if comment:
+        c += " / " + comment
+    return c[:80].ljust(80)
+
+
+def write_image(path, image, extra=None, extname=None, mask=None):
+    """Write a 2-D float64 image (and optionally a MASK extension) to FITS.
+
+    extra: dict of additional header keys for the image HDU (e.g. PIXSIZE).
+    With extname the image goes into an IMAGE extension of that name after
+    an empty primary HDU."""
+    img = np.asarray(image, dtype=">f8")
+    assert img.ndim == 2
+
+    def hdu(cards, data):
+        head = "".join(cards + ["END".ljust(80)])
+        head = head.ljust((len(head) + _BLOCK - 1) // _BLOCK * _BLOCK)
+        raw = b"" if data is None else data.tobytes()
+        raw += b"\0" * ((-len(raw)) % _BLOCK)
+        return head.encode("ascii") + raw
+
+    def img_cards(first, a, bitpix, name=None, keys=None):
+        c = [_card("SIMPLE", True)] if first else [_card("XTENSION", "IMAGE")]
+        c += [_card("BITPIX", bitpix), _card("NAXIS", 2),
+              _card("NAXIS1", a.shape[1]), _card("NAXIS2", a.shape[0])]
+        if not first:
+            c += [_card("PCOUNT", 0), _card("GCOUNT", 1)]
+        if name:
+            c.append(_card("EXTNAME", name))
+        for k, v in (keys or {}).items():
+            c.append(_card(k, v))
+        return c
+
+    out = b""
+    if extname is None and mask is None:
+        out += hdu(img_cards(True, img, -64, keys=extra), img)
+    else:
+        out += hdu([_card("SIMPLE", True), _card("BITPIX", 8),
+                    _card("NAXIS", 0), _card("EXTEND", True)], None)
+        out += hdu(img_cards(False, img, -64, extname or "image", extra), img)
+        if mask is not None:
+            m = np.asarray(mask, dtype=">i2")
+            out += hdu(img_cards(False, m, 16, "mask"), m)
+    with open(path, "wb") as fh:
+        fh.write(out)

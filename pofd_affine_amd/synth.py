@@ -81,10 +81,10 @@ def _n_between(a, b):
             + seg(max(a, SBREAK), b, nb, SBREAK, GAMMA2))
 
 
-def _draw_fluxes(rng, n):
-    """Inverse-CDF draw of n fluxes in [SMIN_SRC, SMAX_SRC]."""
+def _draw_fluxes(rng, n, smax=SMAX_SRC):
+    """Inverse-CDF draw of n fluxes in [SMIN_SRC, smax]."""
     n_lo = _n_between(SMIN_SRC, SBREAK)
-    n_tot = _n_between(SMIN_SRC, SMAX_SRC)
+    n_tot = _n_between(SMIN_SRC, smax)
     u = rng.random(n) * n_tot
     out = np.empty(n)
     lo = u < n_lo
@@ -101,14 +101,18 @@ def _draw_fluxes(rng, n):
 
 
 def make_map(npix_side=1000, sigma_noise=0.0025, seed=42, beam=None,
-             pix_arcsec=6.0):
-    """Synthetic confused map (flattened float64), mean subtracted."""
+             pix_arcsec=6.0, smax=SMAX_SRC):
+    """Synthetic confused map (flattened float64), mean subtracted.
+
+    smax: brightest source drawn (Jy); the measured configurations use
+    600 mJy, above the last knot (450 mJy), which puts the brightest pixels
+    in the far tail of the model P(D)."""
     rng = np.random.default_rng(seed)
     if beam is None:
         beam, pix_arcsec = gaussian_beam(pix_arcsec=pix_arcsec)
     area_deg2 = (npix_side * pix_arcsec / 3600.0) ** 2
-    nsrc = rng.poisson(area_deg2 * _n_between(SMIN_SRC, SMAX_SRC))
-    flux = _draw_fluxes(rng, nsrc)
+    nsrc = rng.poisson(area_deg2 * _n_between(SMIN_SRC, smax))
+    flux = _draw_fluxes(rng, nsrc, smax)
     pos = rng.integers(0, npix_side * npix_side, size=nsrc)
     img = np.bincount(pos, weights=flux,
                       minlength=npix_side * npix_side).reshape(

@@ -13,9 +13,9 @@ from ._lib import Beam, Context, data_bin
 
 
 def make_inputs(npix_side=1000, sigma=0.0025, seed=42, minbeamval=1e-6,
-                nbeamhist=120, nbins=1000):
+                nbeamhist=120, nbins=1000, smax=synth.SMAX_SRC):
     beam2d, pixsize = synth.gaussian_beam()
-    data, nsrc = synth.make_map(npix_side, sigma, seed=seed)
+    data, nsrc = synth.make_map(npix_side, sigma, seed=seed, smax=smax)
     return dict(beam=beam2d.ravel(), pixsize=pixsize, data=data, nsrc=nsrc,
                 sigma=sigma, minbeamval=minbeamval, nbeamhist=nbeamhist,
                 nbins=nbins, knots=synth.knot_positions())

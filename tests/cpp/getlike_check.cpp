@@ -1,0 +1,94 @@
+// Drives the reference-shaped C++ API (include/pofd_affine_amd.hpp) the way
+// pofd_affine_getLike does (src/pofd_affine_getLike.cc:16-106): read data
+// and beam files, set knots and R ranges, evaluate log likelihoods.
+//
+// usage: getlike_check fftsize ninterp binned nbins minbeamval nbeamhist
+//                      sigma datafile beamfile knotfile thetafile
+// knotfile: knot positions (Jy), one per line; thetafile: one parameter
+// set per line.  Prints "loglike <value>" per parameter set, then one line
+// "pd <minflux> <dflux> <n> <peak log2>" from PDFactory::getPD for the
+// first set.
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/pofd_affine_amd.hpp"
+
+static std::vector<std::vector<double>> read_rows(const char* path) {
+  std::ifstream in(path);
+  std::vector<std::vector<double>> rows;
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream is(line);
+    std::vector<double> r;
+    double v;
+    while (is >> v) r.push_back(v);
+    if (!r.empty()) rows.push_back(r);
+  }
+  return rows;
+}
+
+int main(int argc, char** argv) {
+  if (argc != 12) {
+    std::cerr << "bad usage" << std::endl;
+    return 2;
+  }
+  try {
+    const unsigned fftsize = std::stoul(argv[1]), ninterp = std::stoul(argv[2]);
+    const bool binned = std::stoi(argv[3]) != 0;
+    const unsigned nbins = std::stoul(argv[4]);
+    const double minbeamval = std::stod(argv[5]);
+    const unsigned nbeamhist = std::stoul(argv[6]);
+    const double sigma = std::stod(argv[7]);
+    std::vector<double> knots;
+    for (auto& r : read_rows(argv[10])) knots.push_back(r[0]);
+    auto thetas = read_rows(argv[11]);
+
+    calcLike likeSet(fftsize, ninterp, binned, nbins);
+    likeSet.readDataFromFiles({argv[8]}, {argv[9]}, {sigma}, {1.0}, false,
+                              false, minbeamval, nbeamhist > 0, nbeamhist,
+                              0.0);
+    likeSet.setKnotPositions(knots);
+    paramSet p0(thetas[0]);
+    likeSet.setRRanges(p0);
+    for (auto& t : thetas) {
+      paramSet p(t);
+      bool rej = false;
+      double ll = likeSet.getLogLike(p, rej);
+      std::printf("loglike %.17g %d\n", ll, rej ? 1 : 0);
+    }
+    // PDFactory::getPD on the same beam and model
+    beam bm(argv[9], nbeamhist > 0, nbeamhist, minbeamval);
+    numberCountsKnotsSpline model(knots);
+    model.setParams(p0);
+    dblpair rr = model.getRRange(bm);
+    PDFactory pfac(ninterp);
+    pfac.initPD(fftsize, 1.05 * rr.first, 1.05 * rr.second + 7.5 * sigma,
+                model, bm);
+    PD pd;
+    pfac.getPD(sigma, pd, true);
+    double peak = pd.pd_[0];
+    for (unsigned i = 1; i < pd.getDim(); ++i)
+      if (pd.pd_[i] > peak) peak = pd.pd_[i];
+    std::printf("pd %.17g %.17g %u %.17g\n", pd.minflux, pd.dflux,
+                pd.getDim(), peak);
+    // an error path: a NaN knot value must raise affineExcept
+    std::vector<double> bad(thetas[0]);
+    bad[1] = std::nan("");
+    try {
+      bool rej = false;
+      likeSet.getLogLike(paramSet(bad), rej);
+      std::printf("error none\n");
+    } catch (const affineExcept& ex) {
+      std::printf("error %s::%s\n", ex.getErrClass().c_str(),
+                  ex.getErrMethod().c_str());
+    }
+  } catch (const affineExcept& ex) {
+    std::cerr << ex << std::endl;
+    return 4;
+  }
+  return 0;
+}

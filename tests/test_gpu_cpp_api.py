@@ -1,0 +1,52 @@
+"""The reference-shaped C++ API (include/pofd_affine_amd.hpp) end to end:
+FITS beam and map files -> calcLike::readDataFromFiles -> setRRanges ->
+getLogLike, as pofd_affine_getLike drives it, checked against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from pofd_affine_amd import fitsio, synth, workload
+from oracle_side import build_oracle
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "_build", "getlike_check")
+
+
+def test_cpp_getlike(tmp_path):
+    print("[gpu-case] C++ calcLike on FITS inputs", flush=True)
+    assert os.path.exists(EXE), "build tests/cpp first (__graft_entry__.build)"
+    inp = workload.make_inputs(npix_side=300, seed=5)
+    side = 300
+    beam2d = inp["beam"].reshape(27, 27)
+    fitsio.write_image(str(tmp_path / "beam.fits"), beam2d,
+                       extra={"PIXSIZE": inp["pixsize"]})
+    fitsio.write_image(str(tmp_path / "map.fits"),
+                       inp["data"].reshape(side, side), extname="image")
+    knots = inp["knots"]
+    np.savetxt(tmp_path / "knots.txt", knots)
+    th = synth.walker_thetas(5, seed=9)[:, :9]
+    np.savetxt(tmp_path / "theta.txt", th.astype(np.float64), fmt="%.9g")
+    for binned in (1, 0):
+        out = subprocess.run(
+            [EXE, "32768", "512", str(binned), "1000", "1e-6", "120",
+             "0.0025", str(tmp_path / "map.fits"),
+             str(tmp_path / "beam.fits"), str(tmp_path / "knots.txt"),
+             str(tmp_path / "theta.txt")],
+            capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        lines = out.stdout.split("\n")
+        got = np.array([float(l.split()[1]) for l in lines
+                        if l.startswith("loglike")])
+        rej = [int(l.split()[2]) for l in lines if l.startswith("loglike")]
+        assert rej == [0] * len(got)
+        L = build_oracle(inp, 32768, 512, bool(binned), th[0])
+        ref, rst = L.loglike_batch(np.hstack([th, np.zeros((5, 2),
+                                                           np.float32)]), 4)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+        err = [l for l in lines if l.startswith("error")][0]
+        assert err == "error PDFactory::initPD"
+        pd = [l for l in lines if l.startswith("pd")][0].split()
+        assert int(pd[3]) == 32768 and float(pd[2]) > 0

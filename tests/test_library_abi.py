@@ -75,3 +75,55 @@ def test_data_binning_bit_exact(refdata):
     m1 = _lib.data_mean_subtract(d)
     m2 = O.data_mean_subtract(d)
     assert np.array_equal(m1[0], m2[0]) and m1[1] == m2[1]
+
+
+def _fits_beam_c(path):
+    L = _lib.lib()
+    L.pofd_fits_read_beam.argtypes = [ctypes.c_char_p, ctypes.c_void_p,
+                                      ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_double)]
+    n = ctypes.c_uint64()
+    ps = ctypes.c_double()
+    assert L.pofd_fits_read_beam(path.encode(), None, 0, ctypes.byref(n),
+                                 ctypes.byref(ps)) == 0
+    out = np.zeros(n.value)
+    assert L.pofd_fits_read_beam(path.encode(), out.ctypes.data, n.value,
+                                 ctypes.byref(n), ctypes.byref(ps)) == 0
+    return out, ps.value
+
+
+def _fits_map_c(path, ignore):
+    L = _lib.lib()
+    L.pofd_fits_read_map.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_uint64)]
+    n = ctypes.c_uint64()
+    assert L.pofd_fits_read_map(path.encode(), ignore, None, 0,
+                                ctypes.byref(n)) == 0
+    out = np.zeros(n.value)
+    assert L.pofd_fits_read_map(path.encode(), ignore, out.ctypes.data,
+                                n.value, ctypes.byref(n)) == 0
+    return out
+
+
+def test_fits_reader_c_matches_python(refdata, tmp_path):
+    for name in ("band1_beam.fits", "band2_beam.fits",
+                 "band12_beam_f100.fits[0]", "band12_beam_f100.fits[1]"):
+        p = os.path.join(refdata, name)
+        a, pa = _fits_beam_c(p)
+        b, pb = fitsio.read_beam(p)
+        assert np.array_equal(a, b) and pa == pb
+    for name in ("testmodel2D_band1.fits", "test_PSW.fits"):
+        p = os.path.join(refdata, name)
+        for ig in (0, 1):
+            assert np.array_equal(_fits_map_c(p, ig),
+                                  fitsio.read_map(p, bool(ig)))
+    # round trip of the writer, with a mask extension
+    img = np.random.default_rng(1).normal(size=(13, 17))
+    mask = (np.arange(13 * 17).reshape(13, 17) % 5 == 0).astype(np.int16)
+    f = str(tmp_path / "m.fits")
+    fitsio.write_image(f, img, extname="image", mask=mask)
+    assert np.array_equal(fitsio.read_map(f), img.ravel()[mask.ravel() == 0])
+    assert np.array_equal(_fits_map_c(f, 0), img.ravel()[mask.ravel() == 0])
+    assert np.array_equal(_fits_map_c(f, 1), img.ravel())
