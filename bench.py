@@ -19,6 +19,10 @@ sys.path.insert(0, ROOT)
 METRIC = ("P(D) logL evals/s (2^17, 256 walkers); DA-vs-plain ESS/s, "
           "|ΔlogL|≤1e-6")
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each kernel at the default workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel of
+# known size; None where not profiled.
+TRAFFIC = {}
 
 
 def main():
@@ -50,6 +54,7 @@ def main():
 
     def run(binned, steps, warmup):
         ctx = workload.build_context(inp, N, args.ninterp, binned=binned)
+        ctx.set_stage_timing(False)
         d_th = hiprt.DeviceBuffer(th.nbytes)
         d_th.upload(th)
         d_ll = hiprt.DeviceBuffer(W * 8)
@@ -61,6 +66,7 @@ def main():
         stream.synchronize()
         hiprt.device_synchronize()
         evs = [hiprt.Event() for _ in range(steps + 1)]
+        ctx.set_stage_timing(True)  # HIP events around every kernel stage
         t0 = time.perf_counter()
         evs[0].record(stream)
         for i in range(steps):
@@ -70,24 +76,38 @@ def main():
         stream.synchronize()
         hiprt.device_synchronize()
         wall = time.perf_counter() - t0
+        stage_ms, nl = ctx.stage_times()
+        stage_ms = {k: v / max(nl, 1) for k, v in stage_ms.items()}
+        ctx.set_stage_timing(False)
         per = [evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)]
         ll = np.zeros(W)
         st = np.zeros(W, dtype=np.int32)
         d_ll.download(ll)
         d_st.download(st)
-        return dict(wall=wall, per_ms=per, ll=ll, st=st, ctx=ctx)
+        return dict(wall=wall, per_ms=per, ll=ll, st=st, ctx=ctx,
+                    stage_ms=stage_ms)
 
     b = run(True, args.steps, args.warmup)
     ms_step = 1e3 * b["wall"] / args.steps
     evals_per_step = W
     value = evals_per_step / (b["wall"] / args.steps)
-    kern_ms = float(np.mean(b["per_ms"]))
-    # algorithmic bytes per evaluation: three compulsory stage boundaries of
-    # the FFT pipeline, each one write + one read of 8N bytes, plus the
-    # binned data amortised over the batch
+    batch_ms = float(np.mean(b["per_ms"]))
     nbins = inp["nbins"]
-    b_alg = 48.0 * N + 4.0 * nbins / W
-    achieved = b_alg * W / (kern_ms * 1e-3) / 1e9
+    # Algorithmic HBM bytes of one evaluation: the three stage boundaries of
+    # the transform pipeline that need a global dependency -- R -> forward
+    # transform, spectrum -> inverse transform, P(D) -> split/normalise/
+    # logL -- each one write plus one read of 8N bytes, plus the binned data
+    # amortised over the batch.  Per kernel: forward holds boundary 1 (write
+    # and read) and writes boundary 2; inverse reads 2 and writes 3;
+    # finalize reads 3 and the data.
+    b_eval = 48.0 * N + 4.0 * nbins / W
+    b_stage = {"forward": 24.0 * N, "inverse": 16.0 * N,
+               "finalize": 8.0 * N + 4.0 * nbins / W}
+    stage_ms = b["stage_ms"]
+    dom = max(b_stage, key=lambda k: stage_ms[k])
+    kern_ms = stage_ms[dom]
+    achieved = b_stage[dom] * W / (kern_ms * 1e-3) / 1e9
+    pipe_achieved = b_eval * W / (batch_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -116,10 +136,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "loglike1d_kernel",
+            "traffic": TRAFFIC.get(dom),
+            "kernel": dom + "1d_kernel",
             "kernel_ms": kern_ms,
-            "bytes_alg_per_eval": b_alg,
+            "bytes_alg_per_launch": b_stage[dom] * W,
+            "stage_ms": stage_ms,
+            "pipeline": {
+                "achieved": pipe_achieved,
+                "frac": pipe_achieved / HBM_PEAK_GBS,
+                "batch_ms": batch_ms,
+                "bytes_alg_per_eval": b_eval,
+            },
         },
     }
     if args.unbinned_steps > 0:
@@ -127,7 +154,8 @@ def main():
         out["unbinned"] = {
             "value": W / (u["wall"] / args.unbinned_steps),
             "unit": "evals/s",
-            "kernel_ms": float(np.mean(u["per_ms"])),
+            "batch_ms": float(np.mean(u["per_ms"])),
+            "stage_ms": u["stage_ms"],
             "pixels": int(inp["data"].size),
             "status_ok": int(np.sum(u["st"] == 0)),
         }

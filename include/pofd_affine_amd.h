@@ -51,7 +51,24 @@ enum {
      Default 44.3614195558365 (= 64 ln 2, i.e. below 2^-64 of the DC term).
      746 reproduces exactly the components the reference's exp() already
      underflows to 0.0 (src/PDFactory.cc:1180-1183). */
-  POFD_OPT_SPECTRAL_CUT = 1
+  POFD_OPT_SPECTRAL_CUT = 1,
+  /* Nonzero: the batched entry points record HIP events around each
+     kernel stage on the launch stream (read back with
+     pofd1d_stage_times); 0, the default, records none.  No reference
+     counterpart: the reference's TIMING build keeps per-stage wall-clock
+     totals in PDFactory (src/PDFactory.cc:93-126). */
+  POFD_OPT_STAGE_TIMING = 2
+};
+
+/* Kernel stages of one beam set's batch, in launch order. */
+enum {
+  POFD_STAGE_PREP = 0,      /* knot spline, R at the nodes, node spline */
+  POFD_STAGE_FORWARD = 1,   /* R on the grid, forward transform, spectrum */
+  POFD_STAGE_INVERSE = 2,   /* per dataset: inverse transform, P(D) */
+  POFD_STAGE_FINALIZE = 3,  /* per dataset: split, normalise, binned logL */
+  POFD_STAGE_PIXELS = 4,    /* unbinned logL over the pixels */
+  POFD_STAGE_COMBINE = 5,   /* sum over datasets into the outputs */
+  POFD_NSTAGES = 6
 };
 
 typedef struct pofd_ctx pofd_ctx;
@@ -156,9 +173,9 @@ int pofd1d_get_pd(pofd_ctx *ctx, uint32_t walker, int beamset_id,
    kmax (last spectral index kept), nrows, field loglike (before offset),
    status, 1/integral, minflux, P(D) stored as log2, kmax of the beam set,
    sum P, sum j P, sum of P above the split, sigma, then wall-clock ticks
-   (100 MHz) of the kernel phases: setup+R nodes, node spline, forward
-   transform, spectrum, inverse transform, split search, normalise+logL,
-   total */
+   (100 MHz) inside the kernels: prep, (unused), forward transform,
+   spectrum and noisy rows, inverse transform, split search,
+   normalise+logL, sum of these */
 int pofd1d_get_debug(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                      int field_id, double *info);
 /* prtrans after initPD for k = 0..kmax (interleaved re, im), kmax from
@@ -169,6 +186,12 @@ int pofd1d_get_spectrum(pofd_ctx *ctx, uint32_t walker, int beamset_id,
    src/PDFactory.cc:426-527) of the most recent batch */
 int pofd1d_get_rinterp(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                        double *flux_out, double *log2r_out);
+/* With POFD_OPT_STAGE_TIMING on: waits for the recorded batches, writes the
+   summed GPU milliseconds of each stage (ms_out[POFD_NSTAGES]) over every
+   beam-set launch since the previous call, *nlaunch the number of such
+   launches, and starts a new tally.  Up to 4096 launches are recorded per
+   tally.  No reference counterpart (see POFD_OPT_STAGE_TIMING). */
+int pofd1d_stage_times(pofd_ctx *ctx, double *ms_out, uint32_t *nlaunch);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,8 @@ POFD_E_ARG = 5
 POFD_E_INTERNAL = 6
 POFD_E_NOGPU = 7
 POFD_OPT_SPECTRAL_CUT = 1
+POFD_OPT_STAGE_TIMING = 2
+STAGES = ("prep", "forward", "inverse", "finalize", "pixels", "combine")
 
 STATUS_NAMES = {0: "ok", 1: "split", 2: "invalid_model", 3: "mean",
                 4: "bad_like", 5: "arg", 6: "internal", 7: "nogpu"}
@@ -64,6 +66,7 @@ _SIGS = {
     "pofd1d_get_spectrum": (C.c_int, [VP, C.c_uint32, C.c_int, DP,
                                       C.c_uint32, U32P]),
     "pofd1d_get_rinterp": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
+    "pofd1d_stage_times": (C.c_int, [VP, DP, U32P]),
 }
 
 _lib = None
@@ -197,6 +200,20 @@ class Context:
         self._check(lib().pofd_set_option(self.ptr, POFD_OPT_SPECTRAL_CUT,
                                           float(value)), "pofd_set_option")
 
+    def set_stage_timing(self, on):
+        self._check(lib().pofd_set_option(self.ptr, POFD_OPT_STAGE_TIMING,
+                                          1.0 if on else 0.0),
+                    "pofd_set_option")
+
+    def stage_times(self):
+        """{stage: summed GPU ms} over the launches since the last call, and
+        the number of beam-set launches counted."""
+        out = np.zeros(len(STAGES))
+        n = C.c_uint32()
+        self._check(lib().pofd1d_stage_times(self.ptr, _dp(out), C.byref(n)),
+                    "pofd1d_stage_times")
+        return dict(zip(STAGES, out)), n.value
+
     def configure(self, nflux, ninterp, knot_pos, exp_conf=0.0):
         kp = _f64(knot_pos)
         self.nknots = kp.size
@@ -278,7 +295,7 @@ class Context:
                   "tmn", "sgpos", "sgneg", "mn", "var_noi", "r0", "kmax",
                   "nrows", "field_loglike", "status", "itot", "minflux",
                   "pofd_is_log", "kmax_beamset", "S", "SJ", "Shigh", "sigma",
-                  "t_setup_rinterp", "t_spline", "t_forward", "t_spectral",
+                  "t_prep", "t_reserved", "t_forward", "t_spectral",
                   "t_inverse", "t_split", "t_loglike", "t_total"]
 
     def get_debug(self, walker, b=0, f=0):

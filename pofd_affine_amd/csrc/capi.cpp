@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -79,9 +80,19 @@ struct pofd_ctx {
   uint32_t last_nwalk = 0;
   std::unique_ptr<DevBuf> dtheta, dll, dst;
   size_t cap_theta = 0, cap_out = 0;
+  // stage timing: one set of POFD_NSTAGES+1 events per beam-set launch
+  bool stage_timing = false;
+  std::vector<std::array<hipEvent_t, POFD_NSTAGES + 1>> stage_events;
+  size_t stage_used = 0;
+  ~pofd_ctx() {
+    for (auto& ev : stage_events)
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
+
+constexpr size_t kMaxTimedLaunches = 4096;
 
 int fail(pofd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -358,7 +369,8 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   const size_t nf = static_cast<size_t>(d.nfields);
   w.coef_stride = static_cast<size_t>(d.ninterp - 1) * 4;
   w.ynode_stride = static_cast<size_t>(d.ninterp);
-  w.zrows_stride = static_cast<size_t>(2) * d.M2 * pofd::kM1;
+  // forward rows, then the noisy rows of each dataset
+  w.zrows_stride = (1 + nf) * static_cast<size_t>(d.M);
   w.spec_stride = static_cast<size_t>(d.M) + 1;
   w.pofd_stride = static_cast<size_t>(d.N);
   w.blk_stride = static_cast<size_t>(d.N / pofd::kBlk);
@@ -366,11 +378,14 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   const size_t n_coef = r8(w.coef_stride) * nwalk;
   const size_t n_ynode = r8(w.ynode_stride) * nwalk;
   const size_t n_z = 2 * r8(w.zrows_stride) * nwalk;
-  const size_t n_spec = 2 * r8(2 * w.spec_stride) * nwalk;  // prtrans + xp
+  const size_t n_spec = r8(2 * w.spec_stride) * nwalk;  // prtrans
   const size_t n_pofd = r8(w.pofd_stride) * nf * nwalk;
   const size_t n_blk = 2 * r8(w.blk_stride) * nf * nwalk;
   const size_t n_dbg = r8(pofd::kDbg) * nf * nwalk;
-  const size_t total = n_coef + n_ynode + n_z + n_spec + n_pofd + n_blk + n_dbg;
+  const size_t n_state = r8(pofd::kState) * nwalk;
+  const size_t n_upart = r8(pofd::kPixBlocks) * nf * nwalk;
+  const size_t total = n_coef + n_ynode + n_z + n_spec + n_pofd + n_blk +
+                       n_dbg + n_state + n_upart;
   s.wbuf.reset();
   int rc = dev_alloc(c, s.wbuf, total * sizeof(double));
   if (rc) return rc;
@@ -386,9 +401,7 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   base += n_z;
   w.spec_stride = r8(2 * w.spec_stride) / 2;
   w.prtrans = reinterpret_cast<double2*>(base);
-  base += n_spec / 2;
-  w.xp = reinterpret_cast<double2*>(base);
-  base += n_spec / 2;
+  base += n_spec;
   w.pofd_stride = r8(w.pofd_stride);
   w.pofd = base;
   base += n_pofd;
@@ -398,6 +411,10 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   w.bsum = base;
   base += n_blk / 2;
   w.dbg = base;
+  base += n_dbg;
+  w.state = base;
+  base += n_state;
+  w.upart = base;
   s.wcap = nwalk;
   return POFD_OK;
 }
@@ -455,6 +472,11 @@ extern "C" int pofd_set_option(pofd_ctx* c, int option, double value) {
       return fail(c, POFD_E_ARG, "pofd_set_option: spectral cut must be > 0");
     c->spectral_cut = value;
     for (auto& s : c->sets) s->dev.spectral_cut = value;
+    return POFD_OK;
+  }
+  if (option == POFD_OPT_STAGE_TIMING) {
+    c->stage_timing = value != 0.0;
+    c->stage_used = 0;
     return POFD_OK;
   }
   return fail(c, POFD_E_ARG, "pofd_set_option: unknown option");
@@ -599,9 +621,21 @@ extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
   for (size_t b = 0; b < c->sets.size(); ++b) {
     Beamset& s = *c->sets[b];
     if ((rc = ensure_workspace(c, s, nwalk))) return rc;
+    hipEvent_t* ev = nullptr;
+    if (c->stage_timing && c->stage_used < kMaxTimedLaunches) {
+      if (c->stage_used == c->stage_events.size()) {
+        std::array<hipEvent_t, POFD_NSTAGES + 1> fresh{};
+        for (auto& e : fresh) {
+          hipError_t he = hipEventCreate(&e);
+          if (he != hipSuccess) return hip_fail(c, he, "hipEventCreate");
+        }
+        c->stage_events.push_back(fresh);
+      }
+      ev = c->stage_events[c->stage_used++].data();
+    }
     rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta,
                                 nwalk, stride, d_ll, d_st, b > 0 ? 1 : 0,
-                                stream);
+                                stream, ev);
     if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: launch failed: " +
                                    std::string(hipGetErrorString(hipGetLastError())));
   }
@@ -726,5 +760,25 @@ extern "C" int pofd1d_get_rinterp(pofd_ctx* c, uint32_t walker, int b,
                   n * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(ynode)");
   }
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_stage_times(pofd_ctx* c, double* ms_out,
+                                  uint32_t* nlaunch) {
+  if (!c || !ms_out) return POFD_E_ARG;
+  for (int i = 0; i < POFD_NSTAGES; ++i) ms_out[i] = 0.0;
+  for (size_t l = 0; l < c->stage_used; ++l) {
+    auto& ev = c->stage_events[l];
+    hipError_t e = hipEventSynchronize(ev[POFD_NSTAGES]);
+    if (e != hipSuccess) return hip_fail(c, e, "hipEventSynchronize");
+    for (int i = 0; i < POFD_NSTAGES; ++i) {
+      float ms = 0.0f;
+      e = hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      if (e != hipSuccess) return hip_fail(c, e, "hipEventElapsedTime");
+      ms_out[i] += ms;
+    }
+  }
+  if (nlaunch) *nlaunch = static_cast<uint32_t>(c->stage_used);
+  c->stage_used = 0;
   return POFD_OK;
 }

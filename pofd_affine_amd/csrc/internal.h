@@ -27,7 +27,7 @@ struct Beam {
 
 void data_minmax(const double* d, size_t n, double& mn, double& mx);
 
-// Limits of the fused 1D kernel.
+// Limits of the 1D kernels.
 constexpr int kThreads = 1024;   // threads per workgroup (one walker)
 constexpr int kM1 = 256;         // column transform length
 constexpr int kChunk = 16;       // columns per LDS chunk
@@ -36,6 +36,8 @@ constexpr int kMaxKnots = 64;
 constexpr int kBlk = 32;         // P(D) values per min/sum block
 constexpr int kWarm = 64;        // warm-up steps of the truncated recurrences
 constexpr int kDbg = 32;         // debug scalars per (walker, field)
+constexpr int kState = 16;       // per-walker scalars handed between kernels
+constexpr int kPixBlocks = 64;   // workgroups per (walker, field), unbinned
 
 // One field (dataset) of a beam set, device view.
 struct FieldDev {
@@ -52,7 +54,7 @@ struct FieldDev {
 
 constexpr int kMaxFields = 16;
 
-// Everything the fused kernel needs for one beam set; theta-independent.
+// Everything the kernels need for one beam set; theta-independent.
 struct BeamsetDev {
   int N;        // nflux
   int M;        // N/2
@@ -90,13 +92,15 @@ struct BeamsetDev {
 struct WorkDev {
   double* coef;      // W * (ninterp-1) * 4 : y, b, c, d per interval
   double* ynode;     // W * ninterp        : log2 R at the nodes
-  double2* zrows;    // W * 2 * M2 * 256   : forward rows, then noisy rows
+  double2* zrows;    // W * (1+nfields) * M: forward rows, then the noisy
+                     //   rows of each dataset (M complex each)
   double2* prtrans;  // W * (M+1)          : exp(R^ - R^0) with phase
-  double2* xp;       // W * (M+1)          : prtrans * noise factor
   double* pofd;      // W * nfields * N    : clamped, unrotated P(D)
   double* bmin;      // W * nfields * N/32
   double* bsum;      // W * nfields * N/32
   double* dbg;       // W * nfields * kDbg : debug scalars
+  double* state;     // W * kState         : per-walker scalars
+  double* upart;     // W * nfields * kPixBlocks : unbinned partial sums
   size_t coef_stride, ynode_stride, zrows_stride, spec_stride, pofd_stride,
       blk_stride;
 };
@@ -109,10 +113,11 @@ struct pofd_beam {
 
 // launcher implemented in pofd1d_kernel.hip
 namespace pofd {
+// ev: nullptr, or POFD_NSTAGES+1 events recorded around the stages
 int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
                      const float* d_theta, uint32_t nwalk, uint32_t stride,
                      double* d_loglike, int32_t* d_status, int accumulate,
-                     void* stream);
+                     void* stream, hipEvent_t* ev);
 int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
                           int is_log, int already_log, double* d_out,
                           void* stream);
