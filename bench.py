@@ -38,6 +38,11 @@ def main():
     ap.add_argument("--cpu-evals", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pcie-steps", type=int, default=10,
+                    help="steps through the host-buffer entry point")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
+                    help="first run a plain device copy of BYTES (calibrates "
+                         "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the walker batch "
@@ -46,6 +51,20 @@ def main():
         sys.exit(2)
 
     from pofd_affine_amd import hiprt, synth, workload
+
+    if args.copy_calib > 0:
+        import ctypes
+        calib = ctypes.CDLL(os.path.join(ROOT, "pofd_affine_amd", "_build",
+                                         "libpofd_calib.so"))
+        calib.pofd_calib_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_uint64, ctypes.c_void_p]
+        nb = (args.copy_calib // 16) * 16
+        src, dst = hiprt.DeviceBuffer(nb), hiprt.DeviceBuffer(nb)
+        for _ in range(3):
+            if calib.pofd_calib_copy(src.value, dst.value, nb // 8, None):
+                raise RuntimeError("calibration copy failed")
+        hiprt.device_synchronize()
+        del src, dst
 
     W, N = args.walkers, args.nflux
     inp = workload.make_inputs(npix_side=args.side, seed=42)
@@ -159,6 +178,18 @@ def main():
             "pixels": int(inp["data"].size),
             "status_ok": int(np.sum(u["st"] == 0)),
         }
+    if args.pcie_steps > 0:
+        # host-buffer entry point: theta in and logL/status out over PCIe
+        ctx = b["ctx"]
+        ctx.loglike_batch(th)
+        t0 = time.perf_counter()
+        for _ in range(args.pcie_steps):
+            ctx.loglike_batch(th)
+        dt = (time.perf_counter() - t0) / args.pcie_steps
+        out["pcie_inclusive"] = {"value": W / dt, "unit": "evals/s",
+                                 "ms_per_step": 1e3 * dt}
+    if args.copy_calib > 0:
+        out["copy_calib_bytes"] = (args.copy_calib // 16) * 16
     if not args.no_cpu and args.cpu_evals > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -824,6 +824,70 @@ int orc_irfft(unsigned n, const double *in, double *out) {
   return ORC_OK;
 }
 
+/* 2D real transforms of an n x n row-major array with FFTW's multi-dim
+   r2c/c2r conventions (src/PDFactoryDouble.cc:727-781: the output is
+   n x (n/2+1), the last dimension halved).  r2c: real transform of every
+   row, then a complex transform down every column.  c2r: complex inverse
+   down every column, then c2r along every row (imaginary parts of the
+   k2 = 0 and n/2 entries of each row ignored, as FFTW's c2r does). */
+int orc_rfft2(unsigned n, const double *in, double *out) {
+  orc_fft *fr = orc_fft_new(n);     /* rows: real length n */
+  orc_fft *fc = orc_fft_new(2 * n); /* columns: complex length n */
+  if (!fr || !fc) {
+    orc_fft_free(fr);
+    orc_fft_free(fc);
+    return ORC_E_ARG;
+  }
+  const unsigned nc = n / 2 + 1;
+  for (unsigned i = 0; i < n; ++i) orc_rfft_plan(fr, in + (size_t)i * n, out + 2 * (size_t)i * nc);
+  double *col = (double *)malloc(sizeof(double) * 2 * n);
+  for (unsigned k2 = 0; k2 < nc; ++k2) {
+    for (unsigned i = 0; i < n; ++i) {
+      col[2 * i] = out[2 * ((size_t)i * nc + k2)];
+      col[2 * i + 1] = out[2 * ((size_t)i * nc + k2) + 1];
+    }
+    orc_cfft(fc, col, -1);
+    for (unsigned i = 0; i < n; ++i) {
+      out[2 * ((size_t)i * nc + k2)] = col[2 * i];
+      out[2 * ((size_t)i * nc + k2) + 1] = col[2 * i + 1];
+    }
+  }
+  free(col);
+  orc_fft_free(fr);
+  orc_fft_free(fc);
+  return ORC_OK;
+}
+
+int orc_irfft2(unsigned n, const double *in, double *out) {
+  orc_fft *fr = orc_fft_new(n);
+  orc_fft *fc = orc_fft_new(2 * n);
+  if (!fr || !fc) {
+    orc_fft_free(fr);
+    orc_fft_free(fc);
+    return ORC_E_ARG;
+  }
+  const unsigned nc = n / 2 + 1;
+  double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)n * nc);
+  double *col = (double *)malloc(sizeof(double) * 2 * n);
+  for (unsigned k2 = 0; k2 < nc; ++k2) {
+    for (unsigned i = 0; i < n; ++i) {
+      col[2 * i] = in[2 * ((size_t)i * nc + k2)];
+      col[2 * i + 1] = in[2 * ((size_t)i * nc + k2) + 1];
+    }
+    orc_cfft(fc, col, +1);
+    for (unsigned i = 0; i < n; ++i) {
+      tmp[2 * ((size_t)i * nc + k2)] = col[2 * i];
+      tmp[2 * ((size_t)i * nc + k2) + 1] = col[2 * i + 1];
+    }
+  }
+  for (unsigned i = 0; i < n; ++i) orc_irfft_plan(fr, tmp + 2 * (size_t)i * nc, out + (size_t)i * n);
+  free(col);
+  free(tmp);
+  orc_fft_free(fr);
+  orc_fft_free(fc);
+  return ORC_OK;
+}
+
 /* Extended-precision (x87 long double, 64-bit significand) versions of the
    same transforms.  They are not the reference's arithmetic: they give the
    exact-math value of the reference's double-precision pipeline to ~1e-19

@@ -130,6 +130,9 @@ int orc_data_bin(const double *d, size_t n, uint32_t nbins, uint32_t *bins,
 int orc_rfft(unsigned n, const double *in, double *out);
 /* in: (n/2+1) interleaved complex; out: n reals, unnormalised backward */
 int orc_irfft(unsigned n, const double *in, double *out);
+/* 2D, n x n real <-> n x (n/2+1) complex (FFTW multi-dim layout) */
+int orc_rfft2(unsigned n, const double *in, double *out);
+int orc_irfft2(unsigned n, const double *in, double *out);
 
 /* ---------------- PD helpers (src/PD.cc) ---------------- */
 double orc_pd_integral(unsigned n, const double *pd, double dflux, int islog);

@@ -283,10 +283,11 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   unsigned minitidx = static_cast<unsigned>(minI / dflux + 0.9999999999999999);
   unsigned maxitidx = static_cast<unsigned>(maxI / dflux);
   if (maxitidx > static_cast<unsigned>(wrapRidx)) maxitidx = wrapRidx;
-  // interval of the first flux of every 8-run of the grid
-  std::vector<uint16_t> kst(N / 8);
-  for (uint32_t b = 0; b < N / 8; ++b) {
-    const double cf = static_cast<double>(8 * b) * dflux;
+  // node interval of every flux of the grid (what gsl_interp_accel_find
+  // returns for it in initR)
+  std::vector<uint16_t> kst(N);
+  for (uint32_t b = 0; b < N; ++b) {
+    const double cf = static_cast<double>(b) * dflux;
     kst[b] = static_cast<uint16_t>(bsearch_idx(xnode.data(), ninterp, cf));
   }
   const uint32_t M = N / 2, M2 = M / pofd::kM1;

@@ -80,7 +80,7 @@ struct BeamsetDev {
   const double* rdelx;    // ninterp*nbeam: log2(S) - logknots[k]
   const double* bw;       // nbeam weights (1.0 for a raw beam)
   const double* bieta;    // nbeam inverse beam values
-  const uint16_t* kst;    // N/8: interval of the first flux of each 8-run
+  const uint16_t* kst;    // N: node interval of every grid flux
   const double2* W256;    // 256: exp(-2 pi i s/256)
   const double2* Wlo;     // 256: exp(-2 pi i s/M)
   const double2* Wm2;     // M2: exp(-2 pi i s/M2)

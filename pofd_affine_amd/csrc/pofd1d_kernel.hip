@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
   double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
   unsigned long long tmark = wall_clock64();
-  const int M = bs.M, M2 = bs.M2, ninterp = bs.ninterp;
+  const int M = bs.M, M2 = bs.M2;
   const double dflux = bs.dflux;
   const double sigmult = st[ST_SIGMULT];
   const double* coef = wk.coef + walker * wk.coef_stride;
@@ -496,22 +496,25 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       // fill: thread (n1 = row4, q) makes R at 8 consecutive grid points
       {
         const int j0 = 2 * M2 * row4 + 2 * kChunk * c + 8 * q;
-        int k = bs.kst[j0 >> 3];
-        double xk = bs.xnode[k], xk1 = bs.xnode[k + 1];
+        // node intervals of the 8 fluxes (gsl_interp_accel_find, precomputed)
+        const uint4 kq = *reinterpret_cast<const uint4*>(bs.kst + j0);
+        const uint32_t kw[4] = {kq.x, kq.y, kq.z, kq.w};
+        int k = kw[0] & 0xffff;
+        double xk = bs.xnode[k];
         double4 cf = *reinterpret_cast<const double4*>(coef + 4 * k);
         double rprev = 0.0;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int j = j0 + e;
+          const int ke = (kw[e >> 1] >> (16 * (e & 1))) & 0xffff;
+          if (ke != k) {
+            k = ke;
+            xk = bs.xnode[k];
+            cf = *reinterpret_cast<const double4*>(coef + 4 * k);
+          }
           double rv = 0.0;
           if (j >= bs.minit && j <= bs.maxit) {
             const double cflux = static_cast<double>(j) * dflux;
-            if (cflux >= xk1 && k < ninterp - 2) {
-              while (k < ninterp - 2 && cflux >= bs.xnode[k + 1]) ++k;
-              xk = bs.xnode[k];
-              xk1 = bs.xnode[k + 1];
-              cf = *reinterpret_cast<const double4*>(coef + 4 * k);
-            }
             const double delx = cflux - xk;
             const double s = cf.x + delx * (cf.y + delx * (cf.z + delx * cf.w));
             rv = exp2(s) * dflux;

@@ -139,6 +139,47 @@ def read_map(path, ignore_mask=False):
     return data
 
 
+def _image_and_mask(path):
+    base, ext = _split_ext(path)
+    hdus = read_hdus(base)
+    if ext is not None:
+        img = hdus[ext]
+    else:
+        named = [h for h in hdus if h.name.lower() == "image"]
+        img = named[0] if named else hdus[0]
+    if img.data is None or img.data.ndim != 2:
+        raise ValueError("Image is not 2D")
+    masks = [h for h in hdus if h.name.lower() == "mask"]
+    mask = None
+    if masks:
+        m = masks[0].data
+        if m is None or m.shape != img.data.shape:
+            raise ValueError("Mask does not match image extent")
+        mask = np.asarray(m).ravel().astype(np.int64).astype(np.uint32)
+    return np.asarray(img.data, dtype=np.float64).ravel(), mask
+
+
+def read_map_pair(path1, path2, ignore_mask=False):
+    """Pixel pairs of two maps, as fitsDataDouble::readData
+    (src/fitsDataDouble.cc:305-406, no mean subtraction): with both masks
+    a pixel is kept when (mask1 & mask2) == 0, with one mask when it is 0."""
+    d1, m1 = _image_and_mask(path1)
+    d2, m2 = _image_and_mask(path2)
+    if d1.size != d2.size:
+        raise ValueError("Data from file1 and file2 not the same size")
+    if ignore_mask:
+        m1 = m2 = None
+    if m1 is None and m2 is None:
+        return d1, d2
+    if m1 is not None and m2 is not None:
+        keep = (m1 & m2) == 0
+    else:
+        keep = (m1 if m1 is not None else m2) == 0
+    if not keep.any():
+        raise ValueError("All data masked")
+    return d1[keep], d2[keep]
+
+
 def _card(key, value, comment=""):
     if isinstance(value, bool):
         v = "%20s" % ("T" if value else "F")

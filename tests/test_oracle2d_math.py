@@ -1,0 +1,115 @@
+"""Mathematical pins of the 2D oracle where the reference's tests leave it
+open: transforms against numpy, the log-normal solver against scipy,
+Campbell's theorem for the P(D) moments, and the Gaussian limit."""
+import math
+
+import numpy as np
+import pytest
+from scipy import optimize
+
+import pyoracle2d as O2
+from pofd_affine_amd import synth2d
+from oracle2d_side import build_oracle2d, make_inputs2d
+
+
+@pytest.mark.parametrize("n", [8, 64, 256])
+def test_rfft2_irfft2_match_numpy(n):
+    x = np.random.default_rng(n).normal(size=(n, n))
+    X = O2.rfft2(x)
+    np.testing.assert_allclose(X, np.fft.rfft2(x), rtol=0,
+                               atol=1e-12 * np.abs(X).max())
+    y = O2.irfft2(X, n)
+    np.testing.assert_allclose(y, x * n * n, rtol=0, atol=1e-10 * n * n)
+
+
+def test_lognormal_solver_matches_brentq():
+    """getMaxFlux band 2 = S1_max * exp(u), u the root above the mode where
+    the log-normal density in u falls to exp(-nsig^2/2) of its peak."""
+    kp, sp, op = synth2d.positions()
+    m = O2.LogNormalModel(kp, sp, op)
+    m.set_params(synth2d.true_theta())
+    mx = m.maxflux()
+    mu = m.offset(kp[-1])
+    sg = m.sigma(kp[-1])
+    sig2 = sg * sg
+    mode = mu - sig2
+
+    def dens(u):
+        return math.exp(-0.5 * (u - mu) ** 2 / sig2) * math.exp(-u) / sg
+
+    target = dens(mode) * math.exp(-0.5 * 9.0)
+    u = optimize.brentq(lambda v: dens(v) - target, mode, mode + 20 * sg,
+                        xtol=1e-14)
+    assert mx[0] == kp[-1]
+    assert mx[1] / kp[-1] == pytest.approx(math.exp(u), rel=2e-3)
+
+
+@pytest.fixture(scope="module")
+def small2d():
+    inp = make_inputs2d(npix_side=120, seed=5)
+    th = synth2d.true_theta()
+    return inp, th
+
+
+def test_pd2d_moments_campbell(small2d):
+    """Compound Poisson + Gaussian noise per band: the normalised 2D P(D)
+    has mean 0 after the un-shift and marginal variances var_noi + sigma^2
+    (the R moments of getRStats)."""
+    inp, th = small2d
+    n = 512  # at 256 the negative side is under 3 bins: findSplitPoint fails
+    L = build_oracle2d(inp, n, th, nedge=64)
+    ip = L.initpd(0, th)
+    pd, info = L.getpd(0, 0, 1.0, 1.0, setlog=False)
+    x1 = info["minflux1"] + np.arange(n) * info["dflux1"]
+    x2 = info["minflux2"] + np.arange(n) * info["dflux2"]
+    w = pd * info["dflux1"] * info["dflux2"]
+    p1, p2 = w.sum(1), w.sum(0)
+    m1, m2 = (x1 * p1).sum() / p1.sum(), (x2 * p2).sum() / p2.sum()
+    v1 = ((x1 - m1) ** 2 * p1).sum() / p1.sum()
+    v2 = ((x2 - m2) ** 2 * p2).sum() / p2.sum()
+    assert abs(m1) < 2 * info["dflux1"] and abs(m2) < 2 * info["dflux2"]
+    assert v1 == pytest.approx(ip["var1"] + synth2d.SIGMA1 ** 2, rel=2e-2)
+    assert v2 == pytest.approx(ip["var2"] + synth2d.SIGMA2 ** 2, rel=2e-2)
+
+
+def test_pd2d_negligible_R_is_gaussian(small2d):
+    """Counts 1e-30 below the fiducial ones: R vanishes, the P(D) is the
+    product of the two instrument Gaussians, and logL is their Gaussian
+    log likelihood (to the bilinear-interpolation error of log2 P)."""
+    inp, _ = small2d
+    th = synth2d.true_theta().copy()
+    th[:len(synth2d.KNOTS_MJY)] -= 30.0
+    rng = np.random.default_rng(3)
+    s1, s2 = synth2d.SIGMA1, synth2d.SIGMA2
+    d1 = rng.normal(0, s1, 20000)
+    d2 = rng.normal(0, s2, 20000)
+    d1 -= d1.mean()
+    d2 -= d2.mean()
+    inp2 = dict(inp, d1=d1, d2=d2)
+    n = 512
+    L = build_oracle2d(inp2, n, th, nedge=64)
+    ll, rc = L.loglike(th)
+    assert rc == 0
+    info = L.beamset_info(0)
+    gauss = np.sum(-0.5 * np.log(2 * np.pi * s1 * s1) - 0.5 * (d1 / s1) ** 2
+                   - 0.5 * np.log(2 * np.pi * s2 * s2) - 0.5 * (d2 / s2) ** 2)
+    curr = ll + info["like_offset0"]
+    df1 = (info["maxR1"] - info["minR1"]) / (n - 1)
+    df2 = (info["maxR2"] - info["minR2"]) / (n - 1)
+    tol = d1.size * (df1 ** 2 / (8 * s1 * s1) + df2 ** 2 / (8 * s2 * s2)) \
+        + 1e-6 * abs(gauss)
+    assert abs(curr - gauss) < tol
+
+
+def test_data2_binning_centres():
+    rng = np.random.default_rng(1)
+    d1, d2 = rng.normal(size=5000), rng.normal(size=5000) * 2
+    bins, info = O2.data2_bin(d1, d2, 40, 30)
+    assert bins.sum() == 5000
+    assert info[0] == d1.min() and info[2] == d2.min()
+    assert info[0] + 39 * info[1] == pytest.approx(d1.max(), rel=1e-14)
+    i1 = ((d1 - info[0]) / info[1] + 0.5).astype(int)
+    i2 = ((d2 - info[2]) / info[3] + 0.5).astype(int)
+    ref = np.zeros((40, 30), dtype=np.uint32)
+    np.add.at(ref, (i1, i2), 1)
+    assert np.array_equal(ref, bins)
