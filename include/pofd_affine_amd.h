@@ -193,6 +193,95 @@ int pofd1d_get_rinterp(pofd_ctx *ctx, uint32_t walker, int beamset_id,
    tally.  No reference counterpart (see POFD_OPT_STAGE_TIMING). */
 int pofd1d_stage_times(pofd_ctx *ctx, double *ms_out, uint32_t *nlaunch);
 
+/* ======================================================================
+   Two-band (2D) path: calcLikeDouble / PDFactoryDouble / PDDouble with the
+   numberCountsDoubleLogNormal colour model.
+   ====================================================================== */
+typedef struct pofd_dbeam pofd_dbeam;
+
+/* ---- doublebeam::setBeams and makeHistogram, src/doublebeam.cc:331-584.
+        Host code, bit-exact.  beam1/beam2 are the two images in FITS
+        order, n pixels each; pixels with |b| > minval in both are kept and
+        sorted into the pp, pn, np, nn sign components. ---- */
+int pofd_dbeam_create(pofd_dbeam **out, const double *beam1,
+                      const double *beam2, uint64_t n, double pixsize_arcsec,
+                      double minval);
+int pofd_dbeam_make_histogram(pofd_dbeam *bm, uint32_t nbins);
+void pofd_dbeam_destroy(pofd_dbeam *bm);
+const char *pofd_dbeam_last_error(void);
+/* out[64]: per sign s at 12*s: hassign, npix, ishist, nhist, tot1, tot2,
+   totsq1, totsq2, min1, max1, min2, max2; out[48..50] = pixsize, minval,
+   nbins (doublebeam accessors, src/doublebeam.cc:598-720) */
+int pofd_dbeam_info(const pofd_dbeam *bm, double *out);
+/* which: 0 pix1, 1 pix2, 2 1/pix1, 3 1/pix2, 4 log(pix1/pix2), 5 bin
+   weights, 6 bin 1/beam1, 7 bin 1/beam2, 8 bin log ratio; *n gets the
+   length, out may be NULL (src/doublebeam.cc:722-776) */
+int pofd_dbeam_get(const pofd_dbeam *bm, int sgn, int which, double *out,
+                   uint32_t *n);
+
+/* ---- fitsDataDouble::applyBinning and meanSubtract,
+        src/fitsDataDouble.cc:517-608.  bins[nb1*nb2] row-major in band 1;
+        info[4] = bincent01, bindelta1, bincent02, bindelta2. ---- */
+int pofd_data2_bin(const double *d1, const double *d2, uint64_t n,
+                   uint32_t nb1, uint32_t nb2, uint32_t *bins, double *info);
+int pofd_data2_mean_subtract(double *d1, double *d2, uint64_t n,
+                             double *mean);
+
+/* ---- context set-up: calcLikeDouble(FFTSIZE, NEDGE, EDGEINTEG)
+        (include/calcLikeDouble.h:241-243), setPositions of the model
+        (src/numberCountsDoubleLogNormal.cc:632-920), setMinSigma (:792),
+        setExpConf1/2.  n a power of two in [64, 4096]. ---- */
+int pofd2d_configure(pofd_ctx *ctx, uint32_t n, uint32_t nedge,
+                     int edge_integ, const double *knots, uint32_t nknots,
+                     const double *sigmapos, uint32_t nsigma,
+                     const double *offsetpos, uint32_t noffset,
+                     double min_sigma, double exp_conf1, double exp_conf2);
+/* calcLikeDouble::readDataFromFiles grouping by beam
+   (src/calcLikeDouble.cc:930-1020): one beam set per beam pair */
+int pofd2d_add_beamset(pofd_ctx *ctx, const pofd_dbeam *bm, int *beamset_id);
+/* one dataset (pixel pairs, or a 2D histogram when bins != NULL) with its
+   instrument noise per band and likelihood normalisation */
+int pofd2d_add_field(pofd_ctx *ctx, int beamset_id, const double *d1,
+                     const double *d2, uint64_t npix, const uint32_t *bins,
+                     uint32_t nb1, uint32_t nb2, const double *bininfo,
+                     double sigma_base1, double sigma_base2,
+                     double like_norm, int *field_id);
+/* calcLikeDouble::setRRanges(p) (src/calcLikeDouble.cc:1093-1097): R
+   ranges of every beam set from the model at theta, like offsets, and the
+   device tables */
+int pofd2d_prepare(pofd_ctx *ctx, const float *theta, uint32_t npar);
+/* override of a beam set's R range: range[4] = min1, max1, min2, max2 */
+int pofd2d_set_rrange(pofd_ctx *ctx, int beamset_id, const double *range);
+/* out[14]: minR1, maxR1, minR2, maxR2, maxsig1, maxsig2, min/max data band
+   1, min/max data band 2, nfields, like_offset of dataset 0, dflux1,
+   dflux2 */
+int pofd2d_beamset_info(pofd_ctx *ctx, int beamset_id, double *out);
+
+/* ---- hot call: calcLikeDouble::getLogLike (src/calcLikeDouble.cc:
+        1195-1226, priors excluded) for nwalk walkers.  theta rows are the
+        reference's float paramSet: [knot values (log10), sigma values,
+        offset values, sigmult1, sigmult2, ...].  Status as the 1D path;
+        a model failing isValid() gives POFD_E_INVALID_MODEL. ---- */
+int pofd2d_loglike_batch(pofd_ctx *ctx, const float *theta, uint32_t nwalk,
+                         uint32_t theta_stride, double *loglike_out,
+                         int32_t *status_out);
+int pofd2d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
+                                uint32_t nwalk, uint32_t theta_stride,
+                                double *d_loglike, int32_t *d_status,
+                                void *hip_stream);
+
+/* ---- parity and debugging (PDFactoryDouble::getPD output) ---- */
+/* info[32]: status, split1, split2, sg1pos, sg2pos, sigma1, sigma2,
+   integral, 1/integral, minflux1, minflux2, dataset logL, mn1, mn2, var1,
+   var2, r0, K1, K2 (spectral bounds kept) */
+int pofd2d_get_debug(pofd_ctx *ctx, uint32_t walker, int beamset_id,
+                     int field_id, double *info);
+/* pd_out[n*n]: rotated, normalised P(D) (log2 when is_log) of the last
+   batch; info[8] = minflux1, minflux2, dflux1, dflux2, split1, split2,
+   sg1pos, sg2pos */
+int pofd2d_get_pd(pofd_ctx *ctx, uint32_t walker, int beamset_id,
+                  int field_id, int is_log, double *pd_out, double *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,32 @@ _SIGS = {
                                       C.c_uint32, U32P]),
     "pofd1d_get_rinterp": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
     "pofd1d_stage_times": (C.c_int, [VP, DP, U32P]),
+    "pofd_dbeam_create": (C.c_int, [C.POINTER(VP), DP, DP, C.c_uint64, D, D]),
+    "pofd_dbeam_make_histogram": (C.c_int, [VP, C.c_uint32]),
+    "pofd_dbeam_destroy": (None, [VP]),
+    "pofd_dbeam_last_error": (C.c_char_p, []),
+    "pofd_dbeam_info": (C.c_int, [VP, DP]),
+    "pofd_dbeam_get": (C.c_int, [VP, C.c_int, C.c_int, DP, U32P]),
+    "pofd_data2_bin": (C.c_int, [DP, DP, C.c_uint64, C.c_uint32, C.c_uint32,
+                                 U32P, DP]),
+    "pofd_data2_mean_subtract": (C.c_int, [DP, DP, C.c_uint64, DP]),
+    "pofd2d_configure": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_int, DP,
+                                   C.c_uint32, DP, C.c_uint32, DP,
+                                   C.c_uint32, D, D, D]),
+    "pofd2d_add_beamset": (C.c_int, [VP, VP, C.POINTER(C.c_int)]),
+    "pofd2d_add_field": (C.c_int, [VP, C.c_int, DP, DP, C.c_uint64, U32P,
+                                   C.c_uint32, C.c_uint32, DP, D, D, D,
+                                   C.POINTER(C.c_int)]),
+    "pofd2d_prepare": (C.c_int, [VP, FP, C.c_uint32]),
+    "pofd2d_set_rrange": (C.c_int, [VP, C.c_int, DP]),
+    "pofd2d_beamset_info": (C.c_int, [VP, C.c_int, DP]),
+    "pofd2d_loglike_batch": (C.c_int, [VP, FP, C.c_uint32, C.c_uint32, DP,
+                                       I32P]),
+    "pofd2d_loglike_batch_device": (C.c_int, [VP, VP, C.c_uint32,
+                                              C.c_uint32, VP, VP, VP]),
+    "pofd2d_get_debug": (C.c_int, [VP, C.c_uint32, C.c_int, C.c_int, DP]),
+    "pofd2d_get_pd": (C.c_int, [VP, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                DP, DP]),
 }
 
 _lib = None
@@ -320,3 +346,170 @@ class Context:
         self._check(lib().pofd1d_get_rinterp(self.ptr, walker, b, _dp(x),
                                              _dp(y)), "pofd1d_get_rinterp")
         return x, y
+
+
+# ------------------------------------------------------------- two bands
+class DoubleBeam:
+    """doublebeam::setBeams + makeHistogram restated (host, bit-exact)."""
+
+    KEYS = ["hassign", "npix", "ishist", "nhist", "tot1", "tot2", "totsq1",
+            "totsq2", "min1", "max1", "min2", "max2"]
+
+    def __init__(self, b1, b2, pixsize, minval=1e-6, nhist=None):
+        b1, b2 = _f64(np.ravel(b1)), _f64(np.ravel(b2))
+        assert b1.size == b2.size
+        self.ptr = VP()
+        rc = lib().pofd_dbeam_create(C.byref(self.ptr), _dp(b1), _dp(b2),
+                                     b1.size, float(pixsize), float(minval))
+        if rc:
+            raise PofdError(rc, lib().pofd_dbeam_last_error().decode())
+        if nhist:
+            rc = lib().pofd_dbeam_make_histogram(self.ptr, int(nhist))
+            if rc:
+                raise PofdError(rc, lib().pofd_dbeam_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.pofd_dbeam_destroy(self.ptr)
+            self.ptr = None
+
+    def info(self):
+        out = np.zeros(64)
+        lib().pofd_dbeam_info(self.ptr, _dp(out))
+        sgn = [dict(zip(self.KEYS, out[12 * k:12 * k + 12])) for k in range(4)]
+        return sgn, dict(pixsize=out[48], minval=out[49], nbins=out[50])
+
+    def array(self, sgn, which):
+        names = ["pix1", "pix2", "ipix1", "ipix2", "logratio", "binw",
+                 "binv1", "binv2", "binlogratio"]
+        n = C.c_uint32()
+        lib().pofd_dbeam_get(self.ptr, sgn, names.index(which), None,
+                             C.byref(n))
+        out = np.zeros(n.value)
+        if n.value:
+            lib().pofd_dbeam_get(self.ptr, sgn, names.index(which), _dp(out),
+                                 C.byref(n))
+        return out
+
+
+def data2_bin(d1, d2, nb1, nb2):
+    """fitsDataDouble::applyBinning restated (host, bit-exact)."""
+    d1, d2 = _f64(d1), _f64(d2)
+    bins = np.zeros(nb1 * nb2, dtype=np.uint32)
+    info = np.zeros(4)
+    rc = lib().pofd_data2_bin(_dp(d1), _dp(d2), d1.size, nb1, nb2,
+                              bins.ctypes.data_as(U32P), _dp(info))
+    if rc:
+        raise PofdError(rc, "fitsDataDouble::applyBinning failed")
+    return bins.reshape(nb1, nb2), info
+
+
+def data2_mean_subtract(d1, d2):
+    d1, d2 = _f64(d1).copy(), _f64(d2).copy()
+    m = np.zeros(2)
+    rc = lib().pofd_data2_mean_subtract(_dp(d1), _dp(d2), d1.size, _dp(m))
+    if rc:
+        raise PofdError(rc, "fitsDataDouble::meanSubtract failed")
+    return d1, d2, m
+
+
+class Context2(Context):
+    """A pofd_ctx used for the two-band likelihood (calcLikeDouble)."""
+
+    DEBUG_KEYS = ["status", "split1", "split2", "sg1pos", "sg2pos", "sigma1",
+                  "sigma2", "integral", "itot", "minflux1", "minflux2",
+                  "field_loglike", "mn1", "mn2", "var1", "var2", "r0", "K1",
+                  "K2"]
+
+    def configure2d(self, n, knots, sigmapos, offsetpos, nedge=256,
+                    edge_integ=True, min_sigma=0.0, exp_conf1=0.0,
+                    exp_conf2=0.0):
+        kp, sp, op = _f64(knots), _f64(sigmapos), _f64(offsetpos)
+        self.n = n
+        self.nmodel = kp.size + sp.size + op.size
+        self._check(lib().pofd2d_configure(self.ptr, n, nedge,
+                                           int(edge_integ), _dp(kp), kp.size,
+                                           _dp(sp), sp.size, _dp(op), op.size,
+                                           min_sigma, exp_conf1, exp_conf2),
+                    "pofd2d_configure")
+
+    def add_beamset2d(self, dbeam):
+        i = C.c_int()
+        self._keep.append(dbeam)
+        self._check(lib().pofd2d_add_beamset(self.ptr, dbeam.ptr, C.byref(i)),
+                    "pofd2d_add_beamset")
+        return i.value
+
+    def add_field2d(self, b, d1, d2, sigma_base1, sigma_base2, like_norm=1.0,
+                    bins=None, bininfo=None):
+        d1, d2 = _f64(d1), _f64(d2)
+        self._keep += [d1, d2]
+        i = C.c_int()
+        if bins is not None:
+            bins = np.ascontiguousarray(bins, dtype=np.uint32)
+            nb1, nb2 = bins.shape
+            info = _f64(bininfo)
+            self._keep += [bins, info]
+            bp, ip = bins.ctypes.data_as(U32P), _dp(info)
+        else:
+            nb1 = nb2 = 0
+            bp, ip = C.cast(None, U32P), C.cast(None, DP)
+        self._check(lib().pofd2d_add_field(self.ptr, b, _dp(d1), _dp(d2),
+                                           d1.size, bp, nb1, nb2, ip,
+                                           sigma_base1, sigma_base2,
+                                           like_norm, C.byref(i)),
+                    "pofd2d_add_field")
+        return i.value
+
+    def prepare2d(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        self._check(lib().pofd2d_prepare(self.ptr, t.ctypes.data_as(FP),
+                                         t.size), "pofd2d_prepare")
+
+    def set_rrange2d(self, b, rng):
+        r = _f64(rng)
+        self._check(lib().pofd2d_set_rrange(self.ptr, b, _dp(r)),
+                    "pofd2d_set_rrange")
+
+    def beamset_info2d(self, b):
+        out = np.zeros(14)
+        self._check(lib().pofd2d_beamset_info(self.ptr, b, _dp(out)),
+                    "pofd2d_beamset_info")
+        keys = ["minR1", "maxR1", "minR2", "maxR2", "maxsig1", "maxsig2",
+                "mind1", "maxd1", "mind2", "maxd2", "nfields", "like_offset0",
+                "dflux1", "dflux2"]
+        return dict(zip(keys, out))
+
+    def loglike2d_batch(self, thetas):
+        t = np.ascontiguousarray(np.atleast_2d(thetas), dtype=np.float32)
+        w, stride = t.shape
+        ll = np.zeros(w)
+        st = np.zeros(w, dtype=np.int32)
+        self._check(lib().pofd2d_loglike_batch(self.ptr, t.ctypes.data_as(FP),
+                                               w, stride, _dp(ll),
+                                               st.ctypes.data_as(I32P)),
+                    "pofd2d_loglike_batch")
+        return ll, st
+
+    def loglike2d_batch_device(self, d_theta, nwalk, stride, d_ll, d_st,
+                               stream=0):
+        self._check(lib().pofd2d_loglike_batch_device(
+            self.ptr, d_theta, nwalk, stride, d_ll, d_st, stream),
+            "pofd2d_loglike_batch_device")
+
+    def get_debug2d(self, walker, b=0, f=0):
+        out = np.zeros(32)
+        self._check(lib().pofd2d_get_debug(self.ptr, walker, b, f, _dp(out)),
+                    "pofd2d_get_debug")
+        return dict(zip(self.DEBUG_KEYS, out))
+
+    def get_pd2d(self, walker, b=0, f=0, is_log=True):
+        n = self.n
+        pd = np.zeros(n * n)
+        info = np.zeros(8)
+        self._check(lib().pofd2d_get_pd(self.ptr, walker, b, f, int(is_log),
+                                        _dp(pd), _dp(info)), "pofd2d_get_pd")
+        keys = ["minflux1", "minflux2", "dflux1", "dflux2", "split1",
+                "split2", "sgpos1", "sgpos2"]
+        return pd.reshape(n, n), dict(zip(keys, info))
+

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "internal2d.h"
 
 namespace {
 
@@ -84,11 +85,24 @@ struct pofd_ctx {
   bool stage_timing = false;
   std::vector<std::array<hipEvent_t, POFD_NSTAGES + 1>> stage_events;
   size_t stage_used = 0;
+  pofd2::Ctx2* c2 = nullptr;  // two-band state (capi2d.cpp)
   ~pofd_ctx() {
     for (auto& ev : stage_events)
       for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    pofd2::ctx2_free(c2);
   }
 };
+
+// accessors for the two-band C ABI (capi2d.cpp)
+namespace pofd2 {
+Ctx2*& ctx2_of(pofd_ctx* c) { return c->c2; }
+int ctx_device(const pofd_ctx* c) { return c->device; }
+double ctx_spectral_cut(const pofd_ctx* c) { return c->spectral_cut; }
+int ctx_fail(pofd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+}  // namespace pofd2
 
 namespace {
 
@@ -554,6 +568,9 @@ extern "C" int pofd1d_add_field(pofd_ctx* c, int b, const double* pix,
     f.maxdata = bincent0 + static_cast<double>(nbins - 1) * bindelta;
   } else {
     if (!pix) return fail(c, POFD_E_ARG, "pofd1d_add_field: no pixels");
+    for (uint64_t i = 0; i < npix; ++i)
+      if (!std::isfinite(pix[i]))
+        return fail(c, POFD_E_ARG, "pofd1d_add_field: non-finite pixel value");
     f.pix.assign(pix, pix + npix);
     pofd::data_minmax(f.pix.data(), f.pix.size(), f.mindata, f.maxdata);
     std::sort(f.pix.begin(), f.pix.end());
