@@ -19,10 +19,16 @@ sys.path.insert(0, ROOT)
 METRIC = ("P(D) logL evals/s (2^17, 256 walkers); DA-vs-plain ESS/s, "
           "|ΔlogL|≤1e-6")
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of each kernel at the default workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel of
-# known size; None where not profiled.
-TRAFFIC = {}
+# HBM bytes per launch of each kernel at the default workload (256 walkers,
+# nflux 2^17, binned), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# scaled by a plain copy of known size (fetched bytes = 2.0 x FETCH_SIZE KiB,
+# written = 1.0 x WRITE_SIZE KiB); None where not profiled or for another
+# workload.
+TRAFFIC = {
+    "forward": (2.0 * 9545.3 + 16156.6) * 1024,
+    "inverse": (2.0 * 2291.5 + 332998.1) * 1024,
+    "finalize": (2.0 * 9930.3 + 64.0) * 1024,
+}
 
 
 def main():
@@ -127,6 +133,8 @@ def main():
     kern_ms = stage_ms[dom]
     achieved = b_stage[dom] * W / (kern_ms * 1e-3) / 1e9
     pipe_achieved = b_eval * W / (batch_ms * 1e-3) / 1e9
+    default_workload = (N == 131072 and W == 256 and args.side == 1000 and
+                        args.ninterp == 2048)
     out = {
         "metric": METRIC,
         "value": value,
@@ -141,10 +149,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "1D P(D) log likelihood, nflux=2^17, ninterp=2048, "
-                        "%d walkers per launch, 1000x1000-pixel synthetic "
-                        "map binned into 1000 bins, 120-bin Gaussian beam "
-                        "histogram, sigma 2.5 mJy" % W,
+            "workload": "1D P(D) log likelihood, nflux=%d, ninterp=%d, "
+                        "%d walkers per launch, %dx%d-pixel synthetic "
+                        "map binned into %d bins, 120-bin Gaussian beam "
+                        "histogram, sigma 2.5 mJy" % (N, args.ninterp, W,
+                                                      args.side, args.side,
+                                                      nbins),
             "nflux": N, "walkers": W, "map_side": args.side,
             "nbins": nbins, "ninterp": args.ninterp,
         },
@@ -155,7 +165,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": TRAFFIC.get(dom),
+            "traffic": TRAFFIC.get(dom) if default_workload else None,
             "kernel": dom + "1d_kernel",
             "kernel_ms": kern_ms,
             "bytes_alg_per_launch": b_stage[dom] * W,

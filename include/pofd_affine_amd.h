@@ -186,6 +186,11 @@ int pofd1d_get_spectrum(pofd_ctx *ctx, uint32_t walker, int beamset_id,
    src/PDFactory.cc:426-527) of the most recent batch */
 int pofd1d_get_rinterp(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                        double *flux_out, double *log2r_out);
+/* The same for the negative beam: RinterpFlux_neg / RinterpVals_neg
+   (src/PDFactory.cc:486-524), ascending fluxes below zero; POFD_E_ARG when
+   the beam has no negative pixels */
+int pofd1d_get_rinterp_neg(pofd_ctx *ctx, uint32_t walker, int beamset_id,
+                           double *flux_out, double *log2r_out);
 /* With POFD_OPT_STAGE_TIMING on: waits for the recorded batches, writes the
    summed GPU milliseconds of each stage (ms_out[POFD_NSTAGES]) over every
    beam-set launch since the previous call, *nlaunch the number of such

@@ -12,8 +12,8 @@
  *
  * Differences a caller can observe:
  *  - calcLike::getLogLikeBatch evaluates many walkers in one launch;
- *  - beams with negative pixels, the CFIRB and Poisson priors and the bonus
- *    parameters (fillBonusParams) throw affineExcept("...", "...",
+ *  - the CFIRB and Poisson priors and the bonus parameters
+ *    (fillBonusParams) throw affineExcept("...", "...",
  *    "not supported by the device path");
  *  - PD::getLogLike is not provided: log likelihoods come from
  *    calcLike::getLogLike, which runs the whole chain on the device.

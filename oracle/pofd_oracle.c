@@ -2060,6 +2060,16 @@ int orc_like_initpd(orc_like *L, int b, const float *theta, unsigned npar,
   return ORC_OK;
 }
 
+int orc_like_rinterp_neg(orc_like *L, double *flux, double *vals) {
+  /* negative-beam interpolation table of the last orc_like_initpd
+     (RinterpFlux_neg / RinterpVals_neg, src/PDFactory.cc:486-524) */
+  orc_pdfac *p = orc_like_fac(L);
+  if (!p->has_interp_neg) return ORC_E_ARG;
+  if (flux) memcpy(flux, p->RinterpFlux_neg, sizeof(double) * L->ninterp);
+  if (vals) memcpy(vals, p->RinterpVals_neg, sizeof(double) * L->ninterp);
+  return ORC_OK;
+}
+
 int orc_like_getpd(orc_like *L, int b, int f, double sigmult, int setlog,
                    double *pofd_raw, double *pd_out, double *info) {
   memset(&L->err, 0, sizeof L->err);

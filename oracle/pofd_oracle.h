@@ -186,6 +186,9 @@ int orc_like_loglike_batch(orc_like *L, const float *theta, unsigned nwalk,
 int orc_like_initpd(orc_like *L, int b, const float *theta, unsigned npar,
                     double *rinterp_flux, double *rinterp_vals, double *rvals,
                     double *prtrans, double *scalars);
+/* RinterpFlux_neg / RinterpVals_neg of the last orc_like_initpd;
+   ORC_E_ARG when the beam had no negative part */
+int orc_like_rinterp_neg(orc_like *L, double *flux, double *vals);
 /* getPD for field f of beam set b (after orc_like_initpd on the same
    context).  pd_out n values; info[8] = minflux, dflux, splitidx, peak,
    sgpos, sgneg, integral_before_norm, islog.  setlog as getPD(setLog). */

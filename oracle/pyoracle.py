@@ -100,6 +100,7 @@ def lib():
                                       DP, DP, DP, DP]),
         "orc_like_getpd": (C.c_int, [C.c_void_p, C.c_int, C.c_int, D, C.c_int,
                                      DP, DP, DP]),
+        "orc_like_rinterp_neg": (C.c_int, [C.c_void_p, DP, DP]),
         "orc_like_field_loglike": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
                                              DP]),
     }
@@ -413,6 +414,13 @@ class Like:
                     prtrans=pr[0::2] + 1j * pr[1::2], dflux=sc[0],
                     minflux_R=sc[1], wrapRidx=int(sc[2]), mn=sc[3],
                     var_noi=sc[4], doshift=bool(sc[5]), shift=sc[6], r0=sc[7])
+
+    def rinterp_neg(self):
+        """Negative-beam interpolation nodes and log2 R of the last initpd."""
+        rf = np.zeros(self.ninterp)
+        rv = np.zeros(self.ninterp)
+        self._check(lib().orc_like_rinterp_neg(self.ptr, _dp(rf), _dp(rv)))
+        return rf, rv
 
     def getpd(self, b, f, sigmult=1.0, setlog=True):
         n = self.fftsize

@@ -66,6 +66,7 @@ _SIGS = {
     "pofd1d_get_spectrum": (C.c_int, [VP, C.c_uint32, C.c_int, DP,
                                       C.c_uint32, U32P]),
     "pofd1d_get_rinterp": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
+    "pofd1d_get_rinterp_neg": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
     "pofd1d_stage_times": (C.c_int, [VP, DP, U32P]),
     "pofd_dbeam_create": (C.c_int, [C.POINTER(VP), DP, DP, C.c_uint64, D, D]),
     "pofd_dbeam_make_histogram": (C.c_int, [VP, C.c_uint32]),
@@ -345,6 +346,15 @@ class Context:
         y = np.zeros(self.ninterp)
         self._check(lib().pofd1d_get_rinterp(self.ptr, walker, b, _dp(x),
                                              _dp(y)), "pofd1d_get_rinterp")
+        return x, y
+
+    def get_rinterp_neg(self, walker, b=0):
+        """Negative-beam interpolation nodes (ascending, < 0) and log2 R."""
+        x = np.zeros(self.ninterp)
+        y = np.zeros(self.ninterp)
+        self._check(lib().pofd1d_get_rinterp_neg(self.ptr, walker, b, _dp(x),
+                                                 _dp(y)),
+                    "pofd1d_get_rinterp_neg")
         return x, y
 
 

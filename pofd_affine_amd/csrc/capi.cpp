@@ -178,13 +178,9 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   const unsigned nk = static_cast<unsigned>(c->knots.size());
   if (s.fields.empty())
     return fail(c, POFD_E_ARG, "calcLikeSingle::readDataFromFile: No data sets");
-  if (bm.pos.empty())
-    return fail(c, POFD_E_ARG,
-                "pofd1d_prepare: the beam has no positive pixels");
-  if (!bm.neg.empty())
-    return fail(c, POFD_E_ARG,
-                "pofd1d_prepare: beams with negative pixels are not supported "
-                "by the device path");
+  const bool haspos = !bm.pos.empty(), hasneg = !bm.neg.empty();
+  if (!haspos && !hasneg)
+    return fail(c, POFD_E_ARG, "PDFactory::initPD: Empty beam");
   // data range (src/calcLike.cc:205-230) and sigma base (:283-308)
   s.minDataFlux = s.fields[0].mindata;
   s.maxDataFlux = s.fields[0].maxdata;
@@ -202,107 +198,182 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
       f.like_offset = -0.5 * ndd * (log(2.0 * kPi * var) + 1.0);
     }
   }
-  // setRRange
+  // range where R is non-zero (numberCountsKnots::getRRangeInternal,
+  // src/numberCountsKnots.cc:345-365)
+  const double minknot = c->knots[0], maxknot = c->knots[nk - 1];
+  double rmin, rmax;
+  if (haspos) {
+    rmax = maxknot * bm.pos.back();
+    rmin = hasneg ? -maxknot * bm.neg.back() : 0.0;
+  } else {
+    rmax = 0.0;
+    rmin = -maxknot * bm.neg.back();
+  }
+  // setRRange (src/calcLike.cc:346-371)
   if (std::isnan(s.minRFlux) || std::isnan(s.maxRFlux)) {
-    const double maxknot = c->knots[nk - 1];
-    const double maxRF = maxknot * bm.pos.back();
-    const double minRF = 0.0;  // positive beam only
     const double sigma = sqrt(c->exp_conf * c->exp_conf +
                               s.maxsigma_base * s.maxsigma_base);
-    s.minRFlux = 1.05 * minRF;
-    s.maxRFlux = 1.05 * maxRF + 7.5 * sigma;
-    if (s.maxDataFlux > s.maxRFlux) s.maxRFlux = s.maxDataFlux;
+    s.minRFlux = 1.05 * rmin;
+    s.maxRFlux = 1.05 * rmax + 7.5 * sigma;
+    if (s.minDataFlux < s.minRFlux && hasneg) s.minRFlux = s.minDataFlux;
+    if (s.maxDataFlux > s.maxRFlux && haspos) s.maxRFlux = s.maxDataFlux;
   }
-  // initRFlux (simple case: 0 .. maxflux)
+  // initRFlux (src/PDFactory.cc:361-414): negative fluxes wrap to the top
   double minflux = s.minRFlux, maxflux = s.maxRFlux;
   if (maxflux < minflux) std::swap(minflux, maxflux);
   if (!(maxflux > 0))
     return fail(c, POFD_E_ARG, "PDFactory::initPD: Invalid (non-positive) maxflux");
   if (minflux > 0) minflux = 0.0;
-  if (minflux != 0.0)
-    return fail(c, POFD_E_ARG,
-                "pofd1d_prepare: negative R fluxes are not supported by the "
-                "device path");
   const double inm1 = 1.0 / static_cast<double>(N - 1);
-  const int wrapRidx = static_cast<int>(N - 1);
-  const double minflux_R = 0.0;
-  const double dflux = maxflux * inm1;
-
-  // initRInterp node fluxes (positive part)
-  const double ininterpm1 = 1.0 / static_cast<double>(ninterp - 1);
-  const double mininterpflux = c->knots[0] * 1e-5;  // subedgemult
-  double maxinterpflux = c->knots[nk - 1] * bm.pos.back();
-  const double dinterp = 0.001 * (maxinterpflux - mininterpflux) * ininterpm1;
-  maxinterpflux -= dinterp;
-  const double dinterpflux = (log2(maxinterpflux / mininterpflux)) * ininterpm1;
-  std::vector<double> xnode(ninterp);
-  for (uint32_t i = 0; i < ninterp; ++i)
-    xnode[i] = mininterpflux * exp2(static_cast<double>(i) * dinterpflux);
-  for (uint32_t i = 1; i < ninterp; ++i)
-    if (!(xnode[i] > xnode[i - 1]))
-      return fail(c, POFD_E_ARG, "PDFactory::initRInterp: interpolation "
-                                 "fluxes not increasing");
-
-  // beam entries: histogram (weights, inverse bin means) or raw inverse
-  std::vector<double> bw, bieta;
-  if (bm.is_pos_hist) {
-    bw = bm.posw;
-    bieta = bm.posv;
+  int wrapRidx;
+  double minflux_R, dflux;
+  if (minflux == 0) {
+    wrapRidx = static_cast<int>(N - 1);
+    minflux_R = 0.0;
+    dflux = maxflux * inm1;
   } else {
-    bw.assign(bm.posinv.size(), 1.0);
-    bieta = bm.posinv;
+    dflux = (maxflux - minflux) * inm1;
+    dflux = maxflux / (N - floor(-minflux / dflux) - 1.0);
+    const double wr = maxflux / dflux + 0.9999999999;
+    if (!(wr >= 1.0 && wr < static_cast<double>(N)))
+      return fail(c, POFD_E_ARG, "PDFactory::initRFlux: R flux range too "
+                                 "lopsided for the transform size");
+    wrapRidx = static_cast<int>(static_cast<unsigned>(wr));
+    minflux_R = (wrapRidx + 1) * dflux - static_cast<double>(N) * dflux;
   }
+  // fluxes of grid points above wrapRidx: i * dflux + mval (initR)
+  const double mval = minflux_R - static_cast<double>(wrapRidx + 1) * dflux;
+
+  // initRInterp node fluxes (src/PDFactory.cc:426-527): segment 0 the
+  // positive beam, segment 1 the negative beam (ascending fluxes < 0)
+  const double ininterpm1 = 1.0 / static_cast<double>(ninterp - 1);
+  const double subedgemult = 1e-5;  // include/PDFactory.h:29
+  const int nseg = hasneg ? 2 : 1;
+  std::vector<double> xnode(static_cast<size_t>(nseg) * ninterp, 0.0);
+  if (haspos) {
+    const double mininterpflux = minknot * subedgemult;
+    double maxinterpflux = maxknot * bm.pos.back();
+    const double dinterp = 0.001 * (maxinterpflux - mininterpflux) * ininterpm1;
+    maxinterpflux -= dinterp;
+    const double dinterpflux = (log2(maxinterpflux / mininterpflux)) * ininterpm1;
+    for (uint32_t i = 0; i < ninterp; ++i)
+      xnode[i] = mininterpflux * exp2(static_cast<double>(i) * dinterpflux);
+  } else {  // unused segment, kept increasing for the tables below
+    for (uint32_t i = 0; i < ninterp; ++i) xnode[i] = 1.0 + i;
+  }
+  if (hasneg) {
+    double maxinterpflux = -minknot * subedgemult;
+    double mininterpflux = -maxknot * bm.neg.back();
+    const double dinterp = 0.001 * (maxinterpflux - mininterpflux) * ininterpm1;
+    mininterpflux += dinterp;
+    maxinterpflux -= dinterp;
+    const double dinterpflux = (log2(mininterpflux / maxinterpflux)) * ininterpm1;
+    for (uint32_t i = 0; i < ninterp; ++i)
+      xnode[ninterp + i] =
+          maxinterpflux * exp2(static_cast<double>(ninterp - i - 1) * dinterpflux);
+  }
+  for (int g = 0; g < nseg; ++g)
+    for (uint32_t i = 1; i < ninterp; ++i)
+      if (!(xnode[g * ninterp + i] > xnode[g * ninterp + i - 1]))
+        return fail(c, POFD_E_ARG, "PDFactory::initRInterp: interpolation "
+                                   "fluxes not increasing");
+
+  // beam entries per segment: histogram (weights, inverse bin means) or raw
+  // inverse pixel values with weight 1
+  std::vector<double> bw, bieta;
+  auto add_entries = [&](bool ishist, const std::vector<double>& w,
+                         const std::vector<double>& v, const std::vector<double>& inv) {
+    if (ishist) {
+      bw.insert(bw.end(), w.begin(), w.end());
+      bieta.insert(bieta.end(), v.begin(), v.end());
+    } else {
+      bw.insert(bw.end(), inv.size(), 1.0);
+      bieta.insert(bieta.end(), inv.begin(), inv.end());
+    }
+  };
+  if (haspos) add_entries(bm.is_pos_hist, bm.posw, bm.posv, bm.posinv);
   const size_t nbeam = bw.size();
+  if (hasneg) add_entries(bm.is_neg_hist, bm.negw, bm.negv, bm.neginv);
+  const size_t nbeamn = bw.size() - nbeam;
   // knot-interval table: for every (beam entry, node) the knot interval of
-  // log2(flux * inverse beam) and its offset, or -1 where getR skips it
-  const double minknot = c->knots[0], maxknot = c->knots[nk - 1];
-  const double rmax = maxknot * bm.pos.back();  // getRRangeInternal().second
-  std::vector<int8_t> rk(nbeam * ninterp, -1);
-  std::vector<double> rdelx(nbeam * ninterp, 0.0);
-  for (size_t j = 0; j < nbeam; ++j)
+  // log2(|flux| * inverse beam) and its offset, or -1 where getR skips it
+  // (array getR, src/numberCountsKnotsSpline.cc:436-529)
+  std::vector<int8_t> rk(bw.size() * ninterp, -1);
+  std::vector<double> rdelx(bw.size() * ninterp, 0.0);
+  for (size_t j = 0; j < bw.size(); ++j) {
+    const bool negseg = j >= nbeam;
+    const double* xs = xnode.data() + (negseg ? ninterp : 0);
     for (uint32_t i = 0; i < ninterp; ++i) {
-      const double fd = xnode[i];
-      if (!(fd > 0 && fd <= rmax)) continue;
-      const double currarg = fd * bieta[j];
+      const double fd = xs[i];
+      if (negseg ? !(fd < 0 && fd >= rmin) : !(fd > 0 && fd <= rmax)) continue;
+      const double currarg = (negseg ? -fd : fd) * bieta[j];
       if (currarg < minknot || currarg >= maxknot) continue;
       const double lx = log2(currarg);
       const size_t k = bsearch_idx(c->logknots.data(), nk, lx);
       rk[j * ninterp + i] = static_cast<int8_t>(k);
       rdelx[j * ninterp + i] = lx - c->logknots[k];
     }
-  // fixed LDL^T factors of the node spline system (GSL cspline_init +
-  // gsl_linalg_solve_symm_tridiag)
+  }
+  // fixed LDL^T factors of each segment's node spline system (GSL
+  // cspline_init + gsl_linalg_solve_symm_tridiag)
   const uint32_t sys = ninterp - 2;
-  std::vector<double> ginv(ninterp - 1), diag(sys), off(sys), lalpha(sys),
-      lgamma(sys, 0.0);
-  for (uint32_t i = 0; i + 1 < ninterp; ++i) {
-    const double h = xnode[i + 1] - xnode[i];
-    ginv[i] = (h != 0.0) ? 1.0 / h : 0.0;
+  std::vector<double> ginv(nseg * (ninterp - 1)), lalpha(nseg * sys), lgamma(nseg * sys, 0.0);
+  for (int g = 0; g < nseg; ++g) {
+    const double* xs = xnode.data() + g * ninterp;
+    double* gi = ginv.data() + g * (ninterp - 1);
+    double* la = lalpha.data() + g * sys;
+    double* lg = lgamma.data() + g * sys;
+    std::vector<double> diag(sys), off(sys);
+    for (uint32_t i = 0; i + 1 < ninterp; ++i) {
+      const double h = xs[i + 1] - xs[i];
+      gi[i] = (h != 0.0) ? 1.0 / h : 0.0;
+    }
+    for (uint32_t i = 0; i < sys; ++i) {
+      const double h_i = xs[i + 1] - xs[i];
+      const double h_ip1 = xs[i + 2] - xs[i + 1];
+      off[i] = h_ip1;
+      diag[i] = 2.0 * (h_ip1 + h_i);
+    }
+    la[0] = diag[0];
+    lg[0] = off[0] / la[0];
+    for (uint32_t i = 1; i + 1 < sys; ++i) {
+      la[i] = diag[i] - off[i - 1] * lg[i - 1];
+      lg[i] = off[i] / la[i];
+    }
+    if (sys > 1) la[sys - 1] = diag[sys - 1] - off[sys - 2] * lg[sys - 2];
   }
-  for (uint32_t i = 0; i < sys; ++i) {
-    const double h_i = xnode[i + 1] - xnode[i];
-    const double h_ip1 = xnode[i + 2] - xnode[i + 1];
-    off[i] = h_ip1;
-    diag[i] = 2.0 * (h_ip1 + h_i);
+  // initR fill ranges (src/PDFactory.cc:560-613); empty ranges are [1, 0]
+  const bool fillpos = haspos && (maxflux > 0);
+  const bool fillneg = hasneg && (minflux < 0) && (wrapRidx > 0);
+  long minit = 1, maxit = 0, minitn = 1, maxitn = 0;
+  if (fillpos) {
+    const double minI = xnode[0], maxI = xnode[ninterp - 1];
+    minit = static_cast<long>(static_cast<unsigned>(minI / dflux + 0.9999999999999999));
+    maxit = static_cast<long>(static_cast<unsigned>(maxI / dflux));
+    if (maxit > wrapRidx) maxit = wrapRidx;
   }
-  lalpha[0] = diag[0];
-  lgamma[0] = off[0] / lalpha[0];
-  for (uint32_t i = 1; i + 1 < sys; ++i) {
-    lalpha[i] = diag[i] - off[i - 1] * lgamma[i - 1];
-    lgamma[i] = off[i] / lalpha[i];
+  if (fillneg) {
+    const double minI = xnode[ninterp], maxI = xnode[2 * ninterp - 1];
+    const double lo = (minI - minflux_R) / dflux + wrapRidx + 1 + 0.9999999999999999;
+    const double hi = (maxI - minflux_R) / dflux + wrapRidx + 1;
+    minitn = (lo > 0) ? static_cast<long>(static_cast<unsigned>(lo)) : 0;
+    if (minitn <= wrapRidx) minitn = wrapRidx + 1;
+    maxitn = (hi > 0) ? static_cast<long>(static_cast<unsigned>(hi)) : -1;
+    if (maxitn > static_cast<long>(N) - 1) maxitn = N - 1;
   }
-  if (sys > 1) lalpha[sys - 1] = diag[sys - 1] - off[sys - 2] * lgamma[sys - 2];
-  // initR fill range
-  const double minI = xnode[0], maxI = xnode[ninterp - 1];
-  unsigned minitidx = static_cast<unsigned>(minI / dflux + 0.9999999999999999);
-  unsigned maxitidx = static_cast<unsigned>(maxI / dflux);
-  if (maxitidx > static_cast<unsigned>(wrapRidx)) maxitidx = wrapRidx;
   // node interval of every flux of the grid (what gsl_interp_accel_find
-  // returns for it in initR)
-  std::vector<uint16_t> kst(N);
+  // returns for it in initR), as segment * ninterp + interval
+  std::vector<uint16_t> kst(N, 0);
   for (uint32_t b = 0; b < N; ++b) {
-    const double cf = static_cast<double>(b) * dflux;
-    kst[b] = static_cast<uint16_t>(bsearch_idx(xnode.data(), ninterp, cf));
+    if (static_cast<int>(b) <= wrapRidx) {
+      if (!fillpos) continue;
+      const double cf = static_cast<double>(b) * dflux;
+      kst[b] = static_cast<uint16_t>(bsearch_idx(xnode.data(), ninterp, cf));
+    } else {
+      if (!fillneg) continue;
+      const double cf = static_cast<double>(b) * dflux + mval;
+      kst[b] = static_cast<uint16_t>(ninterp + bsearch_idx(xnode.data() + ninterp, ninterp, cf));
+    }
   }
   const uint32_t M = N / 2, M2 = M / pofd::kM1;
 
@@ -314,11 +385,17 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   d.ninterp = static_cast<int>(ninterp);
   d.nknots = static_cast<int>(nk);
   d.nbeam = static_cast<int>(nbeam);
-  d.minit = static_cast<int>(minitidx);
-  d.maxit = static_cast<int>(maxitidx);
+  d.nbeamn = static_cast<int>(nbeamn);
+  d.nseg = nseg;
+  d.haspos = haspos ? 1 : 0;
+  d.minit = static_cast<int>(minit);
+  d.maxit = static_cast<int>(maxit);
+  d.minitn = static_cast<int>(minitn);
+  d.maxitn = static_cast<int>(maxitn);
   d.wrapRidx = wrapRidx;
   d.dflux = dflux;
   d.minflux_R = minflux_R;
+  d.mval = mval;
   d.logfac = log2(10.0);
   double prefac = bm.pixsize / 3600.0;
   d.prefac = prefac * prefac;
@@ -382,8 +459,8 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   const BeamsetDev& d = s.dev;
   WorkDev& w = s.wk;
   const size_t nf = static_cast<size_t>(d.nfields);
-  w.coef_stride = static_cast<size_t>(d.ninterp - 1) * 4;
-  w.ynode_stride = static_cast<size_t>(d.ninterp);
+  w.coef_stride = static_cast<size_t>(d.nseg) * d.ninterp * 4;
+  w.ynode_stride = static_cast<size_t>(d.nseg) * d.ninterp;
   // forward rows, then the noisy rows of each dataset
   w.zrows_stride = (1 + nf) * static_cast<size_t>(d.M);
   w.spec_stride = static_cast<size_t>(d.M) + 1;
@@ -761,24 +838,40 @@ extern "C" int pofd1d_get_spectrum(pofd_ctx* c, uint32_t walker, int b,
   return POFD_OK;
 }
 
-extern "C" int pofd1d_get_rinterp(pofd_ctx* c, uint32_t walker, int b,
-                                  double* flux_out, double* log2r_out) {
+namespace {
+int get_rinterp_seg(pofd_ctx* c, uint32_t walker, int b, int seg,
+                    double* flux_out, double* log2r_out, const char* who) {
   double rec[pofd::kDbg];
   int rc = dbg_record(c, walker, b, 0, rec);
   if (rc) return rc;
   Beamset& s = *c->sets[b];
+  if (seg == 0 ? !s.dev.haspos : s.dev.nseg < 2)
+    return fail(c, POFD_E_ARG, std::string(who) + ": the beam has no " +
+                                   (seg == 0 ? "positive" : "negative") + " pixels");
   const size_t n = static_cast<size_t>(s.dev.ninterp);
+  const size_t off = static_cast<size_t>(seg) * n;
   hipError_t e;
   if (flux_out) {
-    e = hipMemcpy(flux_out, s.dev.xnode, n * sizeof(double), hipMemcpyDeviceToHost);
+    e = hipMemcpy(flux_out, s.dev.xnode + off, n * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(xnode)");
   }
   if (log2r_out) {
-    e = hipMemcpy(log2r_out, s.wk.ynode + walker * s.wk.ynode_stride,
+    e = hipMemcpy(log2r_out, s.wk.ynode + walker * s.wk.ynode_stride + off,
                   n * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(ynode)");
   }
   return POFD_OK;
+}
+}  // namespace
+
+extern "C" int pofd1d_get_rinterp(pofd_ctx* c, uint32_t walker, int b,
+                                  double* flux_out, double* log2r_out) {
+  return get_rinterp_seg(c, walker, b, 0, flux_out, log2r_out, "pofd1d_get_rinterp");
+}
+
+extern "C" int pofd1d_get_rinterp_neg(pofd_ctx* c, uint32_t walker, int b,
+                                      double* flux_out, double* log2r_out) {
+  return get_rinterp_seg(c, walker, b, 1, flux_out, log2r_out, "pofd1d_get_rinterp_neg");
 }
 
 extern "C" int pofd1d_stage_times(pofd_ctx* c, double* ms_out,

@@ -676,6 +676,15 @@ int prepare_set2(pofd_ctx* c, Ctx2& C2, Set2& s) {
     }
     bc.maxf1 = maxknot * s.bm.max1[k];
   }
+  // the R kernel stages 3 doubles per element of a sign pair in LDS
+  for (int k = 0; k < 2; ++k) {
+    const size_t pair = static_cast<size_t>(d.comp[2 * k].n) + d.comp[2 * k + 1].n;
+    if (3 * pair * sizeof(double) > pofd2::kRgridMaxLds)
+      return ctx_fail(c, POFD_E_ARG,
+                      "pofd2d_prepare: beam has " + std::to_string(pair) +
+                          " elements in one band-1 sign; histogram it (at most " +
+                          std::to_string(pofd2::kRgridMaxLds / 24) + ")");
+  }
   // flux-grid tables
   std::vector<double> inv_m, lg_m, inv_e, lg_e;
   std::vector<int> sg_m, sg_e;
@@ -746,7 +755,7 @@ int ensure_work2(pofd_ctx* c, Ctx2& C2, Set2& s, uint32_t nwalk) {
   const size_t nc = n / 2 + 1;
   auto r8 = [](size_t x) { return (x + 7) & ~size_t(7); };
   w.R_stride = r8(n * n);
-  w.ex_stride = r8(std::max(ne, n) * std::max(ne, n));
+  w.ex_stride = r8(ne * n);        // nedge x n (ex) or n x nedge (ey)
   w.edge_stride = r8(std::max(n, ne));
   w.ystride = static_cast<int>(nc);
   w.Y_stride = r8(n * nc);       // complex

@@ -62,25 +62,31 @@ struct BeamsetDev {
   int ninterp;
   int nknots;
   int nbeam;    // beam entries (histogram bins or raw pixels), positive
-  int minit, maxit;       // fill range of R indices (PDFactory.cc:569-574)
-  int wrapRidx;
+  int nbeamn;   // beam entries of the negative beam (after the positive)
+  int nseg;     // node segments: 1, or 2 with the negative beam
+  int haspos;   // segment 0 holds a positive beam
+  int minit, maxit;       // positive fill range of R indices (PDFactory.cc:569-574)
+  int minitn, maxitn;     // negative (wrapped) fill range (:586-613)
+  int wrapRidx;           // last index of a non-negative flux
   double dflux;
   double minflux_R;
+  double mval;            // flux of index i > wrapRidx is i*dflux + mval
   double logfac;          // log2(10.0) as glibc computes it
   double prefac;          // (pixsize/3600)^2
   double maxsigma_base;
   double iflux;           // two_pi / (N * dflux)
   double spectral_cut;    // drop components with sigma^2 w^2/2 > cut
   double logknots[kMaxKnots];
-  const double* xnode;    // ninterp interpolation fluxes
-  const double* ginv;     // ninterp-1: 1/h_i  (GSL cspline_init g_i)
-  const double* lalpha;   // ninterp-2: LDL^T diagonal of the node system
-  const double* lgamma;   // ninterp-2: LDL^T sub-diagonal multipliers
-  const int8_t* rk;       // ninterp*nbeam: knot interval or -1 (skip)
-  const double* rdelx;    // ninterp*nbeam: log2(S) - logknots[k]
-  const double* bw;       // nbeam weights (1.0 for a raw beam)
-  const double* bieta;    // nbeam inverse beam values
-  const uint16_t* kst;    // N: node interval of every grid flux
+  // node tables, per segment (positive beam, then negative beam):
+  const double* xnode;    // nseg*ninterp interpolation fluxes
+  const double* ginv;     // nseg*(ninterp-1): 1/h_i  (GSL cspline_init g_i)
+  const double* lalpha;   // nseg*(ninterp-2): LDL^T diagonal of the node system
+  const double* lgamma;   // nseg*(ninterp-2): LDL^T sub-diagonal multipliers
+  const int8_t* rk;       // (nbeam+nbeamn)*ninterp: knot interval or -1 (skip)
+  const double* rdelx;    // (nbeam+nbeamn)*ninterp: log2|S| - logknots[k]
+  const double* bw;       // nbeam+nbeamn weights (1.0 for a raw beam)
+  const double* bieta;    // nbeam+nbeamn inverse |beam| values
+  const uint16_t* kst;    // N: segment*ninterp + node interval of every grid flux
   const double2* W256;    // 256: exp(-2 pi i s/256)
   const double2* Wlo;     // 256: exp(-2 pi i s/M)
   const double2* Wm2;     // M2: exp(-2 pi i s/M2)
@@ -90,8 +96,8 @@ struct BeamsetDev {
 
 // Per-walker device workspace layout (doubles), sized by the host.
 struct WorkDev {
-  double* coef;      // W * (ninterp-1) * 4 : y, b, c, d per interval
-  double* ynode;     // W * ninterp        : log2 R at the nodes
+  double* coef;      // W * nseg*ninterp*4 : y, b, c, d per interval and segment
+  double* ynode;     // W * nseg*ninterp   : log2 R at the nodes
   double2* zrows;    // W * (1+nfields) * M: forward rows, then the noisy
                      //   rows of each dataset (M complex each)
   double2* prtrans;  // W * (M+1)          : exp(R^ - R^0) with phase

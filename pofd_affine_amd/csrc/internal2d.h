@@ -19,6 +19,8 @@ constexpr int kLikeBlocks = 64;    // workgroups per (walker, dataset) in like2d
 // (4 coefficients per interval each), raw sigma and offset values
 constexpr int kModelStride = 8 + 14 * kMaxK;
 constexpr int kStats = 16;
+// dynamic LDS ceiling of the R-grid kernel (3 doubles per beam element)
+constexpr size_t kRgridMaxLds = 144 * 1024;
 constexpr int kDbg2 = 32;
 
 enum {  // stats slots (per walker)

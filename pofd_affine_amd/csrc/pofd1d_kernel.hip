@@ -364,93 +364,101 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
       }
     }
   }
-  // theta-independent factors of the node system into LDS
-  for (int i = tid; i < ninterp - 2; i += kThreads) {
-    sm.lg[i] = bs.lgamma[i];
-    sm.la[i] = bs.lalpha[i];
-  }
-  for (int i = tid; i < ninterp - 1; i += kThreads) sm.gi[i] = bs.ginv[i];
   __syncthreads();
   if (sm.isc[0] != POFD_OK) return;  // uniform across the workgroup
 
-  // ---- R at the interpolation nodes (initRInterp + array getR) -----------
-  {
-#pragma clang fp contract(off)
-    double* ynode_g = wk.ynode + walker * wk.ynode_stride;
-    for (int i = tid; i < ninterp; i += kThreads) {
-      double workval = 0.0;
-      for (int j = 0; j < bs.nbeam; ++j) {
-        const size_t e = static_cast<size_t>(j) * ninterp + i;
-        const int k = bs.rk[e];
-        if (k < 0) continue;
-        const double dx = bs.rdelx[e];
-        const double s =
-            sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
-        const double cts = exp2(s);
-        if (cts > 0) workval += bs.bw[j] * cts * bs.bieta[j];
-      }
-      const double R = bs.prefac * workval;
-      sm.yn[i] = log2_or_small(R);
-      ynode_g[i] = sm.yn[i];
-    }
-  }
-  __syncthreads();
-
-  // ---- natural spline over the nodes.  The tridiagonal matrix is fixed
-  // (theta-independent LDL^T factors from the host); forward and backward
-  // substitution run as truncated recurrences: the multipliers are < 1/2,
-  // so starting kWarm = 64 steps early changes no significant bit.
+  // one pass per node segment: the positive beam, then the negative beam
   const int sys = ninterp - 2;
-  {
+  for (int seg = bs.haspos ? 0 : 1; seg < bs.nseg; ++seg) {
+    const double* xnode = bs.xnode + seg * ninterp;
+    // theta-independent factors of the segment's node system into LDS
+    for (int i = tid; i < sys; i += kThreads) {
+      sm.lg[i] = bs.lgamma[seg * sys + i];
+      sm.la[i] = bs.lalpha[seg * sys + i];
+    }
+    for (int i = tid; i < ninterp - 1; i += kThreads) sm.gi[i] = bs.ginv[seg * (ninterp - 1) + i];
+
+    // ---- R at the interpolation nodes (initRInterp + array getR) ---------
+    {
 #pragma clang fp contract(off)
-    for (int i = tid; i < sys; i += kThreads)
-      sm.gR[i] = 3.0 * ((sm.yn[i + 2] - sm.yn[i + 1]) * sm.gi[i + 1] -
-                        (sm.yn[i + 1] - sm.yn[i]) * sm.gi[i]);
-    __syncthreads();
-    const int per = (sys + kThreads - 1) / kThreads;
-    const int lo = tid * per, hi = min(lo + per, sys);
-    if (lo < hi) {
-      int i0 = lo - kWarm;
-      double z;
-      if (i0 <= 0) {
-        i0 = 0;
-        z = sm.gR[0];
-      } else {
-        z = sm.gR[i0];
-      }
-      if (i0 >= lo) sm.zz[i0] = z;
-      for (int i = i0 + 1; i < hi; ++i) {
-        z = sm.gR[i] - sm.lg[i - 1] * z;
-        if (i >= lo) sm.zz[i] = z;
-      }
-    }
-    __syncthreads();
-    for (int i = lo; i < hi; ++i) sm.zz[i] = sm.zz[i] / sm.la[i];
-    __syncthreads();
-    if (lo < hi) {
-      int i1 = hi - 1 + kWarm;
-      if (i1 > sys - 1) i1 = sys - 1;
-      double x = sm.zz[i1];
-      if (i1 < hi) sm.xs[i1] = x;
-      for (int i = i1 - 1; i >= lo; --i) {
-        x = sm.zz[i] - sm.lg[i] * x;
-        if (i < hi) sm.xs[i] = x;
+      const int j0 = (seg == 0) ? 0 : bs.nbeam;
+      const int j1 = (seg == 0) ? bs.nbeam : bs.nbeam + bs.nbeamn;
+      double* ynode_g = wk.ynode + walker * wk.ynode_stride + seg * ninterp;
+      for (int i = tid; i < ninterp; i += kThreads) {
+        double workval = 0.0;
+        for (int j = j0; j < j1; ++j) {
+          const size_t e = static_cast<size_t>(j) * ninterp + i;
+          const int k = bs.rk[e];
+          if (k < 0) continue;
+          const double dx = bs.rdelx[e];
+          const double s =
+              sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
+          const double cts = exp2(s);
+          if (cts > 0) workval += bs.bw[j] * cts * bs.bieta[j];
+        }
+        const double R = bs.prefac * workval;
+        sm.yn[i] = log2_or_small(R);
+        ynode_g[i] = sm.yn[i];
       }
     }
     __syncthreads();
-    // per-interval coefficients (GSL coeff_calc), node c = xs shifted
-    double* coef = wk.coef + walker * wk.coef_stride;
-    for (int k = tid; k < ninterp - 1; k += kThreads) {
-      const double c_i = (k == 0) ? 0.0 : sm.xs[k - 1];
-      const double c_ip1 = (k + 1 == ninterp - 1) ? 0.0 : sm.xs[k];
-      const double dx = bs.xnode[k + 1] - bs.xnode[k];
-      const double dy = sm.yn[k + 1] - sm.yn[k];
-      double* cf = coef + 4 * k;
-      cf[0] = sm.yn[k];
-      cf[1] = (dy / dx) - dx * (c_ip1 + 2.0 * c_i) / 3.0;
-      cf[2] = c_i;
-      cf[3] = (c_ip1 - c_i) / (3.0 * dx);
+
+    // ---- natural spline over the nodes.  The tridiagonal matrix is fixed
+    // (theta-independent LDL^T factors from the host); forward and backward
+    // substitution run as truncated recurrences: the multipliers are < 1/2,
+    // so starting kWarm = 64 steps early changes no significant bit.
+    {
+#pragma clang fp contract(off)
+      for (int i = tid; i < sys; i += kThreads)
+        sm.gR[i] = 3.0 * ((sm.yn[i + 2] - sm.yn[i + 1]) * sm.gi[i + 1] -
+                          (sm.yn[i + 1] - sm.yn[i]) * sm.gi[i]);
+      __syncthreads();
+      const int per = (sys + kThreads - 1) / kThreads;
+      const int lo = tid * per, hi = min(lo + per, sys);
+      if (lo < hi) {
+        int i0 = lo - kWarm;
+        double z;
+        if (i0 <= 0) {
+          i0 = 0;
+          z = sm.gR[0];
+        } else {
+          z = sm.gR[i0];
+        }
+        if (i0 >= lo) sm.zz[i0] = z;
+        for (int i = i0 + 1; i < hi; ++i) {
+          z = sm.gR[i] - sm.lg[i - 1] * z;
+          if (i >= lo) sm.zz[i] = z;
+        }
+      }
+      __syncthreads();
+      for (int i = lo; i < hi; ++i) sm.zz[i] = sm.zz[i] / sm.la[i];
+      __syncthreads();
+      if (lo < hi) {
+        int i1 = hi - 1 + kWarm;
+        if (i1 > sys - 1) i1 = sys - 1;
+        double x = sm.zz[i1];
+        if (i1 < hi) sm.xs[i1] = x;
+        for (int i = i1 - 1; i >= lo; --i) {
+          x = sm.zz[i] - sm.lg[i] * x;
+          if (i < hi) sm.xs[i] = x;
+        }
+      }
+      __syncthreads();
+      // per-interval coefficients (GSL coeff_calc), node c = xs shifted
+      double* coef = wk.coef + walker * wk.coef_stride + 4 * seg * ninterp;
+      for (int k = tid; k < ninterp - 1; k += kThreads) {
+        const double c_i = (k == 0) ? 0.0 : sm.xs[k - 1];
+        const double c_ip1 = (k + 1 == ninterp - 1) ? 0.0 : sm.xs[k];
+        const double dx = xnode[k + 1] - xnode[k];
+        const double dy = sm.yn[k + 1] - sm.yn[k];
+        double* cf = coef + 4 * k;
+        cf[0] = sm.yn[k];
+        cf[1] = (dy / dx) - dx * (c_ip1 + 2.0 * c_i) / 3.0;
+        cf[2] = c_i;
+        cf[3] = (c_ip1 - c_i) / (3.0 * dx);
+      }
     }
+    __syncthreads();  // LDS reused by the next segment
   }
   if (tid == 0) st[ST_TPREP] = static_cast<double>(wall_clock64() - t0);
 }
@@ -513,18 +521,36 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
             cf = *reinterpret_cast<const double4*>(coef + 4 * k);
           }
           double rv = 0.0;
-          if (j >= bs.minit && j <= bs.maxit) {
-            const double cflux = static_cast<double>(j) * dflux;
+          const bool inneg = (j >= bs.minitn && j <= bs.maxitn);
+          if (inneg || (j >= bs.minit && j <= bs.maxit)) {
+            // grid points above wrapRidx hold the wrapped negative fluxes
+            const double cflux = inneg ? static_cast<double>(j) * dflux + bs.mval
+                                       : static_cast<double>(j) * dflux;
             const double delx = cflux - xk;
             const double s = cf.x + delx * (cf.y + delx * (cf.z + delx * cf.w));
             rv = exp2(s) * dflux;
           }
-          if (p0 == 0 && j >= 1 && j <= bs.wrapRidx) {
-            // getMeanVarFromR (positive R only: wrapRidx = N-1)
-            const double cfl = static_cast<double>(j) * dflux;
-            const double prod = (j == bs.wrapRidx) ? 0.5 * cfl * rv : cfl * rv;
+          if (p0 == 0 && j >= 1) {
+            // getMeanVarFromR (src/PDFactory.cc:625-679)
+            const int wr = bs.wrapRidx;
+            double prod, cfl;
+            if (j <= wr) {
+              cfl = static_cast<double>(j) * dflux;
+              prod = (j == wr) ? 0.5 * cfl * rv : cfl * rv;
+            } else {
+              cfl = static_cast<double>(j - wr - 1) * dflux + bs.minflux_R;
+              prod = cfl * rv;
+            }
             mn_part += prod;
             var_part += cfl * prod;
+            if (j == wr - 1 && wr < bs.N - 1) {
+              // the wrap correction reads R[wrapRidx - 1] (src/PDFactory.cc:
+              // 661-668), kept as the reference has it
+              const double cb = bs.minflux_R;
+              const double pb = 0.5 * cb * rv;
+              mn_part -= pb;
+              var_part -= cb * pb;
+            }
           }
           if (e & 1)
             sm.bufa[(4 * q + (e >> 1)) * kSlots + slot(row4)] = make_double2(rprev, rv);

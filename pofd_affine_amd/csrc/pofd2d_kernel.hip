@@ -1111,6 +1111,11 @@ int launch_loglike2d(const Set2Dev& s, const Work2& wk, const RGrid& gmain,
   int maxc = 0;
   for (int k = 0; k < 2; ++k) maxc = std::max(maxc, s.comp[2 * k].n + s.comp[2 * k + 1].n);
   const size_t rg_lds = static_cast<size_t>(3) * std::max(maxc, 1) * sizeof(double);
+  if (rg_lds > kRgridMaxLds) return POFD_E_ARG;
+  if (rg_lds + 8192 > 65536)  // + the static knot tables (4.3 KB)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(rgrid2d_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(rg_lds));
   const int rblocks = 64;  // row blocks per walker (grid-stride over rows)
   mark(POFD_STAGE_PREP);
   hipLaunchKernelGGL(prep2d_kernel, dim3(W), dim3(64), 0, st, s, wk, d_theta, stride, d_st,
@@ -1136,6 +1141,14 @@ int launch_loglike2d(const Set2Dev& s, const Work2& wk, const RGrid& gmain,
   hipLaunchKernelGGL(rstats2d_final_kernel, dim3((W + 63) / 64), dim3(64), 0, st, s, wk, nparts);
   const size_t lds_row = static_cast<size_t>(n) * sizeof(double2);      // 2 x n/2
   const size_t lds_col = static_cast<size_t>(2 * n) * sizeof(double2);  // 2 x n
+  if (lds_row + 1024 > 65536) {  // n = 4096: 64 KB plus the static reduction slots
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(rowfft2d_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(lds_row));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(rowifft2d_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(lds_row));
+  }
   if (lds_col > 65536) {  // n = 4096: 128 KB of the CU's 160 KB
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(colfft2d_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize,

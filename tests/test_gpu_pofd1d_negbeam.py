@@ -1,0 +1,107 @@
+"""GPU parity of the 1D likelihood for a beam with negative pixels against
+the CPU oracle.
+
+Negative beam pixels make R non-zero below zero flux; the reference wraps
+those fluxes to the top of the transform grid (PDFactory::initRFlux,
+src/PDFactory.cc:361-414) and interpolates them with a second node spline
+(initRInterp, :486-524).  The beam is the reference's own two-signed test
+beam (test/beam_test.cc:151-172, band12_beam_f100.fits[0]); the data are the
+small synthetic map of the positive-beam tests.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REFDATA
+from oracle_side import build_oracle
+from pofd_affine_amd import fitsio, synth, workload
+
+pytestmark = pytest.mark.gpu
+
+NF_S, NI_S = 32768, 512
+
+
+def _say(msg):
+    print("[gpu-negbeam-case] " + msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def negbeam():
+    pix, ps = fitsio.read_beam(os.path.join(REFDATA,
+                                            "band12_beam_f100.fits[0]"))
+    inp = dict(workload.make_inputs(npix_side=200, seed=7), beam=pix,
+               pixsize=ps)
+    th = synth.walker_thetas(6, seed=13)
+    out = {}
+    for binned in (True, False):
+        _say("build negative-beam context binned=%s" % binned)
+        ctx = workload.build_context(inp, NF_S, NI_S, binned=binned)
+        L = build_oracle(inp, NF_S, NI_S, binned, th[0])
+        ll, st = ctx.loglike_batch(th)
+        out[binned] = (ctx, L, ll, st)
+    return inp, th, out
+
+
+def test_negbeam_setup_matches(negbeam):
+    inp, th, out = negbeam
+    ctx, L, _, _ = out[True]
+    gi = ctx.beamset_info(0)
+    oi = L.beamset_info(0)
+    for k in ("minRFlux", "maxRFlux", "maxsigma_base", "minDataFlux",
+              "maxDataFlux"):
+        assert gi[k] == oi[k], k
+    assert gi["minRFlux"] < 0
+
+
+@pytest.mark.parametrize("binned", [True, False])
+def test_negbeam_loglike(negbeam, binned):
+    inp, th, out = negbeam
+    ctx, L, ll, st = out[binned]
+    _say("negative-beam loglike binned=%s" % binned)
+    ref, rst = L.loglike_batch(th, 6)
+    assert np.all(rst == 0) and np.all(st == 0), (st, rst)
+    np.testing.assert_allclose(ll, ref, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("w", [0, 4])
+def test_negbeam_nodes_moments_spectrum(negbeam, w):
+    inp, th, out = negbeam
+    ctx, L, _, _ = out[True]
+    _say("negative-beam nodes/moments walker %d" % w)
+    ip = L.initpd(0, th[w])
+    assert ip["minflux_R"] < 0
+    x, y = ctx.get_rinterp(w)
+    np.testing.assert_array_equal(x, ip["rinterp_flux"])
+    np.testing.assert_allclose(y, ip["rinterp_vals"], rtol=0, atol=1e-12)
+    xn, yn = ctx.get_rinterp_neg(w)
+    rf, rv = L.rinterp_neg()
+    np.testing.assert_array_equal(xn, rf)
+    np.testing.assert_allclose(yn, rv, rtol=0, atol=1e-12)
+    dbg = ctx.get_debug(w)
+    # the positive and negative halves of the sums partly cancel
+    assert dbg["mn"] == pytest.approx(ip["mn"], rel=1e-9)
+    assert dbg["var_noi"] == pytest.approx(ip["var_noi"], rel=1e-12)
+    assert dbg["r0"] == pytest.approx(ip["r0"], rel=1e-13)
+    spec = ctx.get_spectrum(w)
+    kmax = int(dbg["kmax_beamset"])
+    np.testing.assert_allclose(spec, ip["prtrans"][:kmax + 1], rtol=0,
+                               atol=1e-12)
+
+
+@pytest.mark.parametrize("w", [0, 3])
+def test_negbeam_pd(negbeam, w):
+    inp, th, out = negbeam
+    ctx, L, _, _ = out[True]
+    _say("negative-beam pd walker %d" % w)
+    L.initpd(0, th[w])
+    ref = L.getpd(0, 0, float(th[w][8]), setlog=False)
+    raw = ctx.get_pd(w, is_log="raw")[0]
+    assert np.max(np.abs(raw - ref["pofd_raw"])) <= 1e-10 * ref["peak"]
+    dbg = ctx.get_debug(w)
+    assert int(dbg["split"]) == ref["split"]
+    pdl, mf, df = ctx.get_pd(w, is_log=False)
+    assert df == ref["dflux"]
+    assert abs(mf - ref["minflux"]) < 1e-9 * df
+    np.testing.assert_allclose(pdl, ref["pd"], rtol=0,
+                               atol=1e-10 * ref["pd"].max())

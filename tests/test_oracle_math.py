@@ -189,3 +189,85 @@ def test_pd_pipeline_direct_dft_small_n():
     got = L.getpd(0, 0, 1.0, setlog=False)["pofd_raw"]
     ref = np.where(pofd < 0, 0.0, pofd)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * ref.max())
+
+
+# ---- negative beams: the wrapped R grid (src/PDFactory.cc:361-414, 541-620)
+def _negbeam_setup(n, ninterp, sigma, exp_conf=0.0, nhist=120, seed=7):
+    import os
+    from conftest import REFDATA
+    from pofd_affine_amd import fitsio
+    pix, ps = fitsio.read_beam(os.path.join(REFDATA,
+                                            "band12_beam_f100.fits[0]"))
+    beam = O.Beam(pix, ps, 1e-6, nhist)
+    d = np.random.default_rng(seed).normal(0.0, sigma, 20000)
+    d -= d.mean()
+    L = O.Like(n, ninterp, synth.knot_positions())
+    b = L.add_beamset(beam)
+    L.add_field(b, pix=d, sigma_base=sigma)
+    th = synth.true_theta()
+    L.finalize(th, exp_conf=exp_conf)
+    return L, th, beam
+
+
+def _grid_flux(ip, n):
+    i = np.arange(n)
+    wr, dflux = ip["wrapRidx"], ip["dflux"]
+    return np.where(i <= wr, i * dflux, (i - wr - 1) * dflux + ip["minflux_R"])
+
+
+def test_negative_beam_wrapped_grid_is_getR():
+    """Grid points above wrapRidx hold R at the negative fluxes
+    (i - wrapRidx - 1) dflux + minflux_R; on both sides the filled values are
+    the spline-interpolated array getR (itself pinned by the reference's
+    negative-beam golden vectors) at those fluxes."""
+    n = 32768
+    L, th, beam = _negbeam_setup(n, 2048, 0.0025)
+    ip = L.initpd(0, th)
+    wr = ip["wrapRidx"]
+    assert ip["minflux_R"] < 0 and 0 < wr < n - 1
+    # the wrapped fluxes continue the positive ones around the circle
+    x = _grid_flux(ip, n)
+    assert x[wr + 1] == pytest.approx(x[wr] + ip["dflux"] - n * ip["dflux"],
+                                      rel=1e-9)
+    m = O.Knots(synth.knot_positions(), th[:len(synth.knot_positions())])
+    r = ip["rvals"] / ip["dflux"]
+    ref = m.getR(x, beam)
+    for side in (x > 0, x < 0):
+        sel = side & (ref > 1e-8 * ref[side].max())
+        assert sel.sum() > 100
+        rel = np.abs(r[sel] - ref[sel]) / ref[sel]
+        # the tails where beam entries leave the knot range interpolate worst
+        assert np.median(rel) < 1e-4 and np.percentile(rel, 90) < 2e-2, (
+            np.median(rel), np.percentile(rel, 90))
+    # a one-cell shift of the wrapped fluxes would be caught
+    neg = (x < 0) & (ref > 1e-8 * ref[x < 0].max())
+    shifted = m.getR(x[neg] + ip["dflux"], beam)
+    ok = shifted > 0
+    assert np.median(np.abs(r[neg][ok] - shifted[ok]) / shifted[ok]) > 1e-3
+
+
+def test_negative_beam_pd_cumulants_campbell():
+    """With a beam of both signs the normalised P(D) has variance
+    sum x^2 R dx + sigma^2 and third cumulant sum x^3 R dx over the wrapped
+    grid fluxes (Campbell)."""
+    sigma = 0.0025
+    n = 32768
+    L, th, _ = _negbeam_setup(n, 512, sigma, exp_conf=0.2)
+    ip = L.initpd(0, th)
+    pd = L.getpd(0, 0, 1.0, setlog=False)
+    p = pd["pd"]
+    x = pd["minflux"] + np.arange(p.size) * pd["dflux"]
+    w = p * pd["dflux"]
+    m1 = np.sum(x * w)
+    k2 = np.sum((x - m1) ** 2 * w)
+    k3 = np.sum((x - m1) ** 3 * w)
+    assert abs(m1) < 1e-6
+    xr = _grid_flux(ip, n)
+    k2_campbell = np.sum(xr ** 2 * ip["rvals"]) + sigma * sigma
+    k3_campbell = np.sum(xr ** 3 * ip["rvals"])
+    assert k2 == pytest.approx(k2_campbell, rel=1e-8)
+    assert k3 == pytest.approx(k3_campbell, rel=1e-6)
+    # the negative lobe matters: dropping it changes the third cumulant
+    pos = xr > 0
+    assert abs(np.sum(xr[pos] ** 3 * ip["rvals"][pos]) - k3_campbell) > \
+        1e-4 * abs(k3_campbell)
