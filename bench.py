@@ -46,6 +46,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pcie-steps", type=int, default=10,
                     help="steps through the host-buffer entry point")
+    ap.add_argument("--steps-2d", type=int, default=10,
+                    help="timed 2D batches (0: skip the 2D leg)")
+    ap.add_argument("--walkers-2d", type=int, default=32)
+    ap.add_argument("--n-2d", type=int, default=2048)
+    ap.add_argument("--cpu-evals-2d", type=int, default=16)
+    ap.add_argument("--only-2d", action="store_true",
+                    help="run the 2D leg alone (for profiling it)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (calibrates "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -71,6 +78,10 @@ def main():
                 raise RuntimeError("calibration copy failed")
         hiprt.device_synchronize()
         del src, dst
+
+    if args.only_2d:
+        print(json.dumps({"twod": bench_2d(args, hiprt)}))
+        return
 
     W, N = args.walkers, args.nflux
     inp = workload.make_inputs(npix_side=args.side, seed=42)
@@ -200,6 +211,8 @@ def main():
                                  "ms_per_step": 1e3 * dt}
     if args.copy_calib > 0:
         out["copy_calib_bytes"] = (args.copy_calib // 16) * 16
+    if args.steps_2d > 0:
+        out["twod"] = bench_2d(args, hiprt)
     if not args.no_cpu and args.cpu_evals > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -221,6 +234,90 @@ def main():
         out["parity_max_abs_dlogL"] = float(
             np.max(np.abs(ref[ok] - b["ll"][:ne][ok]))) if ok.any() else None
     print(json.dumps(out))
+
+
+def bench_2d(args, hiprt):
+    """The 2D leg: calcLikeDouble::getLogLike for a batch of walkers on a
+    n x n grid (2048^2 FP64), 1000x1000 synthetic pixel pairs (unbinned),
+    Gaussian beams of FWHM 18.1" and 25.2" at 6" in a 120x120-bin 2D
+    histogram, sigma 2.5 / 2.7 mJy, 32 walkers per launch."""
+    from pofd_affine_amd import synth2d, workload2d
+    n, W = args.n_2d, args.walkers_2d
+    b1, b2, pix = synth2d.gaussian_beams()
+    d1, d2, nsrc = synth2d.make_maps(args.side, seed=99, beams=(b1, b2, pix))
+    inp = dict(beam1=b1, beam2=b2, pixsize=pix, d1=d1, d2=d2,
+               minbeamval=1e-6, nbeamhist=120, nsrc=nsrc, nbins=None)
+    th = synth2d.walker_thetas(W, seed=4321)
+    stride = th.shape[1]
+    ctx = workload2d.build_context2d(inp, n, synth2d.true_theta())
+    d_th = hiprt.DeviceBuffer(th.nbytes)
+    d_th.upload(th)
+    d_ll = hiprt.DeviceBuffer(W * 8)
+    d_st = hiprt.DeviceBuffer(W * 4)
+    stream = hiprt.Stream()
+    for _ in range(2):
+        ctx.loglike2d_batch_device(d_th.value, W, stride, d_ll.value,
+                                   d_st.value, stream.ptr.value)
+    stream.synchronize()
+    steps = args.steps_2d
+    evs = [hiprt.Event() for _ in range(steps + 1)]
+    ctx.set_stage_timing(True)
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for i in range(steps):
+        ctx.loglike2d_batch_device(d_th.value, W, stride, d_ll.value,
+                                   d_st.value, stream.ptr.value)
+        evs[i + 1].record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    stage_ms, nl = ctx.stage_times()
+    stage_ms = {k: v / max(nl, 1) for k, v in stage_ms.items()}
+    ctx.set_stage_timing(False)
+    per = [evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)]
+    ll = np.zeros(W)
+    st = np.zeros(W, dtype=np.int32)
+    d_ll.download(ll)
+    d_st.download(st)
+    batch_ms = float(np.mean(per))
+    # algorithmic bytes of one evaluation: the three global stage
+    # boundaries (R -> forward transform, spectrum -> inverse transform,
+    # P(D) -> split/normalise/logL), each a write and a read of 8 n^2
+    # bytes, plus the pixel pairs amortised over the batch
+    b_eval = 48.0 * n * n + 16.0 * d1.size / W
+    achieved = b_eval * W / (batch_ms * 1e-3) / 1e9
+    res = {
+        "value": W / (wall / steps), "unit": "evals/s",
+        "workload": "2D P(D) log likelihood, %dx%d grid, %d walkers per "
+                    "launch, %dx%d synthetic pixel pairs (unbinned), "
+                    "120x120-bin 2D beam histogram, sigma 2.5/2.7 mJy"
+                    % (n, n, W, args.side, args.side),
+        "steps": steps, "batch_ms": batch_ms, "stage_ms": stage_ms,
+        "status_ok": int(np.sum(st == 0)),
+        "pipeline": {"bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "bytes_alg_per_eval": b_eval},
+    }
+    if not args.no_cpu and args.cpu_evals_2d > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle2d_side import build_oracle2d
+        ne = min(args.cpu_evals_2d, W)
+        nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        L = build_oracle2d(inp, n, synth2d.true_theta())
+        t0 = time.perf_counter()
+        ref, rst = L.loglike_batch(th[:ne], nthr)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {
+            "value": ne / dt, "unit": "evals/s", "cores": nthr,
+            "kind": "port",
+            "sample": "%d walkers of the same 2D workload on %d host "
+                      "threads, one evaluation per thread at a time; own "
+                      "radix-2 FFT (FFTW is absent)" % (ne, nthr)}
+        ok = (rst == 0) & (st[:ne] == 0)
+        res["parity_max_abs_dlogL"] = float(
+            np.max(np.abs(ref[ok] - ll[:ne][ok]))) if ok.any() else None
+    return res
 
 
 if __name__ == "__main__":

@@ -195,7 +195,12 @@ int pofd1d_get_rinterp_neg(pofd_ctx *ctx, uint32_t walker, int beamset_id,
    summed GPU milliseconds of each stage (ms_out[POFD_NSTAGES]) over every
    beam-set launch since the previous call, *nlaunch the number of such
    launches, and starts a new tally.  Up to 4096 launches are recorded per
-   tally.  No reference counterpart (see POFD_OPT_STAGE_TIMING). */
+   tally.  Two-band launches (pofd2d_loglike_batch*) enter the same tally
+   with the stages: prep = model set-up; forward = R on the grid and its
+   edges, moments, forward transforms and spectrum; inverse = noise factor
+   and inverse transforms; finalize = split, rotation, normalisation;
+   pixels = log likelihood; combine.  No reference counterpart (see
+   POFD_OPT_STAGE_TIMING). */
 int pofd1d_stage_times(pofd_ctx *ctx, double *ms_out, uint32_t *nlaunch);
 
 /* ======================================================================
@@ -278,7 +283,8 @@ int pofd2d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
 /* ---- parity and debugging (PDFactoryDouble::getPD output) ---- */
 /* info[32]: status, split1, split2, sg1pos, sg2pos, sigma1, sigma2,
    integral, 1/integral, minflux1, minflux2, dataset logL, mn1, mn2, var1,
-   var2, r0, K1, K2 (spectral bounds kept) */
+   var2, r0, K1, K2 (spectral bounds kept), split status of band 1 and of
+   band 2 */
 int pofd2d_get_debug(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                      int field_id, double *info);
 /* pd_out[n*n]: rotated, normalised P(D) (log2 when is_log) of the last

@@ -429,7 +429,7 @@ class Context2(Context):
     DEBUG_KEYS = ["status", "split1", "split2", "sg1pos", "sg2pos", "sigma1",
                   "sigma2", "integral", "itot", "minflux1", "minflux2",
                   "field_loglike", "mn1", "mn2", "var1", "var2", "r0", "K1",
-                  "K2"]
+                  "K2", "split_status1", "split_status2"]
 
     def configure2d(self, n, knots, sigmapos, offsetpos, nedge=256,
                     edge_integ=True, min_sigma=0.0, exp_conf1=0.0,

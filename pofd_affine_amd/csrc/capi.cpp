@@ -102,11 +102,28 @@ int ctx_fail(pofd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
 }
+// the next set of POFD_NSTAGES+1 stage events while stage timing is on
+// (nullptr when it is off or the tally is full)
+int ctx_stage_events(pofd_ctx* c, hipEvent_t** ev) {
+  constexpr size_t kMaxTimedLaunches = 4096;
+  *ev = nullptr;
+  if (!c->stage_timing || c->stage_used >= kMaxTimedLaunches) return POFD_OK;
+  if (c->stage_used == c->stage_events.size()) {
+    std::array<hipEvent_t, POFD_NSTAGES + 1> fresh{};
+    for (auto& e : fresh) {
+      hipError_t he = hipEventCreate(&e);
+      if (he != hipSuccess)
+        return ctx_fail(c, POFD_E_INTERNAL,
+                        std::string("pofd::hip: hipEventCreate: ") + hipGetErrorString(he));
+    }
+    c->stage_events.push_back(fresh);
+  }
+  *ev = c->stage_events[c->stage_used++].data();
+  return POFD_OK;
+}
 }  // namespace pofd2
 
 namespace {
-
-constexpr size_t kMaxTimedLaunches = 4096;
 
 int fail(pofd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -717,17 +734,7 @@ extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
     Beamset& s = *c->sets[b];
     if ((rc = ensure_workspace(c, s, nwalk))) return rc;
     hipEvent_t* ev = nullptr;
-    if (c->stage_timing && c->stage_used < kMaxTimedLaunches) {
-      if (c->stage_used == c->stage_events.size()) {
-        std::array<hipEvent_t, POFD_NSTAGES + 1> fresh{};
-        for (auto& e : fresh) {
-          hipError_t he = hipEventCreate(&e);
-          if (he != hipSuccess) return hip_fail(c, he, "hipEventCreate");
-        }
-        c->stage_events.push_back(fresh);
-      }
-      ev = c->stage_events[c->stage_used++].data();
-    }
+    if ((rc = pofd2::ctx_stage_events(c, &ev))) return rc;
     rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta,
                                 nwalk, stride, d_ll, d_st, b > 0 ? 1 : 0,
                                 stream, ev);

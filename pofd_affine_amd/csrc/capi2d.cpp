@@ -783,6 +783,8 @@ int ensure_work2(pofd_ctx* c, Ctx2& C2, Set2& s, uint32_t nwalk) {
   const size_t o_rs2 = tot; tot += w.edge_stride * nf * W;
   const size_t o_rA = tot; tot += w.edge_stride * nf * W;
   const size_t o_rB = tot; tot += w.edge_stride * nf * W;
+  w.colpart_stride = r8(static_cast<size_t>(pofd2::kRowBlocks2) * n);
+  const size_t o_cp = tot; tot += w.colpart_stride * W;
   const size_t o_rsp = tot; tot += r8(nparts * 4) * W;
   const size_t o_lp = tot; tot += r8(pofd2::kLikeBlocks) * nf * W;
   const size_t o_dbg = tot; tot += r8(pofd2::kDbg2) * nf * W;
@@ -809,6 +811,7 @@ int ensure_work2(pofd_ctx* c, Ctx2& C2, Set2& s, uint32_t nwalk) {
   w.rsum2 = b + o_rs2;
   w.rowA = b + o_rA;
   w.rowB = b + o_rB;
+  w.colpart = b + o_cp;
   w.rstat_part = b + o_rsp;
   w.lpart = b + o_lp;
   w.dbg = b + o_dbg;
@@ -1048,6 +1051,28 @@ int pofd2d_add_field(pofd_ctx* c, int b, const double* d1, const double* d2, uin
       if (d2[i] < f.mn2) f.mn2 = d2[i];
       if (d2[i] > f.mx2) f.mx2 = d2[i];
     }
+    // The log likelihood is a sum over pixel pairs, so their order is free
+    // (up to rounding): store them in strips of band-1 flux, ascending in
+    // band 2 inside a strip, so that neighbouring lanes interpolate in
+    // neighbouring cells of the P(D).
+    const double span1 = f.mx1 - f.mn1;
+    const double sfac = (span1 > 0) ? 512.0 / span1 : 0.0;
+    std::vector<uint32_t> strip(npix);
+    std::vector<uint64_t> order(npix);
+    for (uint64_t i = 0; i < npix; ++i) {
+      const double sv = (d1[i] - f.mn1) * sfac;
+      strip[i] = static_cast<uint32_t>(sv < 511.0 ? sv : 511.0);
+      order[i] = i;
+    }
+    std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+      if (strip[a] != strip[b]) return strip[a] < strip[b];
+      if (d2[a] != d2[b]) return d2[a] < d2[b];
+      return a < b;
+    });
+    for (uint64_t i = 0; i < npix; ++i) {
+      f.d1[i] = d1[order[i]];
+      f.d2[i] = d2[order[i]];
+    }
   }
   s.fields.push_back(std::move(f));
   s.prepared = false;
@@ -1188,8 +1213,10 @@ int pofd2d_loglike_batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwal
     Set2& s = *C2->sets[b];
     if ((rc = ensure_work2(c, *C2, s, nwalk))) return rc;
     s.dev.spectral_cut = pofd2::ctx_spectral_cut(c);
+    hipEvent_t* ev = nullptr;
+    if ((rc = pofd2::ctx_stage_events(c, &ev))) return rc;
     rc = pofd2::launch_loglike2d(s.dev, s.wk, s.gmain, s.g00, s.gx, s.gy, d_theta, stride, d_ll,
-                                 d_st, b > 0 ? 1 : 0, stream, nullptr);
+                                 d_st, b > 0 ? 1 : 0, stream, ev);
     if (rc)
       return ctx_fail(c, rc, std::string("pofd2d_loglike_batch_device: launch failed: ") +
                                  hipGetErrorString(hipGetLastError()));

@@ -15,6 +15,7 @@ namespace pofd2 {
 constexpr int kMaxK = 32;          // knots, sigma knots, offset knots (each)
 constexpr int kMaxFields2 = 8;     // datasets per beam set
 constexpr int kLikeBlocks = 64;    // workgroups per (walker, dataset) in like2d
+constexpr int kRowBlocks2 = 64;    // workgroups per walker in the inverse row transforms
 // per-walker model block: header, knot spline, sigma and offset interpolants
 // (4 coefficients per interval each), raw sigma and offset values
 constexpr int kModelStride = 8 + 14 * kMaxK;
@@ -30,7 +31,8 @@ enum {  // stats slots (per walker)
 enum {  // debug slots (per walker and dataset)
   D2_STATUS = 0, D2_SPLIT1, D2_SPLIT2, D2_SG1POS, D2_SG2POS, D2_SIGMA1,
   D2_SIGMA2, D2_INTEGRAL, D2_ITOT, D2_MINFLUX1, D2_MINFLUX2, D2_LL,
-  D2_MN1, D2_MN2, D2_VAR1, D2_VAR2, D2_R0, D2_K1, D2_K2
+  D2_MN1, D2_MN2, D2_VAR1, D2_VAR2, D2_R0, D2_K1, D2_K2,
+  D2_ST1, D2_ST2  // split status of each axis
 };
 
 // one sign component (pp, pn, np, nn) of the beam, device view
@@ -110,11 +112,12 @@ struct Work2 {
   double* rsum2;
   double* rowA;
   double* rowB;
+  double* colpart;                // W * kRowBlocks2 * n  column partial sums
   double* rstat_part;             // W * nparts * 4
   double* lpart;                  // W * nf * kLikeBlocks
   double* dbg;                    // W * nf * kDbg2
   size_t R_stride, ex_stride, edge_stride, Y_stride, YT_stride, PR_stride,
-      Q_stride, QT_stride, P_stride;
+      Q_stride, QT_stride, P_stride, colpart_stride;
   int ystride;                    // n/2 + 1 (complex)
 };
 
@@ -129,6 +132,7 @@ Ctx2*& ctx2_of(pofd_ctx* c);
 int ctx_device(const pofd_ctx* c);
 double ctx_spectral_cut(const pofd_ctx* c);
 int ctx_fail(pofd_ctx* c, int code, const std::string& msg);
+int ctx_stage_events(pofd_ctx* c, hipEvent_t** ev);
 
 // launcher (pofd2d_kernel.hip)
 int launch_loglike2d(const Set2Dev& s, const Work2& wk, const RGrid& gmain,

@@ -76,6 +76,69 @@ __device__ double block_sum(double v, double* red) {
   return s;
 }
 
+__device__ double block_max(double v, double* red) {
+  const int tid = threadIdx.x;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int i = 1; i < nw; ++i) s = fmax(s, red[i]);
+  __syncthreads();
+  return s;
+}
+
+__device__ int block_max_int(int v, int* ired) {
+  const int tid = threadIdx.x;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  __syncthreads();
+  if ((tid & 63) == 0) ired[tid >> 6] = v;
+  __syncthreads();
+  int s = ired[0];
+  for (int i = 1; i < nw; ++i) s = max(s, ired[i]);
+  __syncthreads();
+  return s;
+}
+
+// (value, index) minimum, the lowest index among equal values; idx < 0 is
+// an empty slot
+__device__ void block_argmin(double& v, int& idx, double* red, int* ired) {
+  const int tid = threadIdx.x;
+  const int nw = blockDim.x >> 6;
+  auto better = [](double a, int ia, double b, int ib) {
+    if (ib < 0) return true;
+    if (ia < 0) return false;
+    return (a < b) || (a == b && ia < ib);
+  };
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o);
+    const int oi = __shfl_xor(idx, o);
+    if (!better(v, idx, ov, oi)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    red[tid >> 6] = v;
+    ired[tid >> 6] = idx;
+  }
+  __syncthreads();
+  v = red[0];
+  idx = ired[0];
+  for (int i = 1; i < nw; ++i)
+    if (!better(v, idx, red[i], ired[i])) {
+      v = red[i];
+      idx = ired[i];
+    }
+  __syncthreads();
+}
+
 // ---- per-walker model block (doubles): [0] status, [1] sigmult1,
 // [2] sigmult2, [3] K1, [4] K2, then the knot spline (4 per interval, log2
 // space), sigma and offset interpolants (4 per interval), raw sigma and
@@ -708,6 +771,10 @@ __global__ __launch_bounds__(kT) void rowifft2d_kernel(const Set2Dev s, const Wo
   double2* a = lds;
   double2* b = lds + h;
   const size_t wf = static_cast<size_t>(w) * s.nfields + f;
+  constexpr int kMaxPairs = 4096 / 2 / kT;  // output pairs per thread, n <= 4096
+  double cacc[2 * kMaxPairs];
+#pragma unroll
+  for (int q = 0; q < 2 * kMaxPairs; ++q) cacc[q] = 0.0;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const double2* X = wk.QT + static_cast<size_t>(w) * wk.QT_stride + static_cast<size_t>(i) * wk.ystride;
     for (int k = threadIdx.x; k < h; k += kT) {
@@ -727,23 +794,43 @@ __global__ __launch_bounds__(kT) void rowifft2d_kernel(const Set2Dev s, const Wo
     __syncthreads();
     const double2* z = fft_lds<+1>(a, b, h, s.tw_half);
     double* P = wk.P + wf * wk.P_stride + static_cast<size_t>(i) * n;
+    const double wi = (i == 0 || i == n - 1) ? 0.5 : 1.0;
     double part = 0.0;
-    for (int m2 = threadIdx.x; m2 < h; m2 += kT) {
-      double x0 = z[m2].x, x1 = z[m2].y;
-      if (x0 < 0) x0 = 0.0;  // unwrapAndNormalizePD positivity
-      if (x1 < 0) x1 = 0.0;
-      P[2 * m2] = x0;
-      P[2 * m2 + 1] = x1;
-      const double w0 = (2 * m2 == 0) ? 0.5 : 1.0;
-      const double w1 = (2 * m2 + 1 == n - 1) ? 0.5 : 1.0;
-      part += w0 * x0 + w1 * x1;
+#pragma unroll
+    for (int q = 0; q < kMaxPairs; ++q) {
+      const int m2 = threadIdx.x + q * kT;
+      if (m2 < h) {
+        double x0 = z[m2].x, x1 = z[m2].y;
+        if (x0 < 0) x0 = 0.0;  // unwrapAndNormalizePD positivity
+        if (x1 < 0) x1 = 0.0;
+        P[2 * m2] = x0;
+        P[2 * m2 + 1] = x1;
+        const double w0 = (2 * m2 == 0) ? 0.5 : 1.0;
+        const double w1 = (2 * m2 + 1 == n - 1) ? 0.5 : 1.0;
+        part += w0 * x0 + w1 * x1;
+        cacc[2 * q] += wi * x0;
+        cacc[2 * q + 1] += wi * x1;
+      }
     }
     const double rs = block_sum(part, red);
     if (threadIdx.x == 0) wk.rsum1[wf * wk.edge_stride + i] = rs;
   }
+  // this workgroup's share of the column trapezoid sums
+  double* cp = wk.colpart + static_cast<size_t>(w) * wk.colpart_stride +
+               static_cast<size_t>(blockIdx.x) * n;
+#pragma unroll
+  for (int q = 0; q < kMaxPairs; ++q) {
+    const int m2 = threadIdx.x + q * kT;
+    if (m2 < h) {
+      cp[2 * m2] = cacc[2 * q];
+      cp[2 * m2 + 1] = cacc[2 * q + 1];
+    }
+  }
 }
 
-// column trapezoid marginal, thread per column, serial over rows
+// column trapezoid marginal (unwrapAndNormalizePD, src/PDFactoryDouble.cc:
+// 1001-1011) from the per-workgroup partial sums of rowifft2d, thread per
+// column
 __global__ void colsum2d_kernel(const Set2Dev s, const Work2 wk, const int f) {
   const int w = blockIdx.y;
   if (model_ptr(wk, w)[0] != POFD_OK) return;
@@ -751,118 +838,138 @@ __global__ void colsum2d_kernel(const Set2Dev s, const Work2 wk, const int f) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const size_t wf = static_cast<size_t>(w) * s.nfields + f;
-  const double* P = wk.P + wf * wk.P_stride;
-  double acc;
-  {
-#pragma clang fp contract(off)
-    acc = 0.5 * P[j];
-    for (int i = 1; i < n - 1; ++i) acc += P[static_cast<size_t>(i) * n + j];
-    acc += 0.5 * P[static_cast<size_t>(n - 1) * n + j];
-  }
+  const double* cp = wk.colpart + static_cast<size_t>(w) * wk.colpart_stride + j;
+  double acc = 0.0;
+  for (int b = 0; b < kRowBlocks2; ++b) acc += cp[static_cast<size_t>(b) * n];
   wk.rsum2[wf * wk.edge_stride + j] = acc;
 }
 
-namespace {
-// findSplitPoint (src/PDFactoryDouble.cc:784-975), serial
-__device__ int find_split2(const double* data, int cs, double dflux, double sigmapos,
-                           double sigmaneg, int* out) {
-  const double n1 = 2.25, f1 = 5e-14, f2 = 1.5e-5, f3 = 5;
-  if (sigmapos <= 0 || sigmaneg <= 0) return POFD_E_SPLIT;
-  const double fluxrange = cs * dflux;
-  const double n1sigpos = n1 * sigmapos, n1signeg = n1 * sigmaneg;
-  if (fluxrange < (n1sigpos + n1signeg)) return POFD_E_SPLIT;
-  double peak = data[0];
-  for (int i = 1; i < cs; ++i)
-    if (data[i] > peak) peak = data[i];
-  if (peak <= 0.0) return POFD_E_SPLIT;
-  const int topidx = cs - static_cast<int>(n1signeg / dflux) - 1;
-  const int botidx = static_cast<int>(n1sigpos / dflux);
-  if (topidx >= cs || botidx < 0 || botidx > topidx) return POFD_E_SPLIT;
-  int minidx = botidx;
-  double minval = data[minidx];
-  for (int i = botidx + 1; i <= topidx; ++i)
-    if (data[i] < minval) {
-      minidx = i;
-      minval = data[i];
-    }
-  if (minval > f2 * peak) return POFD_E_SPLIT;
-  const int pre_jitter_n = 17;
-  double jitter = 0.0;
-  if (minidx > pre_jitter_n) {
-    bool mono = true;
-    const int initidx = minidx - pre_jitter_n;
-    double prev = data[initidx];
-    jitter = prev;
-    for (int i = initidx + 1; i < minidx; ++i) {
-      const double cv = data[i];
-      if (cv >= prev) mono = false;
-      jitter += cv;
-      prev = cv;
-    }
-    if (mono) {
-      *out = minidx;
-      return POFD_OK;
-    }
-    jitter += minval;
-    jitter = 1.1 * jitter / static_cast<double>(pre_jitter_n + 1);
-  } else {
-    jitter = f1 * peak;
-    if (minval > jitter) {
-      *out = minidx;
-      return POFD_OK;
-    }
-  }
-  if (minval <= 0.0) {
-    const int nminup = 3;
-    int bi, ti;
-    if (minidx < botidx)
-      bi = botidx;
-    else
-      bi = (minidx >= nminup) ? (minidx - nminup) : minidx;
-    ti = (minidx > topidx - nminup) ? topidx - nminup : minidx + nminup;
-    ti = (ti >= cs) ? (cs - 1) : ti;
-    for (int i = bi; i < ti; ++i)
-      if (data[i] > minval) minval = data[i];
-  }
-  if (minval > 0) jitter = (f3 * minval) < jitter ? (f3 * minval) : jitter;
-  if ((botidx >= topidx) || (abs(topidx - botidx) < 1)) return POFD_E_SPLIT;
-  if (data[topidx] < jitter) return POFD_E_SPLIT;
-  for (int i = topidx - 1; i >= botidx; --i)
-    if (data[i] < jitter) {
-      *out = i + 1;
-      return POFD_OK;
-    }
-  return POFD_E_SPLIT;
-}
-}  // namespace
-
-// split2d: one thread per walker (both axes)
-__global__ void split2d_kernel(const Set2Dev s, const Work2 wk, const int f) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= wk.nwalk) return;
+// ===========================================================================
+// split2d: findSplitPoint (src/PDFactoryDouble.cc:784-975) on the marginal of
+// one axis; one workgroup per (axis, walker)
+// ===========================================================================
+__global__ __launch_bounds__(kT) void split2d_kernel(const Set2Dev s, const Work2 wk,
+                                                     const int f) {
+  __shared__ double red[kT / 64];
+  __shared__ int ired[kT / 64];
+  __shared__ int ires[3];
+  __shared__ double dres[1];
+  const int axis = blockIdx.x, w = blockIdx.y;
   const double* m = model_ptr(wk, w);
+  if (m[0] != POFD_OK) return;  // uniform
+  const int tid = threadIdx.x;
   const size_t wf = static_cast<size_t>(w) * s.nfields + f;
   double* dbg = wk.dbg + wf * kDbg2;
-  if (m[0] != POFD_OK) return;
   const double* st = wk.stats + static_cast<size_t>(w) * kStats;
-  const double sigma1 = m[1] * s.fields[f].sigma_base1;
-  const double sigma2 = m[2] * s.fields[f].sigma_base2;
-  const double var1 = st[ST2_VAR1], var2 = st[ST2_VAR2];
-  const double sg1pos = sqrt(var1 + sigma1 * sigma1);
-  const double sg2pos = sqrt(var2 + sigma2 * sigma2);
-  const double sg1neg = (9 * sigma1 * sigma1 > var1) ? sigma1 : sg1pos;
-  const double sg2neg = (9 * sigma2 * sigma2 > var2) ? sigma2 : sg2pos;
-  int s1 = -1, s2 = -1;
-  int status = find_split2(wk.rsum1 + wf * wk.edge_stride, s.n, s.dflux1, sg1pos, sg1neg, &s1);
-  if (status == POFD_OK)
-    status = find_split2(wk.rsum2 + wf * wk.edge_stride, s.n, s.dflux2, sg2pos, sg2neg, &s2);
-  dbg[D2_STATUS] = status;
-  dbg[D2_SPLIT1] = s1;
-  dbg[D2_SPLIT2] = s2;
-  dbg[D2_SG1POS] = sg1pos;
-  dbg[D2_SG2POS] = sg2pos;
-  dbg[D2_SIGMA1] = sigma1;
-  dbg[D2_SIGMA2] = sigma2;
+  const double sigma = m[1 + axis] * (axis ? s.fields[f].sigma_base2 : s.fields[f].sigma_base1);
+  const double var = st[ST2_VAR1 + axis];
+  const double sgpos = sqrt(var + sigma * sigma);
+  const double sgneg = (9 * sigma * sigma > var) ? sigma : sgpos;  // getPD, :1413-1420
+  const double* data = (axis ? wk.rsum2 : wk.rsum1) + wf * wk.edge_stride;
+  const double dflux = axis ? s.dflux2 : s.dflux1;
+  const int cs = s.n;
+  const double n1 = 2.25, f1 = 5e-14, f2 = 1.5e-5, f3 = 5;
+  int status = POFD_OK;
+  if (sgpos <= 0 || sgneg <= 0) status = POFD_E_SPLIT;
+  const double fluxrange = cs * dflux;
+  const double n1sigpos = n1 * sgpos, n1signeg = n1 * sgneg;
+  if (fluxrange < (n1sigpos + n1signeg)) status = POFD_E_SPLIT;
+  auto clampi = [](double v) { return static_cast<int>(v > 2e9 ? 2e9 : (v < -2e9 ? -2e9 : v)); };
+  const int topidx = cs - clampi(n1signeg / dflux) - 1;
+  const int botidx = clampi(n1sigpos / dflux);
+  if (topidx >= cs || botidx < 0 || botidx > topidx) status = POFD_E_SPLIT;
+  int split = -1;
+  if (status == POFD_OK) {  // uniform across the workgroup
+    double pk = -1.0;
+    for (int i = tid; i < cs; i += kT) pk = fmax(pk, data[i]);
+    const double peak = block_max(pk, red);
+    // first minimum over [botidx, topidx]
+    double mv = 0.0;
+    int mi = -1;
+    for (int i = botidx + tid; i <= topidx; i += kT) {
+      const double v = data[i];
+      if (mi < 0 || v < mv) {
+        mv = v;
+        mi = i;
+      }
+    }
+    block_argmin(mv, mi, red, ired);
+    if (tid == 0) {
+      int stt = (peak <= 0.0 || mv > f2 * peak) ? POFD_E_SPLIT : POFD_OK;
+      int decided = 0, sp = -1;
+      double minval = mv, jitter = 0.0;
+      const int minidx = mi;
+      if (stt == POFD_OK) {
+        const int pre_jitter_n = 17;
+        if (minidx > pre_jitter_n) {
+          bool mono = true;
+          const int initidx = minidx - pre_jitter_n;
+          double prev = data[initidx];
+          jitter = prev;
+          for (int i = initidx + 1; i < minidx; ++i) {
+            const double cv = data[i];
+            if (cv >= prev) mono = false;
+            jitter += cv;
+            prev = cv;
+          }
+          if (mono) {
+            sp = minidx;
+            decided = 1;
+          } else {
+            jitter += minval;
+            jitter = 1.1 * jitter / static_cast<double>(pre_jitter_n + 1);
+          }
+        } else {
+          jitter = f1 * peak;
+          if (minval > jitter) {
+            sp = minidx;
+            decided = 1;
+          }
+        }
+      }
+      if (stt == POFD_OK && !decided) {
+        if (minval <= 0.0) {
+          const int nminup = 3;
+          int bi, ti;
+          if (minidx < botidx)
+            bi = botidx;
+          else
+            bi = (minidx >= nminup) ? (minidx - nminup) : minidx;
+          ti = (minidx > topidx - nminup) ? topidx - nminup : minidx + nminup;
+          ti = (ti >= cs) ? (cs - 1) : ti;
+          for (int i = bi; i < ti; ++i)
+            if (data[i] > minval) minval = data[i];
+        }
+        if (minval > 0) jitter = (f3 * minval) < jitter ? (f3 * minval) : jitter;
+        if (botidx >= topidx || data[topidx] < jitter) stt = POFD_E_SPLIT;
+      }
+      ires[0] = stt;
+      ires[1] = decided;
+      ires[2] = sp;
+      dres[0] = jitter;
+    }
+    __syncthreads();
+    status = ires[0];
+    split = ires[2];
+    if (status == POFD_OK && !ires[1]) {
+      // the largest i in [botidx, topidx - 1] with data[i] < jitter
+      const double jitter = dres[0];
+      int cand = -1;
+      for (int i = botidx + tid; i <= topidx - 1; i += kT)
+        if (data[i] < jitter) cand = i;
+      const int found = block_max_int(cand, ired);
+      if (found < 0)
+        status = POFD_E_SPLIT;
+      else
+        split = found + 1;
+    }
+  }
+  if (tid == 0) {
+    dbg[D2_ST1 + axis] = status;
+    dbg[D2_SPLIT1 + axis] = split;
+    dbg[D2_SG1POS + axis] = sgpos;
+    dbg[D2_SIGMA1 + axis] = sigma;
+  }
 }
 
 // norm2d_rows: per rotated row i' the trapezoid row sum A and the weighted
@@ -874,7 +981,7 @@ __global__ __launch_bounds__(kT) void norm2d_rows_kernel(const Set2Dev s, const 
   if (model_ptr(wk, w)[0] != POFD_OK) return;
   const size_t wf = static_cast<size_t>(w) * s.nfields + f;
   const double* dbg = wk.dbg + wf * kDbg2;
-  if (dbg[D2_STATUS] != POFD_OK) return;
+  if (dbg[D2_ST1] != POFD_OK || dbg[D2_ST2] != POFD_OK) return;
   const int n = s.n;
   const int s1 = static_cast<int>(dbg[D2_SPLIT1]), s2 = static_cast<int>(dbg[D2_SPLIT2]);
   const double* P = wk.P + wf * wk.P_stride;
@@ -897,38 +1004,49 @@ __global__ __launch_bounds__(kT) void norm2d_rows_kernel(const Set2Dev s, const 
   }
 }
 
-// norm2d_final: integral, means, minflux (one thread per walker)
-__global__ void norm2d_final_kernel(const Set2Dev s, const Work2 wk, const int f) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= wk.nwalk) return;
+// norm2d_final: the split status, then integral, means and minflux from
+// the rotated row sums (PDDouble::getIntegral / getMeans), one workgroup
+// per walker
+__global__ __launch_bounds__(kT) void norm2d_final_kernel(const Set2Dev s, const Work2 wk,
+                                                          const int f) {
+  __shared__ double red[kT / 64];
+  const int w = blockIdx.x;
   if (model_ptr(wk, w)[0] != POFD_OK) return;
   const size_t wf = static_cast<size_t>(w) * s.nfields + f;
   double* dbg = wk.dbg + wf * kDbg2;
-  if (dbg[D2_STATUS] != POFD_OK) return;
+  const int st1 = static_cast<int>(dbg[D2_ST1]), st2 = static_cast<int>(dbg[D2_ST2]);
+  const int sst = (st1 != POFD_OK) ? st1 : st2;  // band 1 is searched first
+  if (sst != POFD_OK) {
+    if (threadIdx.x == 0) dbg[D2_STATUS] = sst;
+    return;
+  }
   const int n = s.n;
   const double* A = wk.rowA + wf * wk.edge_stride;
   const double* B = wk.rowB + wf * wk.edge_stride;
-  double tot = 0.5 * A[0], mean1 = 0.0, mean2 = 0.5 * B[0];
-  for (int i = 1; i < n - 1; ++i) {
-    tot += A[i];
-    mean1 += static_cast<double>(i) * A[i];
-    mean2 += B[i];
+  double tp = 0.0, m1p = 0.0, m2p = 0.0;
+  for (int i = threadIdx.x; i < n; i += kT) {
+    const double wi = (i == 0 || i == n - 1) ? 0.5 : 1.0;
+    tp += wi * A[i];
+    m1p += wi * static_cast<double>(i) * A[i];
+    m2p += wi * B[i];
   }
-  tot += 0.5 * A[n - 1];
-  mean1 += 0.5 * static_cast<double>(n - 1) * A[n - 1];
-  mean2 += 0.5 * B[n - 1];
-  const double integral = tot * s.dflux1 * s.dflux2;
-  const double itot = 1.0 / integral;
-  // getMeans on the normalised array (PDDouble.cc:355-452, donorm false)
-  const double m1 = mean1 * itot * (s.dflux1 * s.dflux1 * s.dflux2);
-  const double m2 = mean2 * itot * (s.dflux1 * s.dflux2 * s.dflux2);
-  int status = POFD_OK;
-  if (isinf(m1) || isnan(m1) || isinf(m2) || isnan(m2)) status = POFD_E_MEAN;
-  dbg[D2_STATUS] = status;
-  dbg[D2_INTEGRAL] = integral;
-  dbg[D2_ITOT] = itot;
-  dbg[D2_MINFLUX1] = -m1;
-  dbg[D2_MINFLUX2] = -m2;
+  const double tot = block_sum(tp, red);
+  const double mean1 = block_sum(m1p, red);
+  const double mean2 = block_sum(m2p, red);
+  if (threadIdx.x == 0) {
+    const double integral = tot * s.dflux1 * s.dflux2;
+    const double itot = 1.0 / integral;
+    // getMeans on the normalised array (PDDouble.cc:355-452, donorm false)
+    const double m1 = mean1 * itot * (s.dflux1 * s.dflux1 * s.dflux2);
+    const double m2 = mean2 * itot * (s.dflux1 * s.dflux2 * s.dflux2);
+    int status = POFD_OK;
+    if (isinf(m1) || isnan(m1) || isinf(m2) || isnan(m2)) status = POFD_E_MEAN;
+    dbg[D2_STATUS] = status;
+    dbg[D2_INTEGRAL] = integral;
+    dbg[D2_ITOT] = itot;
+    dbg[D2_MINFLUX1] = -m1;
+    dbg[D2_MINFLUX2] = -m2;
+  }
 }
 
 // like2d: bilinear interpolation of log2 of the normalised, rotated P(D)
@@ -1170,14 +1288,14 @@ int launch_loglike2d(const Set2Dev& s, const Work2& wk, const RGrid& gmain,
     hipLaunchKernelGGL(transpose2d_kernel, dim3((n + 31) / 32, (nc + 31) / 32, W), dim3(kT), 0,
                        st, wk, static_cast<const double2*>(wk.Q), wk.Q_stride, n, wk.QT,
                        wk.QT_stride, wk.ystride, nc, n, 2);
-    hipLaunchKernelGGL(rowifft2d_kernel, dim3(256, W), dim3(kT), lds_row, st, s, wk, f);
+    hipLaunchKernelGGL(rowifft2d_kernel, dim3(kRowBlocks2, W), dim3(kT), lds_row, st, s, wk, f);
     hipLaunchKernelGGL(colsum2d_kernel, dim3((n + 255) / 256, W), dim3(256), 0, st, s, wk, f);
   }
   mark(POFD_STAGE_FINALIZE);
   for (int f = 0; f < s.nfields; ++f) {
-    hipLaunchKernelGGL(split2d_kernel, dim3((W + 63) / 64), dim3(64), 0, st, s, wk, f);
+    hipLaunchKernelGGL(split2d_kernel, dim3(2, W), dim3(kT), 0, st, s, wk, f);
     hipLaunchKernelGGL(norm2d_rows_kernel, dim3(256, W), dim3(kT), 0, st, s, wk, f);
-    hipLaunchKernelGGL(norm2d_final_kernel, dim3((W + 63) / 64), dim3(64), 0, st, s, wk, f);
+    hipLaunchKernelGGL(norm2d_final_kernel, dim3(W), dim3(kT), 0, st, s, wk, f);
   }
   mark(POFD_STAGE_PIXELS);
   for (int f = 0; f < s.nfields; ++f)

@@ -96,3 +96,28 @@ def test_small2d_deterministic(small2d):
     ctx, _, ll, _ = out[False]
     ll2, _ = ctx.loglike2d_batch(th)
     assert np.array_equal(ll, ll2)
+
+
+def _full2d(binned, nwalk=4):
+    """The measured 2D configuration: n = 2048 grid, 1000x1000 pixel pairs,
+    120x120-bin beam histogram, sigma 2.5 / 2.7 mJy."""
+    inp = make_inputs2d(npix_side=1000, seed=99, nbeamhist=120,
+                        minbeamval=1e-6, nbins=500 if binned else None)
+    th0 = synth2d.true_theta()
+    th = synth2d.walker_thetas(nwalk, seed=4321)
+    ctx = workload2d.build_context2d(inp, 2048, th0, nedge=256,
+                                     binned=binned)
+    ll, st = ctx.loglike2d_batch(th)
+    L = build_oracle2d(inp, 2048, th0, nedge=256, binned=binned)
+    ref, rst = L.loglike_batch(th, nwalk)
+    return ll, st, ref, rst
+
+
+@pytest.mark.parametrize("binned", [False, True])
+def test_config2d_full_size(binned):
+    _say("full size 2D n=2048 binned=%s" % binned)
+    ll, st, ref, rst = _full2d(binned)
+    assert np.all(st == 0) and np.all(rst == 0), (st, rst)
+    print("[gpu2d-case] max |dlogL| = %.3e" % np.max(np.abs(ll - ref)),
+          flush=True)
+    np.testing.assert_allclose(ll, ref, rtol=0, atol=1e-6)
