@@ -108,6 +108,12 @@ int pofd_fits_read_beam(const char *path, double *pix, uint64_t capacity,
                         uint64_t *npix, double *pixsize_arcsec);
 int pofd_fits_read_map(const char *path, int ignore_mask, double *data,
                        uint64_t capacity, uint64_t *n);
+/* Pixel pairs of a band-1 and a band-2 map (fitsDataDouble::readData,
+   src/fitsDataDouble.cc:305-406): with both masks a pixel is kept when
+   (mask1 & mask2) == 0, with one mask when it is 0.  NULL buffers query n. */
+int pofd_fits_read_map_pair(const char *path1, const char *path2,
+                            int ignore_mask, double *d1, double *d2,
+                            uint64_t capacity, uint64_t *n);
 const char *pofd_fits_last_error(void);
 
 /* ---- data: replaces fitsData::applyBinning and fitsData::meanSubtract,

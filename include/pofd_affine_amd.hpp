@@ -3,18 +3,25 @@
  *
  * Header-only layer giving the classes the reference's MPI slaves and CLIs
  * call -- affineExcept, paramSet, beam, fitsData, numberCountsKnotsSpline,
- * PD, PDFactory and calcLike -- with the reference's names, signatures and
- * error behaviour, backed by libpofd_affine_amd.so on a gfx950 GPU.
+ * PD, PDFactory and calcLike, and for two bands doublebeam, fitsDataDouble,
+ * numberCountsDoubleLogNormal, PDDouble, PDFactoryDouble and calcLikeDouble
+ * -- with the reference's names, signatures and error behaviour, backed by
+ * libpofd_affine_amd.so on a gfx950 GPU.
  * Reference interfaces: include/affineExcept.h, include/paramSet.h:17-71,
  * include/beam.h:28-175, include/fitsData.h:16-96,
  * include/numberCountsKnotsSpline.h:21-70, include/PD.h:24-96,
- * include/PDFactory.h:26-153, include/calcLike.h:193-287.
+ * include/PDFactory.h:26-153, include/calcLike.h:193-287,
+ * include/doublebeam.h:78-150, include/fitsDataDouble.h:44-100,
+ * include/numberCountsDoubleLogNormal.h:133-245, include/PDDouble.h:30-100,
+ * include/PDFactoryDouble.h:120-160, include/calcLikeDouble.h:239-360.
  *
  * Differences a caller can observe:
  *  - calcLike::getLogLikeBatch evaluates many walkers in one launch;
  *  - the CFIRB and Poisson priors and the bonus parameters
  *    (fillBonusParams) throw affineExcept("...", "...",
  *    "not supported by the device path");
+ *  - PDFactoryDouble keeps a pointer to the doublebeam given to initPD,
+ *    which must outlive it;
  *  - PD::getLogLike is not provided: log likelihoods come from
  *    calcLike::getLogLike, which runs the whole chain on the device.
  */
@@ -24,6 +31,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <ostream>
@@ -651,6 +659,658 @@ class calcLike {
       }
       if (regularization_alpha > 0.0) {
         numberCountsKnotsSpline m = model;
+        m.setParams(paramSet(stride, t));
+        loglike[w] += m.differenceRegularize(regularization_alpha);
+      }
+    }
+  }
+};
+
+/* ======================================================================
+   Two-band classes: doublebeam, fitsDataDouble, numberCountsDoubleLogNormal,
+   PDDouble, PDFactoryDouble, calcLikeDouble (include/doublebeam.h,
+   include/fitsDataDouble.h, include/numberCountsDoubleLogNormal.h,
+   include/PDDouble.h, include/PDFactoryDouble.h, include/calcLikeDouble.h)
+   ====================================================================== */
+
+/* include/doublebeam.h: two beams split into the pp, pn, np, nn sign
+   components, optionally histogrammed (host, bit-exact) */
+class doublebeam {
+ private:
+  std::shared_ptr<pofd_dbeam> h;
+  double info_[64];
+  void refresh() {
+    if (h) pofd_dbeam_info(h.get(), info_);
+  }
+  static void read_one(const std::string& f, std::vector<double>& pix, double& ps) {
+    uint64_t n = 0;
+    if (pofd_fits_read_beam(f.c_str(), nullptr, 0, &n, &ps))
+      throw affineExcept("doublebeam", "readFiles", pofd_fits_last_error());
+    pix.assign(n, 0.0);
+    if (pofd_fits_read_beam(f.c_str(), pix.data(), n, &n, &ps))
+      throw affineExcept("doublebeam", "readFiles", pofd_fits_last_error());
+  }
+
+ public:
+  doublebeam() { std::memset(info_, 0, sizeof(info_)); }
+  doublebeam(const std::string& f1, const std::string& f2, bool histogram = false,
+             unsigned int NBINS = 150, double MINVAL = 1e-6) {
+    std::memset(info_, 0, sizeof(info_));
+    readFiles(f1, f2, MINVAL);
+    if (histogram) makeHistogram(NBINS);
+  }
+  /* doublebeam::readFiles, src/doublebeam.cc:126-330 */
+  void readFiles(const std::string& f1, const std::string& f2, double MINVAL = 1e-6) {
+    std::vector<double> p1, p2;
+    double ps1 = 0.0, ps2 = 0.0;
+    read_one(f1, p1, ps1);
+    read_one(f2, p2, ps2);
+    if (p1.size() != p2.size())
+      throw affineExcept("doublebeam", "readFiles", "Beam sizes don't match");
+    if (ps1 != ps2)
+      throw affineExcept("doublebeam", "readFiles", "Beam pixel sizes don't match");
+    setBeams(static_cast<unsigned>(p1.size()), p1.data(), p2.data(), ps1, MINVAL);
+  }
+  /* doublebeam::setBeams, src/doublebeam.cc:331-466 */
+  void setBeams(unsigned int n, const double* const b1, const double* const b2,
+                double PIXSIZE, double MINVAL = 1e-6) {
+    pofd_dbeam* b = nullptr;
+    if (pofd_dbeam_create(&b, b1, b2, n, PIXSIZE, MINVAL))
+      throw affineExcept("doublebeam", "setBeams", pofd_dbeam_last_error());
+    h.reset(b, pofd_dbeam_destroy);
+    refresh();
+  }
+  /* doublebeam::makeHistogram, src/doublebeam.cc:467-584 */
+  void makeHistogram(unsigned int NBINS = 150) {
+    if (!h) throw affineExcept("doublebeam", "makeHistogram", "No beam read");
+    if (pofd_dbeam_make_histogram(h.get(), NBINS))
+      throw affineExcept("doublebeam", "makeHistogram", pofd_dbeam_last_error());
+    refresh();
+  }
+  bool hasData() const {
+    for (int s = 0; s < 4; ++s)
+      if (hasSign(s)) return true;
+    return false;
+  }
+  bool hasSign(unsigned int i) const { return info_[12 * i] != 0.0; }
+  unsigned int getNPix(unsigned int i) const { return static_cast<unsigned>(info_[12 * i + 1]); }
+  unsigned int getMaxNPix() const {
+    unsigned m = 0;
+    for (int s = 0; s < 4; ++s) m = std::max(m, getNPix(s));
+    return m;
+  }
+  unsigned int getTotalNPix() const {
+    unsigned t = 0;
+    for (int s = 0; s < 4; ++s) t += getNPix(s);
+    return t;
+  }
+  bool isHistogrammed(unsigned int i) const { return info_[12 * i + 2] != 0.0; }
+  unsigned int getNHist(unsigned int i) const { return static_cast<unsigned>(info_[12 * i + 3]); }
+  dblpair getMinMax1(unsigned int i) const { return dblpair(info_[12 * i + 8], info_[12 * i + 9]); }
+  dblpair getMinMax2(unsigned int i) const { return dblpair(info_[12 * i + 10], info_[12 * i + 11]); }
+  double getPixSize() const { return info_[48]; }
+  double getMinval() const { return info_[49]; }
+  unsigned int getNBins() const { return static_cast<unsigned>(info_[50]); }
+  /* per-sign effective areas in pixels (getEffectiveAreaSign1/2) */
+  double getEffectiveAreaSign1(unsigned int i) const { return info_[12 * i + 4]; }
+  double getEffectiveAreaSign2(unsigned int i) const { return info_[12 * i + 5]; }
+  double getEffectiveAreaSqSign1(unsigned int i) const { return info_[12 * i + 6]; }
+  double getEffectiveAreaSqSign2(unsigned int i) const { return info_[12 * i + 7]; }
+  /* effective areas in square degrees */
+  double getEffectiveArea1() const {
+    double a = 0.0;
+    for (int s = 0; s < 4; ++s) a += getEffectiveAreaSign1(s);
+    const double ps = getPixSize() / 3600.0;
+    return a * ps * ps;
+  }
+  double getEffectiveArea2() const {
+    double a = 0.0;
+    for (int s = 0; s < 4; ++s) a += getEffectiveAreaSign2(s);
+    const double ps = getPixSize() / 3600.0;
+    return a * ps * ps;
+  }
+  const pofd_dbeam* handle() const { return h.get(); }
+};
+
+/* include/fitsDataDouble.h: pixel pairs of two maps, optionally binned
+   (host, bit-exact) */
+class fitsDataDouble {
+ private:
+  std::vector<double> d1, d2;
+  bool is_binned = false;
+  unsigned int nb1 = 0, nb2 = 0;
+  double bininfo[4] = {0.0, 0.0, 0.0, 0.0};
+  std::vector<uint32_t> binval;
+  std::string file1, file2;
+
+ public:
+  fitsDataDouble() {}
+  fitsDataDouble(const std::string& f1, const std::string& f2, bool ignore_mask = false,
+                 bool meansub = false) {
+    readData(f1, f2, ignore_mask, meansub);
+  }
+  /* fitsDataDouble::readData, src/fitsDataDouble.cc:305-406 */
+  void readData(const std::string& f1, const std::string& f2, bool ignore_mask = false,
+                bool meansub = false) {
+    uint64_t n = 0;
+    const int ig = ignore_mask ? 1 : 0;
+    if (pofd_fits_read_map_pair(f1.c_str(), f2.c_str(), ig, nullptr, nullptr, 0, &n))
+      throw affineExcept("fitsDataDouble", "readData", pofd_fits_last_error());
+    d1.assign(n, 0.0);
+    d2.assign(n, 0.0);
+    if (pofd_fits_read_map_pair(f1.c_str(), f2.c_str(), ig, d1.data(), d2.data(), n, &n))
+      throw affineExcept("fitsDataDouble", "readData", pofd_fits_last_error());
+    file1 = f1;
+    file2 = f2;
+    removeBinning();
+    if (meansub) meanSubtract();
+  }
+  void setData(uint64_t n, const double* a, const double* b) {
+    d1.assign(a, a + n);
+    d2.assign(b, b + n);
+    removeBinning();
+  }
+  bool isBinned() const { return is_binned; }
+  /* fitsDataDouble::applyBinning, src/fitsDataDouble.cc:517-566 */
+  void applyBinning(unsigned int NB1, unsigned int NB2) {
+    if (d1.empty()) return;
+    if (NB1 == 0 || NB2 == 0)
+      throw affineExcept("fitsDataDouble", "applyBinning", "Trying to bin with no bins");
+    if (is_binned && NB1 == nb1 && NB2 == nb2) return;
+    binval.assign(static_cast<size_t>(NB1) * NB2, 0u);
+    if (pofd_data2_bin(d1.data(), d2.data(), d1.size(), NB1, NB2, binval.data(), bininfo))
+      throw affineExcept("fitsDataDouble", "applyBinning", "binning failed");
+    nb1 = NB1;
+    nb2 = NB2;
+    is_binned = true;
+  }
+  void removeBinning() {
+    is_binned = false;
+    nb1 = nb2 = 0;
+    binval.clear();
+  }
+  std::pair<unsigned int, unsigned int> getNBins() const { return std::make_pair(nb1, nb2); }
+  dblpair getBinCent0() const { return dblpair(bininfo[0], bininfo[2]); }
+  dblpair getBinDelta() const { return dblpair(bininfo[1], bininfo[3]); }
+  const double* binInfo() const { return bininfo; }
+  const unsigned int* getBinnedData() const {
+    return reinterpret_cast<const unsigned int*>(binval.data());
+  }
+  bool hasData() const { return !d1.empty(); }
+  unsigned int getN() const { return static_cast<unsigned>(d1.size()); }
+  const double* getData1() const { return d1.data(); }
+  const double* getData2() const { return d2.data(); }
+  /* fitsDataDouble::meanSubtract, src/fitsDataDouble.cc:567-608 */
+  dblpair meanSubtract() {
+    double m[2] = {0.0, 0.0};
+    if (!d1.empty()) pofd_data2_mean_subtract(d1.data(), d2.data(), d1.size(), m);
+    if (is_binned) {
+      const unsigned a = nb1, b = nb2;
+      removeBinning();
+      applyBinning(a, b);
+    }
+    return dblpair(m[0], m[1]);
+  }
+  std::pair<dblpair, dblpair> getMinMax() const {
+    if (d1.empty()) return std::make_pair(dblpair(NAN, NAN), dblpair(NAN, NAN));
+    double a0 = d1[0], a1 = d1[0], b0 = d2[0], b1 = d2[0];
+    for (size_t i = 1; i < d1.size(); ++i) {
+      a0 = std::min(a0, d1[i]);
+      a1 = std::max(a1, d1[i]);
+      b0 = std::min(b0, d2[i]);
+      b1 = std::max(b1, d2[i]);
+    }
+    return std::make_pair(dblpair(a0, b0), dblpair(a1, b1));
+  }
+  dblpair getMin() const { return getMinMax().first; }
+  dblpair getMax() const { return getMinMax().second; }
+  std::pair<std::string, std::string> getFiles() const { return std::make_pair(file1, file2); }
+};
+
+/* include/numberCountsDoubleLogNormal.h: band-1 knot spline plus the
+   log-normal colour model (sigma and offset knots) */
+class numberCountsDoubleLogNormal {
+ private:
+  std::vector<double> knots, sigmas, offsets;
+  std::vector<float> values;  // knot values, sigma values, offset values
+  double min_sigma = 0.0;
+  bool loaded = false;
+  static void check_pos(const std::vector<double>& v, const char* meth) {
+    for (size_t i = 0; i < v.size(); ++i) {
+      if (v[i] <= 0.0)
+        throw affineExcept("numberCountsDoubleLogNormal", meth, "Negative knot positions not allowed");
+      if (i > 0 && v[i] <= v[i - 1])
+        throw affineExcept("numberCountsDoubleLogNormal", meth, "Knot positions not monotonically increasing");
+    }
+  }
+
+ public:
+  numberCountsDoubleLogNormal() {}
+  numberCountsDoubleLogNormal(const std::vector<double>& K, const std::vector<double>& S,
+                              const std::vector<double>& O) {
+    setPositions(K, S, O);
+  }
+  /* setKnotPositions / setSigmaPositions / setOffsetPositions,
+     src/numberCountsDoubleLogNormal.cc:632-920 */
+  void setKnotPositions(const std::vector<double>& K) {
+    check_pos(K, "setKnotPositions");
+    knots = K;
+    loaded = false;
+  }
+  void setSigmaPositions(const std::vector<double>& S) {
+    check_pos(S, "setSigmaPositions");
+    sigmas = S;
+    loaded = false;
+  }
+  void setOffsetPositions(const std::vector<double>& O) {
+    check_pos(O, "setOffsetPositions");
+    offsets = O;
+    loaded = false;
+  }
+  void setPositions(const std::vector<double>& K, const std::vector<double>& S,
+                    const std::vector<double>& O) {
+    setKnotPositions(K);
+    setSigmaPositions(S);
+    setOffsetPositions(O);
+  }
+  /* numberCountsDoubleLogNormal::setMinSigma, :792 */
+  void setMinSigma(double v) {
+    if (v < 0.0) throw affineExcept("numberCountsDoubleLogNormal", "setMinSigma", "Invalid (negative) min sigma");
+    min_sigma = v;
+  }
+  double getMinSigma() const noexcept { return min_sigma; }
+  unsigned int getNKnots() const noexcept { return static_cast<unsigned>(knots.size()); }
+  unsigned int getNSigmas() const noexcept { return static_cast<unsigned>(sigmas.size()); }
+  unsigned int getNOffsets() const noexcept { return static_cast<unsigned>(offsets.size()); }
+  unsigned int getNParams() const noexcept { return getNKnots() + getNSigmas() + getNOffsets(); }
+  double getKnotPosition(unsigned int i) const { return knots[i]; }
+  double getSigmaPosition(unsigned int i) const { return sigmas[i]; }
+  double getOffsetPosition(unsigned int i) const { return offsets[i]; }
+  const std::vector<double>& knotPositions() const { return knots; }
+  const std::vector<double>& sigmaPositions() const { return sigmas; }
+  const std::vector<double>& offsetPositions() const { return offsets; }
+  const std::vector<float>& paramValues() const { return values; }
+  /* numberCountsDoubleLogNormal::setParams, :1010-1052 */
+  void setParams(const paramSet& F) {
+    if (F.getNParams() < getNParams())
+      throw affineExcept("numberCountsDoubleLogNormal", "setParams", "Not enough parameters present");
+    values.assign(F.data(), F.data() + getNParams());
+    loaded = true;
+  }
+  /* isValid: knot values not NaN, sigma values finite, offsets not NaN
+     (checkKnotsValid/checkSigmasValid/checkOffsetsValid, :1053-1124) */
+  bool isValid() const noexcept {
+    if (!loaded || knots.empty() || sigmas.empty() || offsets.empty()) return false;
+    const size_t nk = knots.size(), ns = sigmas.size();
+    for (size_t i = 0; i < values.size(); ++i) {
+      const float v = values[i];
+      if (std::isnan(v)) return false;
+      if (i >= nk && i < nk + ns && std::isinf(v)) return false;
+    }
+    return true;
+  }
+  /* numberCountsDoubleLogNormal::differenceRegularize, :2154-2173 */
+  double differenceRegularize(double alpha) const {
+    if (!isValid()) return NAN;
+    const size_t n = knots.size();
+    if (n < 2) return 0.0;
+    const double lf = pofd_amd_detail::logfac();
+    double pen = 0.0, mean_deriv = 0.0;
+    for (size_t i = 1; i < n; ++i)
+      mean_deriv += (lf * static_cast<double>(values[i]) - lf * static_cast<double>(values[i - 1])) /
+                    (log2(knots[i]) - log2(knots[i - 1]));
+    mean_deriv /= static_cast<double>(n - 1);
+    for (size_t i = 1; i < n; ++i) {
+      const double d = (lf * static_cast<double>(values[i]) - lf * static_cast<double>(values[i - 1])) /
+                           (log2(knots[i]) - log2(knots[i - 1])) - mean_deriv;
+      pen -= d * d;
+    }
+    return alpha * pen;
+  }
+};
+
+/* include/PDDouble.h: a 2D P(D), n1 x n2, band 1 the slow index */
+class PDDouble {
+ private:
+  std::vector<double> store;
+
+ public:
+  unsigned int n1 = 0, n2 = 0;
+  bool logflat = false;
+  double minflux1 = 0.0, dflux1 = 1.0, minflux2 = 0.0, dflux2 = 1.0;
+  double* pd_ = nullptr;
+  explicit PDDouble(unsigned int N1 = 0, double MINFLUX1 = 0.0, double DFLUX1 = 1.0,
+                    unsigned int N2 = 0, double MINFLUX2 = 0.0, double DFLUX2 = 1.0)
+      : store(static_cast<size_t>(N1) * N2, 0.0), n1(N1), n2(N2), minflux1(MINFLUX1),
+        dflux1(DFLUX1), minflux2(MINFLUX2), dflux2(DFLUX2) {
+    pd_ = store.empty() ? nullptr : store.data();
+  }
+  void resize(unsigned int N1, unsigned int N2) {
+    store.resize(static_cast<size_t>(N1) * N2);
+    n1 = N1;
+    n2 = N2;
+    pd_ = store.empty() ? nullptr : store.data();
+  }
+  unsigned int getDim1() const { return n1; }
+  unsigned int getDim2() const { return n2; }
+  bool isLog() const { return logflat; }
+  double getFluxVal1(unsigned int i) const { return minflux1 + static_cast<double>(i) * dflux1; }
+  double getFluxVal2(unsigned int j) const { return minflux2 + static_cast<double>(j) * dflux2; }
+  double getPDVal(unsigned int i, unsigned int j) const { return pd_[static_cast<size_t>(i) * n2 + j]; }
+};
+
+namespace pofd_amd_detail {
+inline void throw_status2(pofd_ctx* c, int st, const char* cls, const char* meth) {
+  const std::string msg = c ? pofd_last_error(c) : "";
+  switch (st) {
+    case POFD_E_SPLIT:
+      throw affineExcept("PDFactoryDouble", "findSplitPoint",
+                         "Unable to find split point" + (msg.empty() ? std::string() : ": " + msg));
+    case POFD_E_INVALID_MODEL:
+      throw affineExcept("PDFactoryDouble", "initPD", "Invalid model");
+    case POFD_E_MEAN:
+      throw affineExcept("PDFactoryDouble", "unwrapAndNormalizePD", "Un-shift amounts not finite");
+    case POFD_E_NOGPU:
+      throw affineExcept(cls, meth, "no gfx950 GPU available");
+    default:
+      throw affineExcept(cls, meth, msg.empty() ? "device path failed" : msg);
+  }
+}
+inline std::vector<float> theta2(const numberCountsDoubleLogNormal& m, float sm1, float sm2) {
+  std::vector<float> th(m.paramValues());
+  th.push_back(sm1);
+  th.push_back(sm2);
+  return th;
+}
+}  // namespace pofd_amd_detail
+
+/* include/PDFactoryDouble.h: initPD + getPD for one model and one pair of
+   noise levels; each getPD runs the device chain for a single walker */
+class PDFactoryDouble {
+ private:
+  unsigned int nedge;
+  unsigned int n = 0;
+  double range[4] = {0.0, 0.0, 0.0, 0.0};
+  bool setEdge = true;
+  numberCountsDoubleLogNormal model;
+  const doublebeam* bm = nullptr;
+  bool initialized = false;
+  int device = pofd_amd_detail::default_device();
+
+ public:
+  explicit PDFactoryDouble(unsigned int NEDGE = 256) : nedge(NEDGE) {}
+  void setNEdge(unsigned int v) { nedge = v; }
+  unsigned int getNEdge() const { return nedge; }
+  void setDevice(int d) { device = d; }
+  /* PDFactoryDouble::initPD, src/PDFactoryDouble.cc:1143-1367 (checks as
+     written); the beam must outlive the factory */
+  bool initPD(unsigned int N, double minflux1, double maxflux1, double minflux2,
+              double maxflux2, const numberCountsDoubleLogNormal& m, const doublebeam& b,
+              bool setedge = true) {
+    if (N == 0) throw affineExcept("PDFactoryDouble", "initPD", "Invalid (non-positive) n");
+    if (maxflux1 <= 0.0 || maxflux2 <= 0.0)
+      throw affineExcept("PDFactoryDouble", "initPD", "Invalid (non-positive) maxflux");
+    if (!m.isValid()) throw affineExcept("PDFactoryDouble", "initPD", "Invalid model");
+    if (!b.hasData()) throw affineExcept("PDFactoryDouble", "initPD", "Empty beam");
+    n = N;
+    range[0] = minflux1;
+    range[1] = maxflux1;
+    range[2] = minflux2;
+    range[3] = maxflux2;
+    setEdge = setedge;
+    model = m;
+    bm = &b;
+    initialized = true;
+    return true;
+  }
+  /* PDFactoryDouble::getPD, src/PDFactoryDouble.cc:1382-1499 */
+  void getPD(double sigma1, double sigma2, PDDouble& pd, bool setLog = true) {
+    if (!initialized) throw affineExcept("PDFactoryDouble", "getPD", "Must call initPD first");
+    using namespace pofd_amd_detail;
+    CtxPtr c = make_ctx(device, "PDFactoryDouble", "getPD");
+    const auto& K = model.knotPositions();
+    const auto& S = model.sigmaPositions();
+    const auto& O = model.offsetPositions();
+    int rc = pofd2d_configure(c.get(), n, nedge, setEdge ? 1 : 0, K.data(),
+                              static_cast<uint32_t>(K.size()), S.data(),
+                              static_cast<uint32_t>(S.size()), O.data(),
+                              static_cast<uint32_t>(O.size()), model.getMinSigma(), 0.0, 0.0);
+    int b = 0, f = 0;
+    if (!rc) rc = pofd2d_add_beamset(c.get(), bm->handle(), &b);
+    const double z = 0.0;
+    if (!rc) rc = pofd2d_add_field(c.get(), b, &z, &z, 1, nullptr, 0, 0, nullptr, sigma1, sigma2, 1.0, &f);
+    std::vector<float> th = theta2(model, 1.0f, 1.0f);
+    if (!rc) rc = pofd2d_prepare(c.get(), th.data(), static_cast<uint32_t>(th.size()));
+    if (!rc) rc = pofd2d_set_rrange(c.get(), b, range);
+    if (rc) throw_status2(c.get(), rc, "PDFactoryDouble", "getPD");
+    double ll = 0.0;
+    int32_t st = 0;
+    rc = pofd2d_loglike_batch(c.get(), th.data(), 1, static_cast<uint32_t>(th.size()), &ll, &st);
+    if (rc) throw_status2(c.get(), rc, "PDFactoryDouble", "getPD");
+    if (st) throw_status2(c.get(), st, "PDFactoryDouble", "getPD");
+    pd.resize(n, n);
+    double info[8];
+    rc = pofd2d_get_pd(c.get(), 0, b, f, setLog ? 1 : 0, pd.pd_, info);
+    if (rc) throw_status2(c.get(), rc, "PDFactoryDouble", "getPD");
+    pd.minflux1 = info[0];
+    pd.minflux2 = info[1];
+    pd.dflux1 = info[2];
+    pd.dflux2 = info[3];
+    pd.logflat = setLog;
+  }
+};
+
+/* include/calcLikeDouble.h: data sets grouped by beam pair, priors, the
+   two-band likelihood */
+class calcLikeDouble {
+ private:
+  unsigned int fftsize, nedge;
+  bool edgeInteg, bin_data;
+  unsigned int nbins;
+  mutable numberCountsDoubleLogNormal model;
+  std::vector<fitsDataDouble> datasets;
+  std::vector<std::pair<doublebeam, std::vector<size_t>>> sets;  // beam pair, dataset idx
+  std::vector<double> sigmas1, sigmas2, like_norms;
+  double exp_conf1 = 0.0, exp_conf2 = 0.0;
+  bool has_sigma_prior1 = false, has_sigma_prior2 = false;
+  double sigma_prior_width1 = 0.0, sigma_prior_width2 = 0.0;
+  double regularization_alpha = 0.0;
+  int device = pofd_amd_detail::default_device();
+  mutable pofd_amd_detail::CtxPtr ctx;
+  mutable bool prepared = false;
+
+ public:
+  explicit calcLikeDouble(unsigned int FFTSIZE = 4096, unsigned int NEDGE = 256,
+                          bool EDGEINTEG = true, bool BINNED = false, unsigned int NBINS = 1000)
+      : fftsize(FFTSIZE), nedge(NEDGE), edgeInteg(EDGEINTEG), bin_data(BINNED), nbins(NBINS) {}
+  calcLikeDouble(const calcLikeDouble&) = delete;
+  calcLikeDouble& operator=(const calcLikeDouble&) = delete;
+
+  void setDevice(int d) { device = d; prepared = false; }
+  void setFFTSize(unsigned int v) noexcept { fftsize = v; prepared = false; }
+  unsigned int getFFTSize() const noexcept { return fftsize; }
+  void setEdgeInteg() noexcept { edgeInteg = true; prepared = false; }
+  void unSetEdgeInteg() noexcept { edgeInteg = false; prepared = false; }
+  bool getEdgeInteg() const noexcept { return edgeInteg; }
+  void setNEdge(unsigned int v) { nedge = v; prepared = false; }
+  unsigned int getNEdge() const noexcept { return nedge; }
+  /* setBinData / unSetBinData / setNBins, src/calcLikeDouble.cc:1037-1092 */
+  void setBinData() {
+    bin_data = true;
+    for (auto& d : datasets) d.applyBinning(nbins, nbins);
+    prepared = false;
+  }
+  void unSetBinData() {
+    bin_data = false;
+    for (auto& d : datasets) d.removeBinning();
+    prepared = false;
+  }
+  void setNBins(unsigned int v) {
+    nbins = v;
+    if (bin_data)
+      for (auto& d : datasets) d.applyBinning(nbins, nbins);
+    prepared = false;
+  }
+
+  /* calcLikeDouble::readDataFromFiles, src/calcLikeDouble.cc:931-1024:
+     datasets are grouped by the pair of beam files, groups ordered by the
+     key "beam1:beam2" (std::map) */
+  void readDataFromFiles(const std::vector<std::string>& datafiles1,
+                         const std::vector<std::string>& datafiles2,
+                         const std::vector<std::string>& beamfiles1,
+                         const std::vector<std::string>& beamfiles2,
+                         const std::vector<double>& sigs1, const std::vector<double>& sigs2,
+                         const std::vector<double>& lnorms, bool IGNOREMASK = false,
+                         bool MEANSUB = false, double MINBEAMVAL = 1e-6,
+                         bool HISTOGRAM = false, unsigned int NBEAMHIST = 160,
+                         double EXPCONF1 = 0.0, double EXPCONF2 = 0.0) {
+    const size_t ndat = datafiles1.size();
+    const char* cls = "calcLikeDouble";
+    const char* meth = "readDataFromFiles";
+    if (ndat != datafiles2.size()) throw affineExcept(cls, meth, "Datafiles1 and datafile2 not same length");
+    if (ndat == 0) throw affineExcept(cls, meth, "No datafiles");
+    if (ndat != beamfiles1.size()) throw affineExcept(cls, meth, "datafiles and beamfiles1 not same length");
+    if (ndat != beamfiles2.size()) throw affineExcept(cls, meth, "datafiles and beamfiles2 not same length");
+    if (ndat != sigs1.size()) throw affineExcept(cls, meth, "datafiles and sigmas1 not same length");
+    if (ndat != sigs2.size()) throw affineExcept(cls, meth, "datafiles and sigmas2 not same length");
+    if (ndat != lnorms.size()) throw affineExcept(cls, meth, "datafiles and like_norm not same length");
+    std::map<std::string, std::vector<size_t>> groups;
+    for (size_t i = 0; i < ndat; ++i) groups[beamfiles1[i] + ":" + beamfiles2[i]].push_back(i);
+    datasets.clear();
+    sets.clear();
+    sigmas1.clear();
+    sigmas2.clear();
+    like_norms.clear();
+    for (auto& g : groups) {
+      const size_t first = g.second.front();
+      doublebeam bm(beamfiles1[first], beamfiles2[first], HISTOGRAM, NBEAMHIST, MINBEAMVAL);
+      std::vector<size_t> idx;
+      for (size_t i : g.second) {
+        datasets.emplace_back();
+        fitsDataDouble& d = datasets.back();
+        d.readData(datafiles1[i], datafiles2[i], IGNOREMASK, MEANSUB);
+        if (bin_data) d.applyBinning(nbins, nbins);
+        sigmas1.push_back(sigs1[i]);
+        sigmas2.push_back(sigs2[i]);
+        like_norms.push_back(lnorms[i]);
+        idx.push_back(datasets.size() - 1);
+      }
+      sets.emplace_back(std::move(bm), idx);
+    }
+    exp_conf1 = EXPCONF1;
+    exp_conf2 = EXPCONF2;
+    prepared = false;
+  }
+
+  unsigned int getNBeamSets() const noexcept { return static_cast<unsigned>(sets.size()); }
+  unsigned int getNKnots() const noexcept { return model.getNKnots(); }
+  unsigned int getNSigmas() const noexcept { return model.getNSigmas(); }
+  unsigned int getNOffsets() const noexcept { return model.getNOffsets(); }
+  void setKnotPositions(std::vector<double>& k) { model.setKnotPositions(k); prepared = false; }
+  void setSigmaPositions(std::vector<double>& k) { model.setSigmaPositions(k); prepared = false; }
+  void setOffsetPositions(std::vector<double>& k) { model.setOffsetPositions(k); prepared = false; }
+  void setPositions(std::vector<double>& K, std::vector<double>& S, std::vector<double>& O) {
+    model.setPositions(K, S, O);
+    prepared = false;
+  }
+  void setMinSigma(double v) { model.setMinSigma(v); prepared = false; }
+
+  /* calcLikeDouble::setRRanges, src/calcLikeDouble.cc:1093-1097: the R
+     ranges depend on the model at p (the band-2 maximum flux) */
+  void setRRanges(const paramSet& p) {
+    using namespace pofd_amd_detail;
+    model.setParams(p);
+    ctx = make_ctx(device, "calcLikeDouble", "setRRanges");
+    const auto& K = model.knotPositions();
+    const auto& S = model.sigmaPositions();
+    const auto& O = model.offsetPositions();
+    int rc = pofd2d_configure(ctx.get(), fftsize, nedge, edgeInteg ? 1 : 0, K.data(),
+                              static_cast<uint32_t>(K.size()), S.data(),
+                              static_cast<uint32_t>(S.size()), O.data(),
+                              static_cast<uint32_t>(O.size()), model.getMinSigma(), exp_conf1,
+                              exp_conf2);
+    for (size_t s = 0; s < sets.size() && !rc; ++s) {
+      int b = 0;
+      rc = pofd2d_add_beamset(ctx.get(), sets[s].first.handle(), &b);
+      for (size_t i : sets[s].second) {
+        if (rc) break;
+        const fitsDataDouble& d = datasets[i];
+        int f = 0;
+        if (d.isBinned())
+          rc = pofd2d_add_field(ctx.get(), b, d.getData1(), d.getData2(), d.getN(),
+                                reinterpret_cast<const uint32_t*>(d.getBinnedData()),
+                                d.getNBins().first, d.getNBins().second, d.binInfo(),
+                                sigmas1[i], sigmas2[i], like_norms[i], &f);
+        else
+          rc = pofd2d_add_field(ctx.get(), b, d.getData1(), d.getData2(), d.getN(), nullptr,
+                                0, 0, nullptr, sigmas1[i], sigmas2[i], like_norms[i], &f);
+      }
+    }
+    if (!rc) rc = pofd2d_prepare(ctx.get(), p.data(), p.getNParams());
+    if (rc) throw_status2(ctx.get(), rc, "calcLikeDouble", "setRRanges");
+    prepared = true;
+  }
+  void setSigmaPrior1(double val) noexcept { has_sigma_prior1 = true; sigma_prior_width1 = val; }
+  void unsetSigmaPrior1() noexcept { has_sigma_prior1 = false; }
+  void setSigmaPrior2(double val) noexcept { has_sigma_prior2 = true; sigma_prior_width2 = val; }
+  void unsetSigmaPrior2() noexcept { has_sigma_prior2 = false; }
+  void setCFIRBPrior1(double, double) {
+    throw affineExcept("calcLikeDouble", "setCFIRBPrior1", "not supported by the device path");
+  }
+  void setCFIRBPrior2(double, double) {
+    throw affineExcept("calcLikeDouble", "setCFIRBPrior2", "not supported by the device path");
+  }
+  void setPoissonPrior1(double, double) {
+    throw affineExcept("calcLikeDouble", "setPoissonPrior1", "not supported by the device path");
+  }
+  void setPoissonPrior2(double, double) {
+    throw affineExcept("calcLikeDouble", "setPoissonPrior2", "not supported by the device path");
+  }
+  void setRegularizationAlpha(double alpha) {
+    if (alpha < 0.0)
+      throw affineExcept("calcLikeDouble", "setRegularizationAlpha", "Invalid (negative) regularization alpha");
+    regularization_alpha = alpha;
+  }
+
+  /* calcLikeDouble::getLogLike, src/calcLikeDouble.cc:1195-1281 */
+  double getLogLike(const paramSet& p, bool& pars_invalid) const {
+    double out = 0.0;
+    int st = 0;
+    pars_invalid = false;  /* never set by the reference either */
+    getLogLikeBatch(p.data(), 1, p.getNParams(), &out, &st);
+    if (st && st != POFD_E_BADLIKE)
+      pofd_amd_detail::throw_status2(ctx.get(), st, "calcLikeDouble", "getLogLike");
+    return out;
+  }
+
+  /* many walkers per launch: theta rows [knots..., sigmas..., offsets...,
+     sigmult1, sigmult2, ...] */
+  void getLogLikeBatch(const float* theta, unsigned int nwalk, unsigned int stride,
+                       double* loglike, int* status) const {
+    const double half_log_2pi = 0.918938517570495605469;
+    if (sets.empty()) {
+      for (unsigned w = 0; w < nwalk; ++w) loglike[w] = NAN;
+      return;
+    }
+    const unsigned nmp = model.getNParams();
+    if (stride < nmp + 2) throw affineExcept("calcLikeDouble", "getLogLike", "Not enough elements in params");
+    if (!prepared) throw affineExcept("calcLikeDoubleSingle", "getLogLike", "R range not set");
+    int rc = pofd2d_loglike_batch(ctx.get(), theta, nwalk, stride, loglike,
+                                  reinterpret_cast<int32_t*>(status));
+    if (rc) pofd_amd_detail::throw_status2(ctx.get(), rc, "calcLikeDouble", "getLogLike");
+    for (unsigned w = 0; w < nwalk; ++w) {
+      if (status[w] != 0) continue;
+      const float* t = theta + static_cast<size_t>(w) * stride;
+      if (has_sigma_prior1) {
+        const double val = (static_cast<double>(t[nmp]) - 1.0) / sigma_prior_width1;
+        loglike[w] -= half_log_2pi + log(sigma_prior_width1) + 0.5 * val * val;
+      }
+      if (has_sigma_prior2) {
+        const double val = (static_cast<double>(t[nmp + 1]) - 1.0) / sigma_prior_width2;
+        loglike[w] -= half_log_2pi + log(sigma_prior_width2) + 0.5 * val * val;
+      }
+      if (regularization_alpha > 0.0) {
+        numberCountsDoubleLogNormal m = model;
         m.setParams(paramSet(stride, t));
         loglike[w] += m.differenceRegularize(regularization_alpha);
       }

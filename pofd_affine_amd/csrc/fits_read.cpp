@@ -222,6 +222,96 @@ extern "C" int pofd_fits_read_beam(const char* path, double* pix,
   return POFD_OK;
 }
 
+namespace {
+// image HDU ("file[k]", else the image/IMAGE extension, else the primary)
+// and, unless ignored, the mask/MASK extension of a map file
+int load_map(const char* path, bool ignore_mask, std::vector<double>& img_out,
+             std::vector<double>& mask_out, const char* who) {
+  std::string base;
+  int ext;
+  split_ext(path, base, ext);
+  std::vector<Hdu> hdus;
+  int rc = read_file(base, hdus, g_fits_err);
+  if (rc) return rc;
+  const Hdu* img = nullptr;
+  if (ext >= 0) {
+    if (ext >= static_cast<int>(hdus.size())) {
+      g_fits_err = std::string(who) + ": no such extension";
+      return POFD_E_ARG;
+    }
+    img = &hdus[ext];
+  } else {
+    for (const Hdu& h : hdus)
+      if (name_is(h.extname, "image")) {
+        img = &h;
+        break;
+      }
+    if (!img) img = &hdus[0];
+  }
+  if (img->dims.size() != 2 || img->data.empty()) {
+    g_fits_err = std::string(who) + ": Image is not 2D";
+    return POFD_E_ARG;
+  }
+  mask_out.clear();
+  if (!ignore_mask)
+    for (const Hdu& h : hdus)
+      if (name_is(h.extname, "mask")) {
+        if (h.dims != img->dims) {
+          g_fits_err = std::string(who) + ": Mask does not match image extent";
+          return POFD_E_ARG;
+        }
+        mask_out = h.data;
+        break;
+      }
+  img_out = img->data;
+  return POFD_OK;
+}
+}  // namespace
+
+// Pixel pairs of two maps as fitsDataDouble::readData (src/fitsDataDouble.cc:
+// 305-406): masks are unsigned integers; with both present a pixel is kept
+// when (mask1 & mask2) == 0, with one when it is 0.
+extern "C" int pofd_fits_read_map_pair(const char* path1, const char* path2,
+                                       int ignore_mask, double* d1, double* d2,
+                                       uint64_t capacity, uint64_t* n) {
+  if (!path1 || !path2 || !n) return POFD_E_ARG;
+  std::vector<double> i1, m1, i2, m2;
+  int rc = load_map(path1, ignore_mask != 0, i1, m1, "fitsDataDouble::readData");
+  if (rc) return rc;
+  rc = load_map(path2, ignore_mask != 0, i2, m2, "fitsDataDouble::readData");
+  if (rc) return rc;
+  if (i1.size() != i2.size()) {
+    g_fits_err = "fitsDataDouble::readData: Data from file1 and file2 not the same size";
+    return POFD_E_ARG;
+  }
+  auto mval = [](double v) { return static_cast<uint32_t>(static_cast<int64_t>(v)); };
+  std::vector<size_t> keep;
+  keep.reserve(i1.size());
+  for (size_t i = 0; i < i1.size(); ++i) {
+    bool k = true;
+    if (!m1.empty() && !m2.empty())
+      k = (mval(m1[i]) & mval(m2[i])) == 0;
+    else if (!m1.empty())
+      k = mval(m1[i]) == 0;
+    else if (!m2.empty())
+      k = mval(m2[i]) == 0;
+    if (k) keep.push_back(i);
+  }
+  if (keep.empty()) {
+    g_fits_err = "fitsDataDouble::readData: All data masked";
+    return POFD_E_ARG;
+  }
+  *n = keep.size();
+  if (d1 || d2) {
+    if (!d1 || !d2 || capacity < keep.size()) return POFD_E_ARG;
+    for (size_t k = 0; k < keep.size(); ++k) {
+      d1[k] = i1[keep[k]];
+      d2[k] = i2[keep[k]];
+    }
+  }
+  return POFD_OK;
+}
+
 extern "C" int pofd_fits_read_map(const char* path, int ignore_mask,
                                   double* data, uint64_t capacity,
                                   uint64_t* n) {

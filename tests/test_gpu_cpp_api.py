@@ -50,3 +50,54 @@ def test_cpp_getlike(tmp_path):
         assert err == "error PDFactory::initPD"
         pd = [l for l in lines if l.startswith("pd")][0].split()
         assert int(pd[3]) == 32768 and float(pd[2]) > 0
+
+
+EXE2 = os.path.join(ROOT, "tests", "cpp", "_build", "getlike2d_check")
+
+
+def test_cpp_getlike2d(tmp_path):
+    """calcLikeDouble::readDataFromFiles -> setRRanges -> getLogLike on FITS
+    map and beam pairs, as pofd_affine_getLike drives the two-band path,
+    against the 2D oracle on the same inputs."""
+    from pofd_affine_amd import synth2d
+    from oracle2d_side import build_oracle2d, make_inputs2d
+    print("[gpu-case] C++ calcLikeDouble on FITS inputs", flush=True)
+    assert os.path.exists(EXE2), "build tests/cpp first (__graft_entry__.build)"
+    side = 120
+    for binned in (0, 1):
+        inp = make_inputs2d(npix_side=side, seed=5, nbeamhist=120,
+                            nbins=60 if binned else None)
+        nb = inp["beam1"].shape[0]
+        for k, name in ((1, "beam1"), (2, "beam2")):
+            fitsio.write_image(str(tmp_path / ("b%d.fits" % k)),
+                               inp[name].reshape(nb, nb),
+                               extra={"PIXSIZE": inp["pixsize"]})
+            fitsio.write_image(str(tmp_path / ("m%d.fits" % k)),
+                               inp["d%d" % k].reshape(side, side),
+                               extname="image")
+        kp, sp, op = synth2d.positions()
+        with open(tmp_path / "pos.txt", "w") as fh:
+            for v in (kp, sp, op):
+                fh.write(" ".join("%.17g" % x for x in v) + "\n")
+        th = synth2d.walker_thetas(4, seed=21)[:, :14]
+        np.savetxt(tmp_path / "theta.txt", th.astype(np.float64), fmt="%.9g")
+        out = subprocess.run(
+            [EXE2, "512", "64", str(binned), "60", "1e-6", "120",
+             str(synth2d.SIGMA1), str(synth2d.SIGMA2),
+             str(tmp_path / "m1.fits"), str(tmp_path / "m2.fits"),
+             str(tmp_path / "b1.fits"), str(tmp_path / "b2.fits"),
+             str(tmp_path / "pos.txt"), str(tmp_path / "theta.txt")],
+            capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        lines = out.stdout.split("\n")
+        got = np.array([float(l.split()[1]) for l in lines
+                        if l.startswith("loglike")])
+        L = build_oracle2d(inp, 512, th[0], nedge=64, binned=bool(binned))
+        thp = np.hstack([th, np.zeros((4, 4), np.float32)])
+        ref, rst = L.loglike_batch(thp, 4)
+        assert np.all(rst == 0)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+        err = [l for l in lines if l.startswith("error")][0]
+        assert err == "error PDFactoryDouble::initPD"
+        pd = [l for l in lines if l.startswith("pd")][0].split()
+        assert int(pd[5]) == 512 and int(pd[6]) == 512

@@ -129,6 +129,47 @@ def test_fits_reader_c_matches_python(refdata, tmp_path):
     assert np.array_equal(_fits_map_c(f, 1), img.ravel())
 
 
+def test_fits_map_pair_c_matches_python(refdata, tmp_path):
+    """fitsDataDouble::readData's combined masks: the C reader and the
+    Python one agree on the reference's two-band maps and on synthetic
+    maps with one and with two (bitwise-combined) masks."""
+    L = _lib.lib()
+    L.pofd_fits_read_map_pair.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+
+    def pair_c(p1, p2, ig):
+        n = ctypes.c_uint64(0)
+        assert L.pofd_fits_read_map_pair(p1.encode(), p2.encode(), ig, None,
+                                         None, 0, ctypes.byref(n)) == 0
+        a, b = np.zeros(n.value), np.zeros(n.value)
+        assert L.pofd_fits_read_map_pair(p1.encode(), p2.encode(), ig,
+                                         a.ctypes.data, b.ctypes.data,
+                                         n.value, ctypes.byref(n)) == 0
+        return a, b
+
+    p1 = os.path.join(refdata, "testmodel2D_band1.fits")
+    p2 = os.path.join(refdata, "testmodel2D_band2.fits")
+    for ig in (0, 1):
+        a, b = pair_c(p1, p2, ig)
+        ra, rb = fitsio.read_map_pair(p1, p2, bool(ig))
+        assert np.array_equal(a, ra) and np.array_equal(b, rb)
+    rng = np.random.default_rng(4)
+    i1, i2 = rng.normal(size=(2, 11, 9))
+    m1 = rng.integers(0, 4, size=(11, 9)).astype(np.int16)
+    m2 = rng.integers(0, 4, size=(11, 9)).astype(np.int16)
+    f1, f2, f3 = (str(tmp_path / x) for x in ("a.fits", "b.fits", "c.fits"))
+    fitsio.write_image(f1, i1, extname="image", mask=m1)
+    fitsio.write_image(f2, i2, extname="image", mask=m2)
+    fitsio.write_image(f3, i2, extname="image")
+    for q1, q2, keep in ((f1, f2, (m1 & m2) == 0), (f1, f3, m1 == 0),
+                         (f3, f2, m2 == 0)):
+        a, b = pair_c(q1, q2, 0)
+        ra, rb = fitsio.read_map_pair(q1, q2)
+        assert np.array_equal(a, ra) and np.array_equal(b, rb)
+        assert a.size == int(keep.sum())
+
+
 def test_dbeam_prep_bit_exact(refdata):
     """doublebeam::setBeams + makeHistogram in the library equal the oracle's
     restatement bit for bit, on the reference's two-band beam fixtures

@@ -113,3 +113,74 @@ def test_data2_binning_centres():
     ref = np.zeros((40, 30), dtype=np.uint32)
     np.add.at(ref, (i1, i2), 1)
     assert np.array_equal(ref, bins)
+
+
+def test_r2d_grid_and_edge_integrals(small2d):
+    """R on the grid (initR with setEdge, src/PDFactoryDouble.cc:376-551):
+    away from the axes it is the array getR (pinned by the reference's golden
+    values) times dflux1 dflux2; the first row, first column and corner are
+    the reference's trapezoid averages of R over the cells [0, dflux] (log
+    spaced in band 1, linear in band 2).  With many edge points these must
+    converge to the exact cell averages, computed here by dense quadrature
+    of getR -- which a wrong edge normalisation, edge grid or axis fails."""
+    from scipy import integrate
+    inp, th = small2d
+    n = 512
+    L = build_oracle2d(inp, n, th, nedge=4096)
+    ip = L.initpd(0, th)
+    R = ip["rvals"]
+    d1f, d2f = ip["dflux1"], ip["dflux2"]
+    assert int(ip["wrap1"]) == n - 1  # positive beams: no wrapped fluxes
+    kp, sp, op = synth2d.positions()
+    m = O2.LogNormalModel(kp, sp, op)
+    m.set_params(th[:len(kp) + len(sp) + len(op)])
+    bm = O2.DoubleBeam(inp["beam1"], inp["beam2"], inp["pixsize"],
+                       inp["minbeamval"], inp["nbeamhist"])
+    rng = np.random.default_rng(2)
+    rows = rng.choice(np.arange(1, n), 12, replace=False)
+    ref = m.getR(rows * d1f, np.arange(1, n) * d2f, bm) * d1f * d2f
+    np.testing.assert_allclose(R[rows, 1:], ref, rtol=1e-12, atol=0)
+    lo1, lo2 = 1e-9 * d1f, 1e-9 * d2f  # lowEdgeRMult
+    cell = d1f * d2f
+    # first row: band-1 average over [lo1, dflux1] of R(f1, f2_j), dense in
+    # ln f1 (the integrand has narrow log-normal features)
+    js = np.array(list(np.argsort(R[0, 1:])[::-1][:200:25] + 1))
+    u = np.linspace(math.log(lo1), math.log(d1f), 200001)
+    G = m.getR(np.exp(u), js * d2f, bm) * np.exp(u)[:, None]
+    row0 = np.trapezoid(G, u, axis=0) / (d1f - lo1) * cell
+    np.testing.assert_allclose(R[0, js], row0, rtol=2e-3)
+    # first column: band-2 average over [lo2, dflux2] of R(f1_i, f2)
+    iis = list(np.argsort(R[1:, 0])[::-1][:200:25] + 1)
+    for i in iis:
+        v, _ = integrate.quad(lambda y: m.getR1(i * d1f, y, bm), lo2, d2f,
+                              limit=400, epsrel=1e-10)
+        assert R[i, 0] == pytest.approx(v / (d2f - lo2) * cell, rel=2e-3), i
+    # corner: both averages
+    u = np.linspace(math.log(lo1), math.log(d1f), 40001)
+    y = np.linspace(lo2, d2f, 401)
+    G = m.getR(np.exp(u), y, bm) * np.exp(u)[:, None]
+    c00 = np.trapezoid(np.trapezoid(G, y, axis=1), u) / (
+        (d1f - lo1) * (d2f - lo2)) * cell
+    # (slowest to converge: linear edge grid in band 2 down to f2 -> 0)
+    assert R[0, 0] == pytest.approx(c00, rel=1e-2)
+
+
+def test_split_points_2d_properties(small2d):
+    """findSplitPoint on both marginals (src/PDFactoryDouble.cc:784-975) has
+    no reference fixture: after the rotation the marginals start and end in
+    their tails (below f2 = 1.5e-5 of the peak) and the P(D) has zero mean
+    on both axes."""
+    inp, th = small2d
+    n = 512
+    L = build_oracle2d(inp, n, th, nedge=64)
+    L.initpd(0, th, want_arrays=False)
+    pd, info = L.getpd(0, 0, 1.0, 1.0, setlog=False)
+    p1, p2 = pd.sum(1), pd.sum(0)
+    for p in (p1, p2):
+        assert max(p[0], p[-1]) < 1.5e-5 * p.max()
+    x1 = info["minflux1"] + np.arange(n) * info["dflux1"]
+    x2 = info["minflux2"] + np.arange(n) * info["dflux2"]
+    assert abs(np.sum(x1 * p1) / p1.sum()) < 1e-3 * info["dflux1"]
+    assert abs(np.sum(x2 * p2) / p2.sum()) < 1e-3 * info["dflux2"]
+    assert info["split1"] * info["dflux1"] >= 2.25 * info["sgpos1"]
+    assert info["split2"] * info["dflux2"] >= 2.25 * info["sgpos2"]

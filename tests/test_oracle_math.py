@@ -271,3 +271,24 @@ def test_negative_beam_pd_cumulants_campbell():
     pos = xr > 0
     assert abs(np.sum(xr[pos] ** 3 * ip["rvals"][pos]) - k3_campbell) > \
         1e-4 * abs(k3_campbell)
+
+
+def test_split_point_properties():
+    """findSplitPoint (src/PDFactory.cc:691-921) has no reference fixture.
+    Its contract: the unrotated P(D) is cut where it is below f2 = 1e-5 of
+    its peak, at least n1 = 2.5 sigma above zero flux, so the rotated,
+    normalised P(D) starts and ends in its tails and has zero mean."""
+    sigma = 0.0025
+    L, th, _ = _small_setup(32768, 512, sigma)
+    for w in range(3):
+        t = th.copy()
+        t[:8] += 0.05 * w
+        L.initpd(0, t)
+        pd = L.getpd(0, 0, 1.0, setlog=False)
+        raw, p = pd["pofd_raw"], pd["pd"]
+        s = int(pd["split"])
+        assert raw[s - 1] < 1e-5 * raw.max() or raw[s] < 1e-5 * raw.max()
+        assert s * pd["dflux"] >= 2.5 * sigma
+        assert max(p[0], p[-1]) < 1e-5 * p.max()
+        x = pd["minflux"] + np.arange(p.size) * pd["dflux"]
+        assert abs(np.sum(x * p) / np.sum(p)) < 1e-3 * pd["dflux"]
