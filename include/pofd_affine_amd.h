@@ -197,6 +197,15 @@ int pofd1d_get_rinterp(pofd_ctx *ctx, uint32_t walker, int beamset_id,
    the beam has no negative pixels */
 int pofd1d_get_rinterp_neg(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                            double *flux_out, double *log2r_out);
+/* Bonus quantities of the last batch, per walker w: out[3w..3w+2] =
+   getNS, getFluxPerArea, getFluxSqPerArea of the knot model at that
+   walker's theta (numberCountsKnotsSpline::splineInt,
+   src/numberCountsKnotsSpline.cc:300-343; calcLike::getLogLike computes the
+   last two for the CFIRB and Poisson priors and fillBonusParams,
+   src/calcLike.cc:932-948, 958-980).  Integrated to double precision (the
+   reference's GSL QAG-41 stops at relative 1e-5); NaN for an invalid
+   model. */
+int pofd1d_get_bonus(pofd_ctx *ctx, uint32_t nwalk, double *out);
 /* With POFD_OPT_STAGE_TIMING on: waits for the recorded batches, writes the
    summed GPU milliseconds of each stage (ms_out[POFD_NSTAGES]) over every
    beam-set launch since the previous call, *nlaunch the number of such
@@ -285,6 +294,16 @@ int pofd2d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
                                 uint32_t nwalk, uint32_t theta_stride,
                                 double *d_loglike, int32_t *d_status,
                                 void *hip_stream);
+
+/* Bonus quantities of the last batch, per walker w: out[5w..5w+4] = getNS,
+   getFluxPerArea(0), getFluxPerArea(1), getFluxSqPerArea(0),
+   getFluxSqPerArea(1) of the colour model at that walker's theta
+   (numberCountsDoubleLogNormal::splineInt, src/numberCountsDoubleLogNormal.cc:
+   1549-1673; used by calcLikeDouble's CFIRB/Poisson priors and
+   fillBonusParams, src/calcLikeDouble.cc:1244-1316).  Integrated to double
+   precision (the reference's QAG-41 stops at relative 1e-5); NaN for an
+   invalid model. */
+int pofd2d_get_bonus(pofd_ctx *ctx, uint32_t nwalk, double *out);
 
 /* ---- parity and debugging (PDFactoryDouble::getPD output) ---- */
 /* info[32]: status, split1, split2, sg1pos, sg2pos, sigma1, sigma2,

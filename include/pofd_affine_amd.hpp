@@ -17,9 +17,10 @@
  *
  * Differences a caller can observe:
  *  - calcLike::getLogLikeBatch evaluates many walkers in one launch;
- *  - the CFIRB and Poisson priors and the bonus parameters
- *    (fillBonusParams) throw affineExcept("...", "...",
- *    "not supported by the device path");
+ *  - the bonus quantities behind the CFIRB and Poisson priors and
+ *    fillBonusParams (mean flux and flux^2 per area) are integrated to
+ *    double precision on the device; the reference's GSL QAG-41 stops at
+ *    relative 1e-5;
  *  - PDFactoryDouble keeps a pointer to the doublebeam given to initPD,
  *    which must outlive it;
  *  - PD::getLogLike is not provided: log likelihoods come from
@@ -493,10 +494,15 @@ class calcLike {
   double exp_conf = 0.0;
   bool has_sigma_prior = false;
   double sigma_prior_width = 0.0;
+  bool has_cfirb_prior = false, has_poisson_prior = false;
+  double cfirb_prior_mean = 0.0, cfirb_prior_sigma = 0.0;
+  double poisson_prior_mean = 0.0, poisson_prior_sigma = 0.0;
   double regularization_alpha = 0.0;
   int device = pofd_amd_detail::default_device();
   mutable pofd_amd_detail::CtxPtr ctx;
   mutable bool prepared = false;
+  mutable paramSet meanParams;  // parameters of the stored bonus values
+  mutable double mean_flux_per_area = NAN, mean_fluxsq_per_area = NAN;
 
   void build() const {
     using namespace pofd_amd_detail;
@@ -610,12 +616,25 @@ class calcLike {
   }
   void setSigmaPrior(double val) noexcept { has_sigma_prior = true; sigma_prior_width = val; }
   void unsetSigmaPrior() noexcept { has_sigma_prior = false; }
-  void setCFIRBPrior(double, double) {
-    throw affineExcept("calcLike", "setCFIRBPrior", "not supported by the device path");
+  /* calcLike::setCFIRBPrior / setPoissonPrior, src/calcLike.cc:839-867 */
+  void setCFIRBPrior(double mn, double sg) {
+    has_cfirb_prior = true;
+    if (mn < 0.0) throw affineExcept("calcLike", "setCFIRBPrior", "Invalid (negative) CFIRB mean");
+    if (sg < 0.0) throw affineExcept("calcLike", "setCFIRBPrior", "Invalid (negative) CFIRB sigma");
+    cfirb_prior_mean = mn;
+    cfirb_prior_sigma = sg;
   }
-  void setPoissonPrior(double, double) {
-    throw affineExcept("calcLike", "setPoissonPrior", "not supported by the device path");
+  void unsetCFIRBPrior() noexcept { has_cfirb_prior = false; }
+  void setPoissonPrior(double mn, double sg) {
+    has_poisson_prior = true;
+    if (mn < 0.0)
+      throw affineExcept("calcLike", "setPoissonPrior", "Invalid (negative) Poisson prior mean");
+    if (sg < 0.0)
+      throw affineExcept("calcLike", "setPoissonPrior", "Invalid (negative) Poisson prior sigma");
+    poisson_prior_mean = mn;
+    poisson_prior_sigma = sg;
   }
+  void unsetPoissonPrior() noexcept { has_poisson_prior = false; }
   void setRegularizationAlpha(double alpha) {
     if (alpha < 0.0)
       throw affineExcept("calcLike", "setRegularizationAlpha", "Invalid (negative) regularization alpha");
@@ -627,15 +646,47 @@ class calcLike {
     double out = 0.0;
     int st = 0;
     pars_invalid = false;  /* never set by the reference either (:911-916) */
-    getLogLikeBatch(p.data(), 1, p.getNParams(), &out, &st);
+    double bonus[3];
+    getLogLikeBatch(p.data(), 1, p.getNParams(), &out, &st, bonus);
     if (st && st != POFD_E_BADLIKE)
       pofd_amd_detail::throw_status(ctx.get(), st, "calcLike", "getLogLike");
+    meanParams = p;
+    mean_flux_per_area = bonus[1];
+    mean_fluxsq_per_area = bonus[2];
     return out;
   }
 
-  /* many walkers per launch: theta rows [knots..., sigma_mult, ...] */
+  /* calcLike::fillBonusParams, src/calcLike.cc:958-980: mean flux and
+     flux^2 per area into par[nknots+1], par[nknots+2]; recomputed (one
+     device evaluation) when par is not the last evaluated set */
+  void fillBonusParams(paramSet& par) const {
+    const unsigned nk = model.getNKnots();
+    if (par.getNParams() < nk + 3)
+      throw affineExcept("calcLike", "fillBonusParams", "Not enough room to fill");
+    float lensum = 0.0f, diffsum = 0.0f;
+    const bool have = meanParams.getNParams() >= nk;
+    for (unsigned i = 0; have && i < nk; ++i) {
+      const float v = meanParams[i], d = v - par[i];
+      lensum += v * v;
+      diffsum += d * d;
+    }
+    const float dist = !have ? 1.0f : (lensum == 0.0f ? std::sqrt(diffsum) : std::sqrt(diffsum / lensum));
+    if (dist > 1e-4f) {
+      double ll = 0.0, bonus[3];
+      int st = 0;
+      getLogLikeBatch(par.data(), 1, par.getNParams(), &ll, &st, bonus);
+      meanParams = par;
+      mean_flux_per_area = bonus[1];
+      mean_fluxsq_per_area = bonus[2];
+    }
+    par[nk + 1] = static_cast<float>(mean_flux_per_area);
+    par[nk + 2] = static_cast<float>(mean_fluxsq_per_area);
+  }
+
+  /* many walkers per launch: theta rows [knots..., sigma_mult, ...];
+     bonus (optional, 3 per walker): getNS, getFluxPerArea, getFluxSqPerArea */
   void getLogLikeBatch(const float* theta, unsigned int nwalk, unsigned int stride,
-                       double* loglike, int* status) const {
+                       double* loglike, int* status, double* bonus = nullptr) const {
     const double half_log_2pi = 0.918938517570495605469;
     if (sets.empty()) {
       for (unsigned w = 0; w < nwalk; ++w) loglike[w] = NAN;
@@ -649,13 +700,30 @@ class calcLike {
     int rc = pofd1d_loglike_batch(ctx.get(), theta, nwalk, stride, loglike,
                                   reinterpret_cast<int32_t*>(status));
     if (rc) pofd_amd_detail::throw_status(ctx.get(), rc, "calcLike", "getLogLike");
+    std::vector<double> bo;
+    const bool need_bonus = bonus || has_cfirb_prior || has_poisson_prior;
+    if (need_bonus) {
+      bo.resize(3 * static_cast<size_t>(nwalk));
+      rc = pofd1d_get_bonus(ctx.get(), nwalk, bo.data());
+      if (rc) pofd_amd_detail::throw_status(ctx.get(), rc, "calcLike", "getLogLike");
+      if (bonus) std::copy(bo.begin(), bo.end(), bonus);
+    }
     for (unsigned w = 0; w < nwalk; ++w) {
-      if (status[w] != 0) continue;
+      // the priors follow the dataset sum unless it threw (bad_like does not)
+      if (status[w] != 0 && status[w] != POFD_E_BADLIKE) continue;
       const float* t = theta + static_cast<size_t>(w) * stride;
       const double sigmult = t[nk];
       if (has_sigma_prior) {
         const double val = (sigmult - 1.0) / sigma_prior_width;
         loglike[w] -= half_log_2pi + log(sigma_prior_width) + 0.5 * val * val;
+      }
+      if (has_cfirb_prior) {  // src/calcLike.cc:932-939
+        const double val = (cfirb_prior_mean - bo[3 * w + 1]) / cfirb_prior_sigma;
+        loglike[w] -= half_log_2pi + log(cfirb_prior_sigma) + 0.5 * val * val;
+      }
+      if (has_poisson_prior) {  // :941-947
+        const double val = (poisson_prior_mean - bo[3 * w + 2]) / poisson_prior_sigma;
+        loglike[w] -= half_log_2pi + log(poisson_prior_sigma) + 0.5 * val * val;
       }
       if (regularization_alpha > 0.0) {
         numberCountsKnotsSpline m = model;
@@ -1114,7 +1182,20 @@ class calcLikeDouble {
   double exp_conf1 = 0.0, exp_conf2 = 0.0;
   bool has_sigma_prior1 = false, has_sigma_prior2 = false;
   double sigma_prior_width1 = 0.0, sigma_prior_width2 = 0.0;
+  bool has_cfirb[2] = {false, false}, has_poisson[2] = {false, false};
+  double cfirb_mean[2] = {0.0, 0.0}, cfirb_sigma[2] = {0.0, 0.0};
+  double poisson_mean[2] = {0.0, 0.0}, poisson_sigma[2] = {0.0, 0.0};
   double regularization_alpha = 0.0;
+  mutable paramSet meanParams;
+  mutable double bonus_[5] = {NAN, NAN, NAN, NAN, NAN};
+  void set_prior(bool* has, double* mn, double* sg, int band, double m, double s,
+                 const char* meth, const char* what) {
+    has[band] = true;
+    if (m < 0.0) throw affineExcept("calcLikeDouble", meth, std::string("Invalid (negative) ") + what + " mean");
+    if (s < 0.0) throw affineExcept("calcLikeDouble", meth, std::string("Invalid (negative) ") + what + " sigma");
+    mn[band] = m;
+    sg[band] = s;
+  }
   int device = pofd_amd_detail::default_device();
   mutable pofd_amd_detail::CtxPtr ctx;
   mutable bool prepared = false;
@@ -1254,18 +1335,23 @@ class calcLikeDouble {
   void unsetSigmaPrior1() noexcept { has_sigma_prior1 = false; }
   void setSigmaPrior2(double val) noexcept { has_sigma_prior2 = true; sigma_prior_width2 = val; }
   void unsetSigmaPrior2() noexcept { has_sigma_prior2 = false; }
-  void setCFIRBPrior1(double, double) {
-    throw affineExcept("calcLikeDouble", "setCFIRBPrior1", "not supported by the device path");
+  /* setCFIRBPrior1/2, setPoissonPrior1/2, src/calcLikeDouble.cc:1105-1174 */
+  void setCFIRBPrior1(double m, double sg) {
+    set_prior(has_cfirb, cfirb_mean, cfirb_sigma, 0, m, sg, "setCFIRBPrior1", "CFIRB");
   }
-  void setCFIRBPrior2(double, double) {
-    throw affineExcept("calcLikeDouble", "setCFIRBPrior2", "not supported by the device path");
+  void setCFIRBPrior2(double m, double sg) {
+    set_prior(has_cfirb, cfirb_mean, cfirb_sigma, 1, m, sg, "setCFIRBPrior2", "CFIRB");
   }
-  void setPoissonPrior1(double, double) {
-    throw affineExcept("calcLikeDouble", "setPoissonPrior1", "not supported by the device path");
+  void unsetCFIRBPrior1() noexcept { has_cfirb[0] = false; }
+  void unsetCFIRBPrior2() noexcept { has_cfirb[1] = false; }
+  void setPoissonPrior1(double m, double sg) {
+    set_prior(has_poisson, poisson_mean, poisson_sigma, 0, m, sg, "setPoissonPrior1", "Poisson prior");
   }
-  void setPoissonPrior2(double, double) {
-    throw affineExcept("calcLikeDouble", "setPoissonPrior2", "not supported by the device path");
+  void setPoissonPrior2(double m, double sg) {
+    set_prior(has_poisson, poisson_mean, poisson_sigma, 1, m, sg, "setPoissonPrior2", "Poisson prior");
   }
+  void unsetPoissonPrior1() noexcept { has_poisson[0] = false; }
+  void unsetPoissonPrior2() noexcept { has_poisson[1] = false; }
   void setRegularizationAlpha(double alpha) {
     if (alpha < 0.0)
       throw affineExcept("calcLikeDouble", "setRegularizationAlpha", "Invalid (negative) regularization alpha");
@@ -1277,16 +1363,47 @@ class calcLikeDouble {
     double out = 0.0;
     int st = 0;
     pars_invalid = false;  /* never set by the reference either */
-    getLogLikeBatch(p.data(), 1, p.getNParams(), &out, &st);
+    double bonus[5];
+    getLogLikeBatch(p.data(), 1, p.getNParams(), &out, &st, bonus);
     if (st && st != POFD_E_BADLIKE)
       pofd_amd_detail::throw_status2(ctx.get(), st, "calcLikeDouble", "getLogLike");
+    meanParams = p;
+    std::copy(bonus, bonus + 5, bonus_);
     return out;
+  }
+
+  /* calcLikeDouble::fillBonusParams, src/calcLikeDouble.cc:1283-1316: mean
+     flux per area of both bands, then flux^2 per area, into
+     par[nmodel+2 .. nmodel+5]; recomputed (one device evaluation) when par
+     is not the last evaluated set */
+  void fillBonusParams(paramSet& par) const {
+    const unsigned nmp = model.getNParams();
+    if (par.getNParams() < nmp + 6)
+      throw affineExcept("calcLikeDouble", "fillBonusParams", "Not enough room to fill");
+    float lensum = 0.0f, diffsum = 0.0f;
+    const bool have = meanParams.getNParams() >= nmp;
+    for (unsigned i = 0; have && i < nmp; ++i) {
+      const float v = meanParams[i], d = v - par[i];
+      lensum += v * v;
+      diffsum += d * d;
+    }
+    const float dist = !have ? 1.0f : (lensum == 0.0f ? std::sqrt(diffsum) : std::sqrt(diffsum / lensum));
+    if (dist > 1e-4f) {
+      double ll = 0.0;
+      int st = 0;
+      getLogLikeBatch(par.data(), 1, par.getNParams(), &ll, &st, bonus_);
+      meanParams = par;
+    }
+    par[nmp + 2] = static_cast<float>(bonus_[1]);
+    par[nmp + 3] = static_cast<float>(bonus_[2]);
+    par[nmp + 4] = static_cast<float>(bonus_[3]);
+    par[nmp + 5] = static_cast<float>(bonus_[4]);
   }
 
   /* many walkers per launch: theta rows [knots..., sigmas..., offsets...,
      sigmult1, sigmult2, ...] */
   void getLogLikeBatch(const float* theta, unsigned int nwalk, unsigned int stride,
-                       double* loglike, int* status) const {
+                       double* loglike, int* status, double* bonus = nullptr) const {
     const double half_log_2pi = 0.918938517570495605469;
     if (sets.empty()) {
       for (unsigned w = 0; w < nwalk; ++w) loglike[w] = NAN;
@@ -1298,8 +1415,17 @@ class calcLikeDouble {
     int rc = pofd2d_loglike_batch(ctx.get(), theta, nwalk, stride, loglike,
                                   reinterpret_cast<int32_t*>(status));
     if (rc) pofd_amd_detail::throw_status2(ctx.get(), rc, "calcLikeDouble", "getLogLike");
+    std::vector<double> bo;
+    const bool need_bonus = bonus || has_cfirb[0] || has_cfirb[1] || has_poisson[0] ||
+                            has_poisson[1];
+    if (need_bonus) {
+      bo.resize(5 * static_cast<size_t>(nwalk));
+      rc = pofd2d_get_bonus(ctx.get(), nwalk, bo.data());
+      if (rc) pofd_amd_detail::throw_status2(ctx.get(), rc, "calcLikeDouble", "getLogLike");
+      if (bonus) std::copy(bo.begin(), bo.end(), bonus);
+    }
     for (unsigned w = 0; w < nwalk; ++w) {
-      if (status[w] != 0) continue;
+      if (status[w] != 0 && status[w] != POFD_E_BADLIKE) continue;
       const float* t = theta + static_cast<size_t>(w) * stride;
       if (has_sigma_prior1) {
         const double val = (static_cast<double>(t[nmp]) - 1.0) / sigma_prior_width1;
@@ -1308,6 +1434,18 @@ class calcLikeDouble {
       if (has_sigma_prior2) {
         const double val = (static_cast<double>(t[nmp + 1]) - 1.0) / sigma_prior_width2;
         loglike[w] -= half_log_2pi + log(sigma_prior_width2) + 0.5 * val * val;
+      }
+      for (int band = 0; band < 2; ++band) {  // src/calcLikeDouble.cc:1244-1270
+        if (has_cfirb[band]) {
+          const double val = (cfirb_mean[band] - bo[5 * w + 1 + band]) / cfirb_sigma[band];
+          loglike[w] -= half_log_2pi + log(cfirb_sigma[band]) + 0.5 * val * val;
+        }
+      }
+      for (int band = 0; band < 2; ++band) {
+        if (has_poisson[band]) {
+          const double val = (poisson_mean[band] - bo[5 * w + 3 + band]) / poisson_sigma[band];
+          loglike[w] -= half_log_2pi + log(poisson_sigma[band]) + 0.5 * val * val;
+        }
       }
       if (regularization_alpha > 0.0) {
         numberCountsDoubleLogNormal m = model;

@@ -118,6 +118,10 @@ double orc_knots_getR1(const orc_knots *m, double f, const orc_beam *bm);
 void orc_knots_getR(const orc_knots *m, unsigned n, const double *flux,
                     const orc_beam *bm, double *R);
 double orc_knots_regularize(const orc_knots *m, double alpha);
+/* splineInt (src/numberCountsKnotsSpline.cc:300-343): getNS (alpha 0),
+   getFluxPerArea (1), getFluxSqPerArea (2) */
+double orc_knots_splineint(const orc_knots *m, double alpha);
+void orc_gl_nodes(unsigned n, double *x, double *w);
 
 /* ---------------- data (src/fitsData.cc) ---------------- */
 void orc_data_minmax(const double *d, size_t n, double *mn, double *mx);

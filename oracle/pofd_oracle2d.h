@@ -89,6 +89,10 @@ double orc_ln_sigma(const orc_lnmodel *m, double f1);
 double orc_ln_offset(const orc_lnmodel *m, double f1);
 double orc_ln_counts(const orc_lnmodel *m, double f1, double f2);
 double orc_ln_band1_counts(const orc_lnmodel *m, double f1);
+/* splineInt(alpha, beta) (src/numberCountsDoubleLogNormal.cc:1549-1673):
+   getNS (0,0), getFluxPerArea band 1 (1,0) / band 2 (0,1), getFluxSqPerArea
+   band 1 (2,0) / band 2 (0,2) */
+double orc_ln_splineint(const orc_lnmodel *m, double alpha, double beta);
 /* getMaxFlux: out[2] = band 1, band 2 */
 int orc_ln_maxflux(const orc_lnmodel *m, double *out, orc_err *err);
 /* getRRange: out[4] = min1, max1, min2, max2 */

@@ -69,6 +69,8 @@ def lib():
         "orc_knots_getR1": (D, [C.c_void_p, D, C.c_void_p]),
         "orc_knots_getR": (None, [C.c_void_p, C.c_uint, DP, C.c_void_p, DP]),
         "orc_knots_regularize": (D, [C.c_void_p, D]),
+        "orc_knots_splineint": (D, [C.c_void_p, D]),
+        "orc_gl_nodes": (None, [C.c_uint, DP, DP]),
         "orc_data_bin": (C.c_int, [DP, C.c_size_t, C.c_uint32, U32P, DP, DP]),
         "orc_data_mean_subtract": (D, [DP, C.c_size_t]),
         "orc_rfft": (C.c_int, [C.c_uint, DP, DP]),
@@ -241,6 +243,11 @@ class Knots:
 
     def regularize(self, alpha):
         return lib().orc_knots_regularize(C.byref(self.s), float(alpha))
+
+    def splineint(self, alpha):
+        """int S^alpha n(S) dS over the knot range (getNS / getFluxPerArea /
+        getFluxSqPerArea for alpha = 0, 1, 2)."""
+        return lib().orc_knots_splineint(C.byref(self.s), float(alpha))
 
 
 def data_bin(d, nbins):

@@ -45,6 +45,7 @@ def lib():
         "orc_ln_offset": (D, [VP, D]),
         "orc_ln_counts": (D, [VP, D, D]),
         "orc_ln_band1_counts": (D, [VP, D]),
+        "orc_ln_splineint": (D, [VP, D, D]),
         "orc_ln_maxflux": (C.c_int, [VP, DP, EP]),
         "orc_ln_rrange": (C.c_int, [VP, VP, DP, EP]),
         "orc_ln_getR1": (D, [VP, D, D, VP]),
@@ -210,6 +211,12 @@ class LogNormalModel:
 
     def band1_counts(self, f1):
         return lib().orc_ln_band1_counts(C.byref(self.m), f1)
+
+    def splineint(self, alpha, beta):
+        """int S1^(a+b) exp(b mu + b^2 sigma^2 / 2) n1(S1) dS1 (getNS,
+        getFluxPerArea, getFluxSqPerArea of either band)."""
+        return lib().orc_ln_splineint(C.byref(self.m), float(alpha),
+                                      float(beta))
 
     def maxflux(self):
         out = np.zeros(2)

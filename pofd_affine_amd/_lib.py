@@ -67,6 +67,8 @@ _SIGS = {
                                       C.c_uint32, U32P]),
     "pofd1d_get_rinterp": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
     "pofd1d_get_rinterp_neg": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
+    "pofd1d_get_bonus": (C.c_int, [VP, C.c_uint32, DP]),
+    "pofd2d_get_bonus": (C.c_int, [VP, C.c_uint32, DP]),
     "pofd1d_stage_times": (C.c_int, [VP, DP, U32P]),
     "pofd_dbeam_create": (C.c_int, [C.POINTER(VP), DP, DP, C.c_uint64, D, D]),
     "pofd_dbeam_make_histogram": (C.c_int, [VP, C.c_uint32]),
@@ -348,6 +350,14 @@ class Context:
                                              _dp(y)), "pofd1d_get_rinterp")
         return x, y
 
+    def get_bonus(self, nwalk):
+        """[nwalk, 3]: getNS, getFluxPerArea, getFluxSqPerArea per walker of
+        the last batch."""
+        out = np.zeros(3 * nwalk)
+        self._check(lib().pofd1d_get_bonus(self.ptr, nwalk, _dp(out)),
+                    "pofd1d_get_bonus")
+        return out.reshape(nwalk, 3)
+
     def get_rinterp_neg(self, walker, b=0):
         """Negative-beam interpolation nodes (ascending, < 0) and log2 R."""
         x = np.zeros(self.ninterp)
@@ -506,6 +516,14 @@ class Context2(Context):
         self._check(lib().pofd2d_loglike_batch_device(
             self.ptr, d_theta, nwalk, stride, d_ll, d_st, stream),
             "pofd2d_loglike_batch_device")
+
+    def get_bonus2d(self, nwalk):
+        """[nwalk, 5]: getNS, flux per area (band 1, 2), flux^2 per area
+        (band 1, 2) per walker of the last batch."""
+        out = np.zeros(5 * nwalk)
+        self._check(lib().pofd2d_get_bonus(self.ptr, nwalk, _dp(out)),
+                    "pofd2d_get_bonus")
+        return out.reshape(nwalk, 5)
 
     def get_debug2d(self, walker, b=0, f=0):
         out = np.zeros(32)

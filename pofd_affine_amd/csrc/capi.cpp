@@ -179,6 +179,33 @@ size_t bsearch_idx(const double* x, size_t n, double v) {
   return lo;
 }
 
+}  // namespace
+
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton on the Legendre
+// recurrence in long double)
+void pofd::gauss_legendre(int n, double* x, double* w) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int i = 0; i < n; ++i) {
+    long double z = cosl(pi * (i + 0.75L) / (n + 0.5L)), dp = 0.0L;
+    for (int it = 0; it < 100; ++it) {
+      long double p0 = 1.0L, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        const long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0L);
+      const long double dz = p1 / dp;
+      z -= dz;
+      if (fabsl(dz) < 1e-19L) break;
+    }
+    x[i] = static_cast<double>(z);
+    w[i] = static_cast<double>(2.0L / ((1.0L - z * z) * dp * dp));
+  }
+}
+
+namespace {
+
 std::vector<double2> twiddles(size_t n, size_t count) {
   std::vector<double2> w(count);
   for (size_t s = 0; s < count; ++s) {
@@ -420,6 +447,7 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   d.iflux = (2.0 * kPi) / (N * dflux);
   d.spectral_cut = c->spectral_cut;
   for (unsigned i = 0; i < nk; ++i) d.logknots[i] = c->logknots[i];
+  pofd::gauss_legendre(pofd::kGL, d.glx, d.glw);
   s.tables.clear();
   int rc;
   if ((rc = upload(c, s, xnode, &d.xnode))) return rc;
@@ -493,8 +521,9 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   const size_t n_dbg = r8(pofd::kDbg) * nf * nwalk;
   const size_t n_state = r8(pofd::kState) * nwalk;
   const size_t n_upart = r8(pofd::kPixBlocks) * nf * nwalk;
+  const size_t n_bonus = r8(4 * static_cast<size_t>(nwalk));
   const size_t total = n_coef + n_ynode + n_z + n_spec + n_pofd + n_blk +
-                       n_dbg + n_state + n_upart;
+                       n_dbg + n_state + n_upart + n_bonus;
   s.wbuf.reset();
   int rc = dev_alloc(c, s.wbuf, total * sizeof(double));
   if (rc) return rc;
@@ -524,6 +553,8 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   w.state = base;
   base += n_state;
   w.upart = base;
+  base += n_upart;
+  w.bonus = base;
   s.wcap = nwalk;
   return POFD_OK;
 }
@@ -735,6 +766,7 @@ extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
     if ((rc = ensure_workspace(c, s, nwalk))) return rc;
     hipEvent_t* ev = nullptr;
     if ((rc = pofd2::ctx_stage_events(c, &ev))) return rc;
+    s.dev.bonus = (b == 0) ? 1 : 0;  // the counts integrals once per batch
     rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta,
                                 nwalk, stride, d_ll, d_st, b > 0 ? 1 : 0,
                                 stream, ev);
@@ -879,6 +911,21 @@ extern "C" int pofd1d_get_rinterp(pofd_ctx* c, uint32_t walker, int b,
 extern "C" int pofd1d_get_rinterp_neg(pofd_ctx* c, uint32_t walker, int b,
                                       double* flux_out, double* log2r_out) {
   return get_rinterp_seg(c, walker, b, 1, flux_out, log2r_out, "pofd1d_get_rinterp_neg");
+}
+
+extern "C" int pofd1d_get_bonus(pofd_ctx* c, uint32_t nwalk, double* out) {
+  if (!c || !out) return POFD_E_ARG;
+  if (c->sets.empty() || !c->sets[0]->wbuf || nwalk > c->last_nwalk)
+    return fail(c, POFD_E_ARG, "pofd1d_get_bonus: no such walkers in the last batch");
+  const Beamset& s = *c->sets[0];
+  std::vector<double> tmp(static_cast<size_t>(nwalk) * 4);
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(tmp.data(), s.wk.bonus, tmp.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(bonus)");
+  for (uint32_t w = 0; w < nwalk; ++w)
+    for (int k = 0; k < 3; ++k) out[3 * w + k] = tmp[4 * static_cast<size_t>(w) + k];
+  return POFD_OK;
 }
 
 extern "C" int pofd1d_stage_times(pofd_ctx* c, double* ms_out,

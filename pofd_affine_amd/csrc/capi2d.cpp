@@ -651,6 +651,25 @@ int prepare_set2(pofd_ctx* c, Ctx2& C2, Set2& s) {
   d.iRynorm = dife2 / (dflux2 - minedge2);
   d.iR00norm = dife1 * dife2 / ((dflux1 - minedge1) * (dflux2 - minedge2));
   d.spectral_cut = pofd2::ctx_spectral_cut(c);
+  // breakpoints of the counts integrals: log2 of the band-1 knots and of the
+  // sigma and offset knots inside the band-1 range
+  {
+    const LNHost& lm = C2.model;
+    std::vector<double> bp(lm.logknots, lm.logknots + lm.nk);
+    const double lo = bp.front(), hi = bp.back();
+    for (int i = 0; i < lm.ns; ++i) {
+      const double v = log2(lm.sk[i]);
+      if (v > lo && v < hi) bp.push_back(v);
+    }
+    for (int i = 0; i < lm.no; ++i) {
+      const double v = log2(lm.ok[i]);
+      if (v > lo && v < hi) bp.push_back(v);
+    }
+    std::sort(bp.begin(), bp.end());
+    d.nbp = static_cast<int>(bp.size());
+    for (size_t i = 0; i < bp.size(); ++i) d.bpts[i] = bp[i];
+    pofd::gauss_legendre(32, d.glx, d.glw);
+  }
   int rc;
   if ((rc = upload(c, s, rf1, &d.rflux1))) return rc;
   if ((rc = upload(c, s, rf2, &d.rflux2))) return rc;
@@ -788,6 +807,7 @@ int ensure_work2(pofd_ctx* c, Ctx2& C2, Set2& s, uint32_t nwalk) {
   const size_t o_rsp = tot; tot += r8(nparts * 4) * W;
   const size_t o_lp = tot; tot += r8(pofd2::kLikeBlocks) * nf * W;
   const size_t o_dbg = tot; tot += r8(pofd2::kDbg2) * nf * W;
+  const size_t o_bo = tot; tot += 8 * W;
   s.wbuf.reset();
   int rc = alloc(c, s.wbuf, tot * sizeof(double));
   if (rc) return rc;
@@ -815,6 +835,7 @@ int ensure_work2(pofd_ctx* c, Ctx2& C2, Set2& s, uint32_t nwalk) {
   w.rstat_part = b + o_rsp;
   w.lpart = b + o_lp;
   w.dbg = b + o_dbg;
+  w.bonus = b + o_bo;
   s.gmain.out = w.R;
   s.gmain.wstride = w.R_stride;
   s.g00.out = w.e00;
@@ -1215,6 +1236,7 @@ int pofd2d_loglike_batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwal
     s.dev.spectral_cut = pofd2::ctx_spectral_cut(c);
     hipEvent_t* ev = nullptr;
     if ((rc = pofd2::ctx_stage_events(c, &ev))) return rc;
+    s.dev.bonus = (b == 0) ? 1 : 0;  // the counts integrals once per batch
     rc = pofd2::launch_loglike2d(s.dev, s.wk, s.gmain, s.g00, s.gx, s.gy, d_theta, stride, d_ll,
                                  d_st, b > 0 ? 1 : 0, stream, ev);
     if (rc)
@@ -1251,6 +1273,21 @@ int pofd2d_loglike_batch(pofd_ctx* c, const float* theta, uint32_t nwalk, uint32
 // info[32]: the per-(walker, dataset) record: status, split1, split2,
 // sg1pos, sg2pos, sigma1, sigma2, integral, 1/integral, minflux1, minflux2,
 // dataset logL (before offset), then mn1, mn2, var1, var2, r0, K1, K2
+int pofd2d_get_bonus(pofd_ctx* c, uint32_t nwalk, double* out) {
+  Ctx2* C2 = get2(c);
+  if (!C2 || !out || C2->sets.empty() || !C2->sets[0]->wbuf || nwalk > C2->last_nwalk)
+    return ctx_fail(c, POFD_E_ARG, "pofd2d_get_bonus: no such walkers in the last batch");
+  const Set2& s = *C2->sets[0];
+  std::vector<double> tmp(static_cast<size_t>(nwalk) * 8);
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(tmp.data(), s.wk.bonus, tmp.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hipfail(c, e, "hipMemcpy(bonus)");
+  for (uint32_t w = 0; w < nwalk; ++w)
+    for (int k = 0; k < 5; ++k) out[5 * w + k] = tmp[8 * static_cast<size_t>(w) + k];
+  return POFD_OK;
+}
+
 int pofd2d_get_debug(pofd_ctx* c, uint32_t walker, int b, int f, double* info) {
   Ctx2* C2 = get2(c);
   if (!C2 || !info || b < 0 || static_cast<size_t>(b) >= C2->sets.size())

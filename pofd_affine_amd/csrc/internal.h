@@ -26,6 +26,8 @@ struct Beam {
 };
 
 void data_minmax(const double* d, size_t n, double& mn, double& mx);
+// Gauss-Legendre nodes and weights on [-1, 1]
+void gauss_legendre(int n, double* x, double* w);
 
 // Limits of the 1D kernels.
 constexpr int kThreads = 1024;   // threads per workgroup (one walker)
@@ -38,6 +40,7 @@ constexpr int kWarm = 64;        // warm-up steps of the truncated recurrences
 constexpr int kDbg = 32;         // debug scalars per (walker, field)
 constexpr int kState = 16;       // per-walker scalars handed between kernels
 constexpr int kPixBlocks = 64;   // workgroups per (walker, field), unbinned
+constexpr int kGL = 32;          // Gauss-Legendre points per knot interval
 
 // One field (dataset) of a beam set, device view.
 struct FieldDev {
@@ -77,6 +80,8 @@ struct BeamsetDev {
   double iflux;           // two_pi / (N * dflux)
   double spectral_cut;    // drop components with sigma^2 w^2/2 > cut
   double logknots[kMaxKnots];
+  int bonus;              // this launch also integrates the counts
+  double glx[kGL], glw[kGL];  // Gauss-Legendre nodes and weights on [-1, 1]
   // node tables, per segment (positive beam, then negative beam):
   const double* xnode;    // nseg*ninterp interpolation fluxes
   const double* ginv;     // nseg*(ninterp-1): 1/h_i  (GSL cspline_init g_i)
@@ -107,6 +112,7 @@ struct WorkDev {
   double* dbg;       // W * nfields * kDbg : debug scalars
   double* state;     // W * kState         : per-walker scalars
   double* upart;     // W * nfields * kPixBlocks : unbinned partial sums
+  double* bonus;     // W * 4: int n dS, int S n dS, int S^2 n dS
   size_t coef_stride, ynode_stride, zrows_stride, spec_stride, pofd_stride,
       blk_stride;
 };

@@ -10,6 +10,11 @@
 
 #include "../../include/pofd_affine_amd.h"
 
+namespace pofd {
+// Gauss-Legendre nodes and weights on [-1, 1] (capi.cpp)
+void gauss_legendre(int n, double* x, double* w);
+}  // namespace pofd
+
 namespace pofd2 {
 
 constexpr int kMaxK = 32;          // knots, sigma knots, offset knots (each)
@@ -89,6 +94,12 @@ struct Set2Dev {
   double spectral_cut;
   int nfields;
   Field2Dev fields[kMaxFields2];
+  // counts integrals: breakpoints (log2 of the band-1, sigma and offset
+  // knots inside the band-1 range, sorted) and Gauss-Legendre rule
+  int bonus;                      // this launch also integrates the counts
+  int nbp;
+  double bpts[3 * kMaxK];
+  double glx[32], glw[32];
 };
 
 struct Work2 {
@@ -116,6 +127,7 @@ struct Work2 {
   double* rstat_part;             // W * nparts * 4
   double* lpart;                  // W * nf * kLikeBlocks
   double* dbg;                    // W * nf * kDbg2
+  double* bonus;                  // W * 8: NS, S1, S2, S1^2, S2^2 per area
   size_t R_stride, ex_stride, edge_stride, Y_stride, YT_stride, PR_stride,
       Q_stride, QT_stride, P_stride, colpart_stride;
   int ystride;                    // n/2 + 1 (complex)

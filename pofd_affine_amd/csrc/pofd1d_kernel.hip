@@ -252,6 +252,7 @@ struct PrepSmem {
   double gi[2048];   // 1/h
   double kn[4][kMaxKnots];
   double ks[7][kMaxKnots];
+  double red[kWaves];
   int isc[4];
 };
 
@@ -314,7 +315,8 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
     st[ST_STATUS] = status;
     st[ST_SIGMULT] = sigmult;
     sm.isc[0] = status;
-    if (status == POFD_OK) {
+    sm.isc[1] = valid ? 1 : 0;
+    if (valid) {
       // GSL natural cubic spline through (logknots, logknotvals):
       // cspline_init + gsl_linalg_solve_symm_tridiag + coeff_calc
       const double* xa = bs.logknots;
@@ -365,6 +367,42 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
     }
   }
   __syncthreads();
+  if (bs.bonus) {
+    // getNS / getFluxPerArea / getFluxSqPerArea (numberCountsKnotsSpline::
+    // splineInt, src/numberCountsKnotsSpline.cc:300-343): int S^a n(S) dS
+    // over the knot range, a = 0, 1, 2, as Gauss-Legendre sums in
+    // v = log2 S over each knot interval (the integrand 2^(spline + (a+1) v)
+    // ln 2 is analytic there); NaN for an invalid model
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    if (sm.isc[1]) {
+      const int npts = (nk - 1) * kGL;
+      for (int t = tid; t < npts; t += kThreads) {
+        const int k = t / kGL, g = t - k * kGL;
+        const double v0 = bs.logknots[k], v1 = bs.logknots[k + 1];
+        const double h = 0.5 * (v1 - v0), c = 0.5 * (v1 + v0);
+        const double v = c + h * bs.glx[g];
+        const double dx = v - v0;
+        const double sv =
+            sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
+        const double sflux = exp2(v);
+        const double f = bs.glw[g] * h * exp2(sv + v);
+        b0 += f;
+        b1 += f * sflux;
+        b2 += f * sflux * sflux;
+      }
+    }
+    b0 = block_sum(b0, sm.red);
+    b1 = block_sum(b1, sm.red);
+    b2 = block_sum(b2, sm.red);
+    if (tid == 0) {
+      const double ln2 = 0.69314718055994530942;
+      const double nan = __longlong_as_double(0x7ff8000000000000ULL);
+      double* bo = wk.bonus + static_cast<size_t>(walker) * 4;
+      bo[0] = sm.isc[1] ? ln2 * b0 : nan;
+      bo[1] = sm.isc[1] ? ln2 * b1 : nan;
+      bo[2] = sm.isc[1] ? ln2 * b2 : nan;
+    }
+  }
   if (sm.isc[0] != POFD_OK) return;  // uniform across the workgroup
 
   // one pass per node segment: the positive beam, then the negative beam

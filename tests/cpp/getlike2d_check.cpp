@@ -65,6 +65,26 @@ int main(int argc, char** argv) {
       double ll = likeSet.getLogLike(p, rej);
       std::printf("loglike %.17g %d\n", ll, rej ? 1 : 0);
     }
+    // CFIRB and Poisson priors of both bands on the first set, and its
+    // bonus parameters
+    likeSet.setCFIRBPrior1(150.0, 20.0);
+    likeSet.setCFIRBPrior2(180.0, 25.0);
+    likeSet.setPoissonPrior1(2.0, 0.5);
+    likeSet.setPoissonPrior2(3.0, 0.7);
+    {
+      bool rej = false;
+      double ll = likeSet.getLogLike(p0, rej);
+      const unsigned nmp = pos[0].size() + pos[1].size() + pos[2].size();
+      paramSet pb(nmp + 6);
+      for (unsigned i = 0; i < nmp + 2 && i < p0.getNParams(); ++i) pb[i] = p0[i];
+      likeSet.fillBonusParams(pb);
+      std::printf("prior %.17g %.9g %.9g %.9g %.9g\n", ll, pb[nmp + 2], pb[nmp + 3],
+                  pb[nmp + 4], pb[nmp + 5]);
+    }
+    likeSet.unsetCFIRBPrior1();
+    likeSet.unsetCFIRBPrior2();
+    likeSet.unsetPoissonPrior1();
+    likeSet.unsetPoissonPrior2();
     // PDFactoryDouble::getPD on the same beams and model, over the range
     // the likelihood used
     doublebeam bm(argv[11], argv[12], nbeamhist > 0, nbeamhist, minbeamval);

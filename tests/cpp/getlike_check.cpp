@@ -60,6 +60,20 @@ int main(int argc, char** argv) {
       double ll = likeSet.getLogLike(p, rej);
       std::printf("loglike %.17g %d\n", ll, rej ? 1 : 0);
     }
+    // CFIRB and Poisson priors on the first set, and its bonus parameters
+    likeSet.setCFIRBPrior(7000.0, 300.0);
+    likeSet.setPoissonPrior(20.0, 2.0);
+    {
+      bool rej = false;
+      double ll = likeSet.getLogLike(p0, rej);
+      paramSet pb(p0.getNParams() < knots.size() + 3 ? knots.size() + 3 : p0.getNParams());
+      for (unsigned i = 0; i < p0.getNParams(); ++i) pb[i] = p0[i];
+      likeSet.fillBonusParams(pb);
+      std::printf("prior %.17g %.9g %.9g\n", ll, pb[knots.size() + 1],
+                  pb[knots.size() + 2]);
+    }
+    likeSet.unsetCFIRBPrior();
+    likeSet.unsetPoissonPrior();
     // PDFactory::getPD on the same beam and model
     beam bm(argv[9], nbeamhist > 0, nbeamhist, minbeamval);
     numberCountsKnotsSpline model(knots);

@@ -1,6 +1,7 @@
 """The reference-shaped C++ API (include/pofd_affine_amd.hpp) end to end:
 FITS beam and map files -> calcLike::readDataFromFiles -> setRRanges ->
 getLogLike, as pofd_affine_getLike drives it, checked against the oracle."""
+import math
 import os
 import subprocess
 
@@ -46,6 +47,17 @@ def test_cpp_getlike(tmp_path):
         ref, rst = L.loglike_batch(np.hstack([th, np.zeros((5, 2),
                                                            np.float32)]), 4)
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+        # priors (src/calcLike.cc:932-947) from the oracle's integrals
+        import pyoracle as O
+        m = O.Knots(knots, th[0][:len(knots)])
+        s1, s2 = m.splineint(1), m.splineint(2)
+        h = 0.918938517570495605469
+        pri = -(h + math.log(300.0) + 0.5 * ((7000.0 - s1) / 300.0) ** 2) \
+            - (h + math.log(2.0) + 0.5 * ((20.0 - s2) / 2.0) ** 2)
+        pl = [l for l in lines if l.startswith("prior")][0].split()
+        assert float(pl[1]) == pytest.approx(ref[0] + pri, abs=1e-6)
+        assert float(pl[2]) == pytest.approx(s1, rel=1e-6)
+        assert float(pl[3]) == pytest.approx(s2, rel=1e-6)
         err = [l for l in lines if l.startswith("error")][0]
         assert err == "error PDFactory::initPD"
         pd = [l for l in lines if l.startswith("pd")][0].split()
@@ -97,6 +109,22 @@ def test_cpp_getlike2d(tmp_path):
         ref, rst = L.loglike_batch(thp, 4)
         assert np.all(rst == 0)
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+        # priors of both bands (src/calcLikeDouble.cc:1244-1270) from the
+        # oracle's integrals
+        import pyoracle2d as O2
+        m = O2.LogNormalModel(kp, sp, op)
+        m.set_params(th[0][:len(kp) + len(sp) + len(op)])
+        f1, f2 = m.splineint(1, 0), m.splineint(0, 1)
+        q1, q2 = m.splineint(2, 0), m.splineint(0, 2)
+        h = 0.918938517570495605469
+        pri = 0.0
+        for mu, sg, v in ((150.0, 20.0, f1), (180.0, 25.0, f2),
+                          (2.0, 0.5, q1), (3.0, 0.7, q2)):
+            pri -= h + math.log(sg) + 0.5 * ((mu - v) / sg) ** 2
+        pl = [l for l in lines if l.startswith("prior")][0].split()
+        assert float(pl[1]) == pytest.approx(ref[0] + pri, abs=1e-6)
+        np.testing.assert_allclose([float(x) for x in pl[2:6]],
+                                   [f1, f2, q1, q2], rtol=1e-6)
         err = [l for l in lines if l.startswith("error")][0]
         assert err == "error PDFactoryDouble::initPD"
         pd = [l for l in lines if l.startswith("pd")][0].split()

@@ -98,6 +98,27 @@ def test_small_pd(small, w):
                                    atol=1e-10 * ref["pd"].max())
 
 
+def test_bonus_integrals(small):
+    """getNS / getFluxPerArea / getFluxSqPerArea per walker (splineInt):
+    the device's Gauss-Legendre sums against the oracle's, which are pinned
+    by the reference's golden values and adaptive quadrature."""
+    import pyoracle as O
+    inp, th, out = small
+    ctx = out[True][0]
+    _say("bonus integrals")
+    bo = ctx.get_bonus(len(th))
+    for w in range(len(th)):
+        m = O.Knots(inp["knots"], th[w][:len(inp["knots"])])
+        ref = [m.splineint(a) for a in (0, 1, 2)]
+        np.testing.assert_allclose(bo[w], ref, rtol=1e-12, atol=0)
+    bad = th[:2].copy()
+    bad[0, 2] = np.nan
+    bad[1, 8] = -1.0  # bad_like: the model is valid, the integrals exist
+    ctx.loglike_batch(bad)
+    bo = ctx.get_bonus(2)
+    assert np.all(np.isnan(bo[0])) and np.all(np.isfinite(bo[1]))
+
+
 def test_status_codes(small):
     inp, th, out = small
     ctx = out[True][0]

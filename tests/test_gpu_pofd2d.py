@@ -77,6 +77,24 @@ def test_small2d_moments_and_pd(small2d, w):
     np.testing.assert_allclose(pd, pd_ref, rtol=0, atol=1e-10 * pd_ref.max())
 
 
+def test_small2d_bonus_integrals(small2d):
+    """getNS, getFluxPerArea and getFluxSqPerArea of both bands per walker
+    (numberCountsDoubleLogNormal::splineInt) against the oracle."""
+    import pyoracle2d as O2
+    inp, th, out = small2d
+    ctx = out[False][0]
+    _say("small 2D bonus integrals")
+    bo = ctx.get_bonus2d(len(th))
+    kp, sp, op = synth2d.positions()
+    nmp = len(kp) + len(sp) + len(op)
+    for w in range(len(th)):
+        m = O2.LogNormalModel(kp, sp, op)
+        m.set_params(th[w][:nmp])
+        ref = [m.splineint(a, b) for a, b in ((0, 0), (1, 0), (0, 1),
+                                                (2, 0), (0, 2))]
+        np.testing.assert_allclose(bo[w], ref, rtol=1e-12, atol=0)
+
+
 def test_small2d_status_codes(small2d):
     inp, th, out = small2d
     ctx = out[False][0]

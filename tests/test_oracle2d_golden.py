@@ -123,6 +123,15 @@ def test_model2d_number_counts():
         assert abs(m.counts(a, b) - e) / e < 1e-3
 
 
+def test_model2d_power_integration():
+    # model2DTest.PowerIntegration (:681-703)
+    m = _model([0.002, 0.020], [0.03],
+               [8.0, 6.0, 4.0, 3.3, 2.0, 0.1, 0.2, -0.03])
+    assert abs(m.splineint(0, 0) - 57266.8) < 10.0
+    assert abs(m.splineint(1, 0) - 152.262) < 0.1
+    assert abs(m.splineint(0, 1) - 148.585) < 0.1
+
+
 def test_model2d_band1_projection_matches_1d_spline():
     # model2DTest.Projection band 1 (:706-744): the band-1 counts are the
     # 1D knot spline's

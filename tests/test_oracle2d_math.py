@@ -184,3 +184,41 @@ def test_split_points_2d_properties(small2d):
     assert abs(np.sum(x2 * p2) / p2.sum()) < 1e-3 * info["dflux2"]
     assert info["split1"] * info["dflux1"] >= 2.25 * info["sgpos1"]
     assert info["split2"] * info["dflux2"] >= 2.25 * info["sgpos2"]
+
+
+def test_splineint2d_matches_adaptive_quadrature():
+    """splineInt(alpha, beta) (src/numberCountsDoubleLogNormal.cc:
+    1549-1673): band-2 moments of the log-normal colour model are
+    int S1^(a+b) exp(b mu + b^2 sigma^2/2) n1 dS1; checked against scipy's
+    adaptive integration of the same integrand built from the oracle's
+    golden-pinned band-1 counts, sigma(S1) and offset(S1), and against a
+    direct 2D integral of the counts for band 2."""
+    from scipy import integrate
+    kp, sp, op = synth2d.positions()
+    m = O2.LogNormalModel(kp, sp, op)
+    th = synth2d.true_theta()
+    m.set_params(th[:len(kp) + len(sp) + len(op)])
+    brk = np.log(np.concatenate([kp[1:-1], sp, op]))
+    brk = brk[(brk > math.log(kp[0])) & (brk < math.log(kp[-1]))]
+    for a, b in ((0, 0), (1, 0), (0, 1), (2, 0), (0, 2)):
+        def f(u):
+            s1 = math.exp(u)
+            e = b * m.offset(s1) + 0.5 * b * b * m.sigma(s1) ** 2
+            return s1 ** (a + b + 1) * math.exp(e) * m.band1_counts(s1)
+        ref, _ = integrate.quad(f, math.log(kp[0]), math.log(kp[-1]),
+                                epsrel=1e-13, limit=500, points=brk)
+        assert m.splineint(a, b) == pytest.approx(ref, rel=1e-10), (a, b)
+    # band-2 flux per area directly: int int S2 n(S1, S2) dS2 dS1
+    # (the counts vanish outside [minknot, maxknot): keep the grid just
+    # inside both ends)
+    s1g = np.exp(np.linspace(math.log(kp[0]) + 1e-12,
+                             math.log(kp[-1]) - 1e-12, 4001))
+    inner = []
+    for s1 in s1g:
+        v, _ = integrate.quad(lambda w: math.exp(w) * math.exp(w) *
+                              m.counts(s1, math.exp(w)),
+                              math.log(s1) - 6, math.log(s1) + 6,
+                              epsrel=1e-10, limit=200)
+        inner.append(v * s1)
+    direct = np.trapezoid(inner, np.log(s1g))
+    assert m.splineint(0, 1) == pytest.approx(direct, rel=1e-5)

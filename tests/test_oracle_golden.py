@@ -95,6 +95,15 @@ def test_model_rrange(refdata):
     assert abs(hi - 0.0488993) < 1e-4
 
 
+def test_model_integration():
+    # model1DTest.Integration (model_test.cc:197-213)
+    kp = np.array(KP, dtype=np.float32).astype(np.float64)
+    m = O.Knots(kp, KV)
+    assert abs(m.splineint(0) - 3.12022e6) < 1e4
+    assert abs(m.splineint(1) - 7386.67) < 1.0
+    assert abs(m.splineint(2) - 18.132) < 0.01
+
+
 @pytest.mark.parametrize("nhist", [None, 120])
 def test_model_getR(refdata, nhist):
     # model_test.cc:242-316

@@ -292,3 +292,25 @@ def test_split_point_properties():
         assert max(p[0], p[-1]) < 1e-5 * p.max()
         x = pd["minflux"] + np.arange(p.size) * pd["dflux"]
         assert abs(np.sum(x * p) / np.sum(p)) < 1e-3 * pd["dflux"]
+
+
+def test_splineint_matches_adaptive_quadrature():
+    """getNS / getFluxPerArea / getFluxSqPerArea (splineInt,
+    src/numberCountsKnotsSpline.cc:300-343): the oracle's Gauss-Legendre
+    sums equal scipy's adaptive QUADPACK integration of S^alpha n(S), with
+    n(S) the golden-pinned number counts, to 1e-10 (the reference's own
+    QAG-41 stops at relative 1e-5)."""
+    from scipy import integrate
+    import ctypes
+    kp = synth.knot_positions()
+    rng = np.random.default_rng(8)
+    for t in range(4):
+        vals = synth.true_theta()[:len(kp)] + rng.normal(0, 0.3, len(kp))
+        m = O.Knots(kp, vals.astype(np.float32))
+        nc = lambda s: O.lib().orc_knots_number_counts(ctypes.byref(m.s), s)
+        for a in (0, 1, 2):
+            ref, _ = integrate.quad(lambda u: math.exp((a + 1) * u) *
+                                    nc(math.exp(u)), math.log(kp[0]),
+                                    math.log(kp[-1]), epsrel=1e-13,
+                                    limit=500, points=np.log(kp[1:-1]))
+            assert m.splineint(a) == pytest.approx(ref, rel=1e-10), (t, a)
