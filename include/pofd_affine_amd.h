@@ -318,6 +318,70 @@ int pofd2d_get_debug(pofd_ctx *ctx, uint32_t walker, int beamset_id,
 int pofd2d_get_pd(pofd_ctx *ctx, uint32_t walker, int beamset_id,
                   int field_id, int is_log, double *pd_out, double *info);
 
+/* ======================================================================
+   Device-resident ensemble sampler: the affine-invariant stretch move of
+   affineEnsemble (src/affineEnsemble.cc:116-126, 1144-1420) with the
+   walkers, proposals, acceptance and chain kept on the GPU; the likelihood
+   of each half-ensemble is one batched launch of the 1D or 2D path.
+   Differences from the reference: random numbers come from Philox4x32-10
+   (counter-based, reproducible per seed) instead of ran (src/ran.cc); a
+   proposal that stays outside the limits after 40 redraws, or whose
+   likelihood fails (split, invalid model, non-finite mean or value), is
+   rejected and counted per walker where the reference aborts the run.
+   ====================================================================== */
+typedef struct pofd_ens pofd_ens;
+enum {
+  POFD_TARGET_1D = 1,     /* pofd1d_loglike_batch_device on the context */
+  POFD_TARGET_2D = 2,     /* pofd2d_loglike_batch_device on the context */
+  POFD_TARGET_GAUSS = 3   /* diagonal Gaussian (validation target) */
+};
+/* nwalkers even; npar parameters per walker (<= 64, bonus ones included,
+   the theta stride of the likelihood); scalefac the stretch scale a > 1
+   (default 2 in the reference, src/pofd_affine_mcmc.cc:39) */
+int pofd_ens_create(pofd_ens **out, pofd_ctx *ctx, int target,
+                    uint32_t nwalkers, uint32_t npar, double scalefac,
+                    uint64_t seed);
+void pofd_ens_destroy(pofd_ens *ens);
+const char *pofd_ens_last_error(const pofd_ens *ens);
+/* state[npar]: 0 fit, 1 fixed, 2 bonus (mcmc_affine::FIXED / BONUS) */
+int pofd_ens_set_param_state(pofd_ens *ens, const uint8_t *state);
+/* The acceptance exponent of z: fit_dimension 0 (default) uses the
+   reference's nparams - nfixed - 1, which counts bonus parameters
+   (src/affineEnsemble.cc:1398); 1 uses the number of fit parameters less
+   one, which the stretch move's detailed balance requires when bonus
+   parameters are present.  *exponent (optional) receives the value. */
+int pofd_ens_set_exponent_mode(pofd_ens *ens, int fit_dimension,
+                               float *exponent);
+/* parameter limits (NaN or NULL: none), areParamsValid */
+int pofd_ens_set_limits(pofd_ens *ens, const float *lower, const float *upper);
+int pofd_ens_set_gauss_target(pofd_ens *ens, const double *mean,
+                              const double *sigma);
+/* initial positions theta[nwalkers * npar]; evaluates their likelihoods */
+int pofd_ens_set_positions(pofd_ens *ens, const float *theta);
+/* nsteps stretch-move steps (both halves each, doMasterStep) at the given
+   temperature, storing every thin-th step (0: none) */
+int pofd_ens_run(pofd_ens *ens, uint32_t nsteps, double temperature,
+                 uint32_t thin);
+int pofd_ens_get_state(pofd_ens *ens, float *theta, double *loglike,
+                       uint32_t *naccept, uint32_t *nerr);
+int pofd_ens_chain_length(const pofd_ens *ens, uint32_t *nstored);
+/* stored steps [first, first+count): theta[count*nwalkers*npar],
+   loglike[count*nwalkers] */
+int pofd_ens_get_chain(pofd_ens *ens, uint32_t first, uint32_t count,
+                       float *theta, double *loglike);
+int pofd_ens_clear_chain(pofd_ens *ens);
+/* autocorrelation time of parameter `param` over the stored steps, from
+   the walker-averaged series (affineChainSet::getAcor,
+   src/affineChainSet.cc:959-984) */
+int pofd_ens_acor(pofd_ens *ens, uint32_t param, double *tau, double *mean,
+                  double *sigma);
+/* n draws of the stretch factor z (affineEnsemble::generateZ) */
+int pofd_ens_sample_z(pofd_ens *ens, uint32_t n, double *out);
+/* Goodman's ACOR on a series (affineChainSet::acor,
+   src/affineChainSet.cc:880-950): POFD_E_ARG if n < 50 */
+int pofd_acor(const float *x, uint32_t n, double *mean, double *sigma,
+              double *tau);
+
 #ifdef __cplusplus
 }
 #endif

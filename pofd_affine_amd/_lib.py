@@ -69,6 +69,23 @@ _SIGS = {
     "pofd1d_get_rinterp_neg": (C.c_int, [VP, C.c_uint32, C.c_int, DP, DP]),
     "pofd1d_get_bonus": (C.c_int, [VP, C.c_uint32, DP]),
     "pofd2d_get_bonus": (C.c_int, [VP, C.c_uint32, DP]),
+    "pofd_ens_create": (C.c_int, [C.POINTER(VP), VP, C.c_int, C.c_uint32,
+                                  C.c_uint32, D, C.c_uint64]),
+    "pofd_ens_destroy": (None, [VP]),
+    "pofd_ens_last_error": (C.c_char_p, [VP]),
+    "pofd_ens_set_param_state": (C.c_int, [VP, C.POINTER(C.c_uint8)]),
+    "pofd_ens_set_limits": (C.c_int, [VP, FP, FP]),
+    "pofd_ens_set_exponent_mode": (C.c_int, [VP, C.c_int, FP]),
+    "pofd_ens_set_gauss_target": (C.c_int, [VP, DP, DP]),
+    "pofd_ens_set_positions": (C.c_int, [VP, FP]),
+    "pofd_ens_run": (C.c_int, [VP, C.c_uint32, D, C.c_uint32]),
+    "pofd_ens_get_state": (C.c_int, [VP, FP, DP, U32P, U32P]),
+    "pofd_ens_chain_length": (C.c_int, [VP, U32P]),
+    "pofd_ens_get_chain": (C.c_int, [VP, C.c_uint32, C.c_uint32, FP, DP]),
+    "pofd_ens_clear_chain": (C.c_int, [VP]),
+    "pofd_ens_acor": (C.c_int, [VP, C.c_uint32, DP, DP, DP]),
+    "pofd_ens_sample_z": (C.c_int, [VP, C.c_uint32, DP]),
+    "pofd_acor": (C.c_int, [FP, C.c_uint32, DP, DP, DP]),
     "pofd1d_stage_times": (C.c_int, [VP, DP, U32P]),
     "pofd_dbeam_create": (C.c_int, [C.POINTER(VP), DP, DP, C.c_uint64, D, D]),
     "pofd_dbeam_make_histogram": (C.c_int, [VP, C.c_uint32]),
@@ -541,3 +558,125 @@ class Context2(Context):
                 "split2", "sgpos1", "sgpos2"]
         return pd.reshape(n, n), dict(zip(keys, info))
 
+
+
+# ------------------------------------------------------------ sampler
+TARGET_1D, TARGET_2D, TARGET_GAUSS = 1, 2, 3
+
+
+def acor(x):
+    """(tau, mean, sigma) of a series (affineChainSet::acor)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    m, sg, t = C.c_double(), C.c_double(), C.c_double()
+    rc = lib().pofd_acor(x.ctypes.data_as(FP), x.size, C.byref(m),
+                         C.byref(sg), C.byref(t))
+    if rc:
+        raise PofdError(rc, "pofd_acor: series too short")
+    return t.value, m.value, sg.value
+
+
+class Ensemble:
+    """Device-resident affine stretch-move sampler over a likelihood
+    context (or the Gaussian validation target)."""
+
+    def __init__(self, ctx, target, nwalkers, npar, scalefac=2.0, seed=1):
+        self.ptr = VP()
+        self.ctx = ctx  # keep the context alive
+        self.nwalk, self.npar = nwalkers, npar
+        rc = lib().pofd_ens_create(C.byref(self.ptr),
+                                   ctx.ptr if ctx is not None else None,
+                                   target, nwalkers, npar, scalefac, seed)
+        if rc:
+            raise PofdError(rc, "pofd_ens_create failed")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().pofd_ens_destroy(self.ptr)
+            self.ptr = None
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().pofd_ens_last_error(self.ptr)
+            raise PofdError(rc, "%s: %s" % (what, (msg or b"").decode()))
+
+    def set_param_state(self, state):
+        st = np.ascontiguousarray(state, dtype=np.uint8)
+        self._check(lib().pofd_ens_set_param_state(
+            self.ptr, st.ctypes.data_as(C.POINTER(C.c_uint8))),
+            "pofd_ens_set_param_state")
+
+    def set_exponent_mode(self, fit_dimension):
+        """Acceptance exponent of z: the reference's nparams - nfixed - 1
+        (fit_dimension False) or the fit parameters less one (True).
+        Returns the exponent."""
+        e = C.c_float()
+        self._check(lib().pofd_ens_set_exponent_mode(self.ptr,
+                                                     int(fit_dimension),
+                                                     C.byref(e)),
+                    "pofd_ens_set_exponent_mode")
+        return e.value
+
+    def set_limits(self, lower=None, upper=None):
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            self._keep = getattr(self, "_keep", []) + [a]
+            return a.ctypes.data_as(FP)
+        self._check(lib().pofd_ens_set_limits(self.ptr, arr(lower),
+                                              arr(upper)),
+                    "pofd_ens_set_limits")
+
+    def set_gauss_target(self, mean, sigma):
+        m, s = _f64(mean), _f64(sigma)
+        self._check(lib().pofd_ens_set_gauss_target(self.ptr, _dp(m), _dp(s)),
+                    "pofd_ens_set_gauss_target")
+
+    def set_positions(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32).reshape(
+            self.nwalk, self.npar)
+        self._check(lib().pofd_ens_set_positions(self.ptr, t.ctypes.data_as(FP)),
+                    "pofd_ens_set_positions")
+
+    def run(self, nsteps, temperature=1.0, thin=1):
+        self._check(lib().pofd_ens_run(self.ptr, nsteps, temperature, thin),
+                    "pofd_ens_run")
+
+    def state(self):
+        th = np.zeros((self.nwalk, self.npar), dtype=np.float32)
+        ll = np.zeros(self.nwalk)
+        na = np.zeros(self.nwalk, dtype=np.uint32)
+        ne = np.zeros(self.nwalk, dtype=np.uint32)
+        self._check(lib().pofd_ens_get_state(
+            self.ptr, th.ctypes.data_as(FP), _dp(ll),
+            na.ctypes.data_as(U32P), ne.ctypes.data_as(U32P)),
+            "pofd_ens_get_state")
+        return th, ll, na, ne
+
+    def chain(self):
+        n = C.c_uint32()
+        self._check(lib().pofd_ens_chain_length(self.ptr, C.byref(n)),
+                    "pofd_ens_chain_length")
+        th = np.zeros((n.value, self.nwalk, self.npar), dtype=np.float32)
+        ll = np.zeros((n.value, self.nwalk))
+        if n.value:
+            self._check(lib().pofd_ens_get_chain(
+                self.ptr, 0, n.value, th.ctypes.data_as(FP), _dp(ll)),
+                "pofd_ens_get_chain")
+        return th, ll
+
+    def clear_chain(self):
+        self._check(lib().pofd_ens_clear_chain(self.ptr), "pofd_ens_clear_chain")
+
+    def acor(self, param):
+        t, m, s = C.c_double(), C.c_double(), C.c_double()
+        self._check(lib().pofd_ens_acor(self.ptr, param, C.byref(t),
+                                        C.byref(m), C.byref(s)),
+                    "pofd_ens_acor")
+        return t.value, m.value, s.value
+
+    def sample_z(self, n):
+        out = np.zeros(n)
+        self._check(lib().pofd_ens_sample_z(self.ptr, n, _dp(out)),
+                    "pofd_ens_sample_z")
+        return out

@@ -1,0 +1,384 @@
+// Host side of the device-resident ensemble sampler (C ABI pofd_ens_*) and
+// the autocorrelation estimate of affineChainSet (pofd_acor).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pofd_affine_amd.h"
+#include "ensemble_internal.h"
+
+using pofd_ens_dev::EnsDev;
+
+struct pofd_ens {
+  pofd_ctx* ctx = nullptr;
+  int target = 0;
+  uint32_t nwalk = 0, npar = 0;
+  EnsDev e{};
+  hipStream_t stream = nullptr;
+  void* dbuf[11] = {};
+  uint32_t capacity = 0, nstored = 0, step = 0;
+  bool positioned = false;
+  bool fit_dimension = false;  // exponent from the fit parameters only
+  std::string err;
+  ~pofd_ens() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : dbuf)
+      if (p) (void)hipFree(p);
+    if (e.chain) (void)hipFree(e.chain);
+    if (e.chainll) (void)hipFree(e.chainll);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int efail(pofd_ens* s, int code, const std::string& msg) {
+  if (s) s->err = msg;
+  return code;
+}
+
+int ehip(pofd_ens* s, hipError_t e, const char* what) {
+  return efail(s, POFD_E_INTERNAL,
+               std::string("pofd::hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+void update_dexp(pofd_ens* s) {
+  int nfixed = 0, nbonus = 0;
+  for (uint32_t i = 0; i < s->npar; ++i) {
+    if (s->e.state[i] == pofd_ens_dev::kStateFixed) ++nfixed;
+    if (s->e.state[i] == pofd_ens_dev::kStateBonus) ++nbonus;
+  }
+  // affineEnsemble::emptyMasterQueue, src/affineEnsemble.cc:1398: the
+  // reference's exponent counts all parameters, bonus ones included, less
+  // the fixed; the stretch move's detailed balance needs the number of fit
+  // parameters less one (the two agree when there are no bonus parameters)
+  const int d = static_cast<int>(s->npar) - nfixed - (s->fit_dimension ? nbonus : 0);
+  s->e.dexp = static_cast<float>(d - 1);
+}
+
+// evaluate n parameter rows x (device) into ll / st (device) on the stream
+int evaluate(pofd_ens* s, const float* x, uint32_t n, double* ll, int32_t* st) {
+  switch (s->target) {
+    case POFD_TARGET_1D:
+      return pofd1d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
+    case POFD_TARGET_2D:
+      return pofd2d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
+    case POFD_TARGET_GAUSS:
+      return pofd_ens_dev::launch_gauss(s->e, x, static_cast<int>(n), ll, st, s->stream)
+                 ? POFD_E_INTERNAL
+                 : POFD_OK;
+    default:
+      return POFD_E_ARG;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pofd_ens_create(pofd_ens** out, pofd_ctx* ctx, int target, uint32_t nwalkers,
+                    uint32_t npar, double scalefac, uint64_t seed) {
+  if (!out) return POFD_E_ARG;
+  *out = nullptr;
+  if (nwalkers < 2 || (nwalkers & 1) || npar == 0 ||
+      npar > static_cast<uint32_t>(pofd_ens_dev::kMaxPar) || !(scalefac > 1.0))
+    return POFD_E_ARG;
+  if ((target == POFD_TARGET_1D || target == POFD_TARGET_2D) && !ctx) return POFD_E_ARG;
+  if (target != POFD_TARGET_1D && target != POFD_TARGET_2D && target != POFD_TARGET_GAUSS)
+    return POFD_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return POFD_E_NOGPU;
+  pofd_ens* s = new pofd_ens();
+  s->ctx = ctx;
+  s->target = target;
+  s->nwalk = nwalkers;
+  s->npar = npar;
+  EnsDev& e = s->e;
+  e.nwalk = static_cast<int>(nwalkers);
+  e.npar = static_cast<int>(npar);
+  e.scalefac = scalefac;
+  e.seed = seed;
+  for (int i = 0; i < pofd_ens_dev::kMaxPar; ++i) {
+    e.gisig[i] = 1.0;
+    e.gmean[i] = 0.0;
+  }
+  update_dexp(s);
+  hipError_t he = hipStreamCreate(&s->stream);
+  if (he != hipSuccess) {
+    delete s;
+    return POFD_E_INTERNAL;
+  }
+  const size_t W = nwalkers, H = nwalkers / 2, P = npar;
+  const size_t sizes[11] = {W * P * sizeof(float), W * sizeof(double), H * P * sizeof(float),
+                            H * sizeof(double),    H * sizeof(int32_t), H * sizeof(float),
+                            H * sizeof(int32_t),   W * sizeof(uint32_t), W * sizeof(uint32_t),
+                            W * sizeof(int32_t),   64};
+  for (int i = 0; i < 11; ++i) {
+    he = hipMalloc(&s->dbuf[i], sizes[i]);
+    if (he == hipSuccess) he = hipMemset(s->dbuf[i], 0, sizes[i]);
+    if (he != hipSuccess) {
+      delete s;
+      return POFD_E_INTERNAL;
+    }
+  }
+  e.pos = static_cast<float*>(s->dbuf[0]);
+  e.ll = static_cast<double*>(s->dbuf[1]);
+  e.prop = static_cast<float*>(s->dbuf[2]);
+  e.llp = static_cast<double*>(s->dbuf[3]);
+  e.stp = static_cast<int32_t*>(s->dbuf[4]);
+  e.z = static_cast<float*>(s->dbuf[5]);
+  e.pvalid = static_cast<int32_t*>(s->dbuf[6]);
+  e.naccept = static_cast<uint32_t*>(s->dbuf[7]);
+  e.nerr = static_cast<uint32_t*>(s->dbuf[8]);
+  *out = s;
+  return POFD_OK;
+}
+
+void pofd_ens_destroy(pofd_ens* s) { delete s; }
+
+const char* pofd_ens_last_error(const pofd_ens* s) { return s ? s->err.c_str() : ""; }
+
+int pofd_ens_set_param_state(pofd_ens* s, const uint8_t* state) {
+  if (!s || !state) return POFD_E_ARG;
+  for (uint32_t i = 0; i < s->npar; ++i) {
+    if (state[i] > pofd_ens_dev::kStateBonus)
+      return efail(s, POFD_E_ARG, "pofd_ens_set_param_state: state must be 0, 1 or 2");
+    s->e.state[i] = state[i];
+  }
+  update_dexp(s);
+  return POFD_OK;
+}
+
+int pofd_ens_set_exponent_mode(pofd_ens* s, int fit_dimension, float* exponent) {
+  if (!s) return POFD_E_ARG;
+  s->fit_dimension = fit_dimension != 0;
+  update_dexp(s);
+  if (exponent) *exponent = s->e.dexp;
+  return POFD_OK;
+}
+
+int pofd_ens_set_limits(pofd_ens* s, const float* lower, const float* upper) {
+  if (!s) return POFD_E_ARG;
+  for (uint32_t i = 0; i < s->npar; ++i) {
+    s->e.has_lower[i] = (lower && !std::isnan(lower[i])) ? 1 : 0;
+    s->e.lower[i] = s->e.has_lower[i] ? lower[i] : 0.0f;
+    s->e.has_upper[i] = (upper && !std::isnan(upper[i])) ? 1 : 0;
+    s->e.upper[i] = s->e.has_upper[i] ? upper[i] : 0.0f;
+    if (s->e.has_lower[i] && s->e.has_upper[i] && s->e.lower[i] > s->e.upper[i])
+      return efail(s, POFD_E_ARG, "pofd_ens_set_limits: lower limit above upper limit");
+  }
+  return POFD_OK;
+}
+
+int pofd_ens_set_gauss_target(pofd_ens* s, const double* mean, const double* sigma) {
+  if (!s || !mean || !sigma) return POFD_E_ARG;
+  for (uint32_t i = 0; i < s->npar; ++i) {
+    if (!(sigma[i] > 0.0))
+      return efail(s, POFD_E_ARG, "pofd_ens_set_gauss_target: sigma must be positive");
+    s->e.gmean[i] = mean[i];
+    s->e.gisig[i] = 1.0 / sigma[i];
+  }
+  return POFD_OK;
+}
+
+int pofd_ens_set_positions(pofd_ens* s, const float* theta) {
+  if (!s || !theta) return POFD_E_ARG;
+  const size_t W = s->nwalk, P = s->npar;
+  hipError_t he = hipMemcpy(s->e.pos, theta, W * P * sizeof(float), hipMemcpyHostToDevice);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(positions)");
+  int32_t* st = static_cast<int32_t*>(s->dbuf[9]);
+  int rc = evaluate(s, s->e.pos, s->nwalk, s->e.ll, st);
+  if (rc) return efail(s, rc, "pofd_ens_set_positions: likelihood of the initial positions failed");
+  std::vector<int32_t> hst(W);
+  he = hipStreamSynchronize(s->stream);
+  if (he == hipSuccess) he = hipMemcpy(hst.data(), st, W * sizeof(int32_t), hipMemcpyDeviceToHost);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(status)");
+  for (size_t w = 0; w < W; ++w)
+    if (hst[w] != POFD_OK && hst[w] != POFD_E_BADLIKE)
+      return efail(s, hst[w], "pofd_ens_set_positions: initial walker " + std::to_string(w) +
+                                  " has no valid likelihood");
+  (void)hipMemset(s->e.naccept, 0, W * sizeof(uint32_t));
+  (void)hipMemset(s->e.nerr, 0, W * sizeof(uint32_t));
+  s->nstored = 0;
+  s->step = 0;
+  s->positioned = true;
+  return POFD_OK;
+}
+
+int pofd_ens_run(pofd_ens* s, uint32_t nsteps, double temperature, uint32_t thin) {
+  if (!s) return POFD_E_ARG;
+  if (!s->positioned) return efail(s, POFD_E_ARG, "pofd_ens_run: call pofd_ens_set_positions first");
+  if (!(temperature > 0.0))
+    return efail(s, POFD_E_ARG, "affineEnsemble::doMasterStep: Invalid temperature");
+  const uint32_t nst = (thin == 0) ? 0 : nsteps / thin;
+  if (s->nstored + nst > s->capacity) {  // grow the chain storage
+    const uint32_t cap = std::max(s->nstored + nst, 2 * s->capacity);
+    const size_t W = s->nwalk, P = s->npar;
+    float* c = nullptr;
+    double* cl = nullptr;
+    hipError_t he = hipMalloc(&c, static_cast<size_t>(cap) * W * P * sizeof(float));
+    if (he == hipSuccess) he = hipMalloc(&cl, static_cast<size_t>(cap) * W * sizeof(double));
+    if (he != hipSuccess) {
+      if (c) (void)hipFree(c);
+      return ehip(s, he, "hipMalloc(chain)");
+    }
+    if (s->nstored) {
+      (void)hipStreamSynchronize(s->stream);
+      (void)hipMemcpy(c, s->e.chain, static_cast<size_t>(s->nstored) * W * P * sizeof(float),
+                      hipMemcpyDeviceToDevice);
+      (void)hipMemcpy(cl, s->e.chainll, static_cast<size_t>(s->nstored) * W * sizeof(double),
+                      hipMemcpyDeviceToDevice);
+    }
+    if (s->e.chain) (void)hipFree(s->e.chain);
+    if (s->e.chainll) (void)hipFree(s->e.chainll);
+    s->e.chain = c;
+    s->e.chainll = cl;
+    s->capacity = cap;
+  }
+  const double inv_temp = 1.0 / temperature;
+  const uint32_t hw = s->nwalk / 2;
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    const uint32_t step = s->step++;
+    const int store = (thin != 0 && (k + 1) % thin == 0) ? 1 : 0;
+    for (uint32_t half = 0; half < 2; ++half) {
+      if (pofd_ens_dev::launch_propose(s->e, half, step, s->stream))
+        return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
+      int rc = evaluate(s, s->e.prop, hw, s->e.llp, s->e.stp);
+      if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+      if (pofd_ens_dev::launch_accept(s->e, half, step, inv_temp, store, s->nstored,
+                                      s->stream))
+        return efail(s, POFD_E_INTERNAL, "pofd_ens_run: accept launch failed");
+    }
+    if (store) ++s->nstored;
+  }
+  hipError_t he = hipStreamSynchronize(s->stream);
+  if (he != hipSuccess) return ehip(s, he, "hipStreamSynchronize");
+  return POFD_OK;
+}
+
+int pofd_ens_get_state(pofd_ens* s, float* theta, double* ll, uint32_t* naccept,
+                       uint32_t* nerr) {
+  if (!s) return POFD_E_ARG;
+  const size_t W = s->nwalk, P = s->npar;
+  hipError_t he = hipStreamSynchronize(s->stream);
+  if (he == hipSuccess && theta)
+    he = hipMemcpy(theta, s->e.pos, W * P * sizeof(float), hipMemcpyDeviceToHost);
+  if (he == hipSuccess && ll) he = hipMemcpy(ll, s->e.ll, W * sizeof(double), hipMemcpyDeviceToHost);
+  if (he == hipSuccess && naccept)
+    he = hipMemcpy(naccept, s->e.naccept, W * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (he == hipSuccess && nerr)
+    he = hipMemcpy(nerr, s->e.nerr, W * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(state)");
+  return POFD_OK;
+}
+
+int pofd_ens_chain_length(const pofd_ens* s, uint32_t* nstored) {
+  if (!s || !nstored) return POFD_E_ARG;
+  *nstored = s->nstored;
+  return POFD_OK;
+}
+
+int pofd_ens_get_chain(pofd_ens* s, uint32_t first, uint32_t count, float* theta, double* ll) {
+  if (!s) return POFD_E_ARG;
+  if (first + count > s->nstored)
+    return efail(s, POFD_E_ARG, "pofd_ens_get_chain: beyond the stored steps");
+  const size_t W = s->nwalk, P = s->npar;
+  hipError_t he = hipStreamSynchronize(s->stream);
+  if (he == hipSuccess && theta && count)
+    he = hipMemcpy(theta, s->e.chain + static_cast<size_t>(first) * W * P,
+                   static_cast<size_t>(count) * W * P * sizeof(float), hipMemcpyDeviceToHost);
+  if (he == hipSuccess && ll && count)
+    he = hipMemcpy(ll, s->e.chainll + static_cast<size_t>(first) * W,
+                   static_cast<size_t>(count) * W * sizeof(double), hipMemcpyDeviceToHost);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(chain)");
+  return POFD_OK;
+}
+
+int pofd_ens_clear_chain(pofd_ens* s) {
+  if (!s) return POFD_E_ARG;
+  s->nstored = 0;
+  return POFD_OK;
+}
+
+int pofd_ens_sample_z(pofd_ens* s, uint32_t n, double* out) {
+  if (!s || !out) return POFD_E_ARG;
+  double* d = nullptr;
+  hipError_t he = hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(double));
+  if (he != hipSuccess) return ehip(s, he, "hipMalloc(z)");
+  int rc = pofd_ens_dev::launch_zsample(s->e, n, d, s->stream);
+  he = hipStreamSynchronize(s->stream);
+  if (he == hipSuccess && !rc) he = hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (rc) return efail(s, POFD_E_INTERNAL, "pofd_ens_sample_z: launch failed");
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(z)");
+  return POFD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// affineChainSet::acor (src/affineChainSet.cc:880-950), Goodman's ACOR:
+// self-consistent window of winmult tau over maxlag = taumax * winmult
+// lags, recursing on pairwise sums while tau is too long for the window.
+constexpr unsigned kWinMult = 5, kTauMax = 2, kMaxLag = kTauMax * kWinMult, kMinFac = 5;
+
+void acor_rec(double& mean, double& sigma, double& tau, std::vector<float>& X, unsigned L) {
+  const double invL = 1.0 / static_cast<double>(L);
+  mean = X[0];
+  for (unsigned i = 1; i < L; ++i) mean += X[i];
+  mean *= invL;
+  for (unsigned i = 0; i < L; ++i) X[i] = static_cast<float>(X[i] - mean);
+  if (L < kMinFac * kMaxLag) return;
+  double C[kMaxLag + 1];
+  for (unsigned s = 0; s <= kMaxLag; ++s) C[s] = 0.0;
+  const unsigned iMax = L - kMaxLag;
+  const double invImax = 1.0 / static_cast<double>(iMax);
+  for (unsigned i = 0; i < iMax; ++i)
+    for (unsigned s = 0; s <= kMaxLag; ++s) C[s] += X[i] * X[i + s];
+  for (unsigned s = 0; s <= kMaxLag; ++s) C[s] *= invImax;
+  double D = C[0];
+  for (unsigned s = 1; s <= kMaxLag; ++s) D += 2 * C[s];
+  sigma = std::sqrt(D * invL);
+  tau = D / C[0];
+  if (tau * kWinMult < kMaxLag) return;
+  const unsigned Lh = L / 2;
+  for (unsigned i = 0, j = 0; i < Lh; ++i, j += 2) X[i] = X[j] + X[j + 1];
+  double newMean;
+  acor_rec(newMean, sigma, tau, X, Lh);
+  D = 0.25 * sigma * sigma * static_cast<double>(L);
+  tau = D / C[0];
+  sigma = std::sqrt(D * invL);
+}
+}  // namespace
+
+extern "C" int pofd_acor(const float* x, uint32_t n, double* mean, double* sigma, double* tau) {
+  if (!x || !mean || !sigma || !tau) return POFD_E_ARG;
+  *sigma = *tau = NAN;
+  if (n < kMinFac * kMaxLag) return POFD_E_ARG;  // affineChainSet::getAcor "succ = false"
+  std::vector<float> X(x, x + n);
+  acor_rec(*mean, *sigma, *tau, X, n);
+  return POFD_OK;
+}
+
+extern "C" int pofd_ens_acor(pofd_ens* s, uint32_t param, double* tau, double* mean,
+                             double* sigma) {
+  if (!s || param >= s->npar) return POFD_E_ARG;
+  const uint32_t n = s->nstored;
+  const size_t W = s->nwalk, P = s->npar;
+  std::vector<float> ch(static_cast<size_t>(n) * W * P);
+  int rc = pofd_ens_get_chain(s, 0, n, ch.data(), nullptr);
+  if (rc) return rc;
+  // affineChainSet::getAverageParamVector (:830-866): walker average per step
+  std::vector<float> avg(n);
+  const double norm = 1.0 / static_cast<double>(W);
+  for (uint32_t j = 0; j < n; ++j) {
+    double cum = 0.0;
+    for (size_t k = 0; k < W; ++k) cum += ch[(static_cast<size_t>(j) * W + k) * P + param];
+    avg[j] = static_cast<float>(norm * cum);
+  }
+  return pofd_acor(avg.data(), n, mean, sigma, tau);
+}

@@ -1,0 +1,44 @@
+// Declarations shared by the ensemble sampler's host code and kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace pofd_ens_dev {
+
+constexpr int kMaxPar = 64;
+enum { kStateFit = 0, kStateFixed = 1, kStateBonus = 2 };
+constexpr int kBadLike = 4;  // POFD_E_BADLIKE: a likelihood value, not an error
+
+struct EnsDev {
+  int nwalk, npar;
+  float dexp;                    // nparams - nfixed - 1 (affineEnsemble.cc:1398)
+  double scalefac;
+  uint64_t seed;
+  uint8_t state[kMaxPar];
+  uint8_t has_lower[kMaxPar], has_upper[kMaxPar];
+  float lower[kMaxPar], upper[kMaxPar];
+  double gmean[kMaxPar], gisig[kMaxPar];  // Gaussian test target
+  float* pos;        // nwalk x npar   current positions
+  double* ll;        // nwalk          current log likelihoods
+  float* prop;       // nwalk/2 x npar proposals of one half
+  double* llp;       // nwalk/2
+  int32_t* stp;      // nwalk/2        likelihood status of the proposals
+  float* z;          // nwalk/2
+  int32_t* pvalid;   // nwalk/2        proposal inside the limits
+  uint32_t* naccept; // nwalk
+  uint32_t* nerr;    // nwalk          rejected for an invalid proposal or likelihood
+  float* chain;      // capacity x nwalk x npar
+  double* chainll;   // capacity x nwalk
+};
+
+int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st);
+int launch_gauss(const EnsDev& e, const float* x, int n, double* ll, int32_t* stv,
+                 hipStream_t st);
+int launch_accept(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp, int store,
+                  uint32_t slot, hipStream_t st);
+int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st);
+
+}  // namespace pofd_ens_dev

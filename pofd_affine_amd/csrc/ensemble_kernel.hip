@@ -1,0 +1,220 @@
+// Device-resident affine-invariant ensemble sampler (the stretch move of
+// affineEnsemble, src/affineEnsemble.cc:116-126, 1144-1420): proposals,
+// acceptance and chain storage run on the GPU between the batched
+// likelihood launches, so a whole run needs no host round trip per step.
+//
+//   propose   for every walker of one half: a partner drawn uniformly from
+//             the other half (ran::selectFromRange, src/ran.cc:37-41),
+//             z ~ g(z) ~ 1/sqrt(z) on [1/a, a] (generateZ, :116-126),
+//             new = z old + (1 - z) partner on the fit parameters, fixed
+//             parameters kept, bonus parameters NaN, and up to 40 redraws of
+//             z while the proposal leaves the parameter limits
+//             (generateNewStep, :1144-1228)
+//   accept    prob = exp((nparams - nfixed - 1) ln z + (L_new - L_old)/T),
+//             accept when prob >= 1 or u < prob (emptyMasterQueue,
+//             :1398-1420); the walker's current state and the stored chain
+//             are updated
+//
+// Random numbers: Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11), keyed by the run seed and
+// counted by (step, half, walker, draw), so every run is reproducible and
+// independent of the launch shape.
+#include <hip/hip_runtime.h>
+
+#include "ensemble_internal.h"
+
+namespace pofd_ens_dev {
+
+namespace {
+
+__device__ __forceinline__ void philox_round(uint32_t& c0, uint32_t& c1, uint32_t& c2,
+                                             uint32_t& c3, uint32_t k0, uint32_t k1) {
+  const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+  const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+  const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+  const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+  c0 = hi1 ^ c1 ^ k0;
+  c1 = lo1;
+  c2 = hi0 ^ c3 ^ k1;
+  c3 = lo0;
+}
+
+// four 32-bit words for counter (a, b, c, d) under key (k0, k1)
+__device__ __forceinline__ uint4 philox(uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                                        uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    philox_round(a, b, c, d, k0, k1);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return make_uint4(a, b, c, d);
+}
+
+// uniform double in [0, 1) from 53 random bits
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+  const uint64_t v = (static_cast<uint64_t>(hi) << 21) ^ (static_cast<uint64_t>(lo) >> 11);
+  return static_cast<double>(v) * 1.1102230246251565404e-16;  // 2^-53
+}
+
+struct Draws {
+  double u0, u1;
+};
+
+__device__ __forceinline__ Draws draws(const EnsDev& e, uint32_t step, uint32_t half,
+                                       uint32_t walker, uint32_t draw) {
+  const uint4 r = philox(step, (half << 16) ^ draw, walker, 0x50F0D00Du,
+                         static_cast<uint32_t>(e.seed), static_cast<uint32_t>(e.seed >> 32));
+  return Draws{u53(r.x, r.y), u53(r.z, r.w)};
+}
+
+// affineEnsemble::generateZ: G^{-1}(x) = ((a-1) x + 1)^2 / a
+__device__ __forceinline__ float gen_z(double a, double u) {
+  const double val = (a - 1.0) * u + 1.0;
+  return static_cast<float>(val * val / a);
+}
+
+__device__ __forceinline__ bool params_valid(const EnsDev& e, const float* p) {
+  for (int i = 0; i < e.npar; ++i) {
+    if (e.state[i] != kStateFit) continue;
+    const float v = p[i];
+    if (!(v == v)) return false;  // NaN
+    if (e.has_lower[i] && v < e.lower[i]) return false;
+    if (e.has_upper[i] && v > e.upper[i]) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+// one thread per walker of half `half`
+__global__ __launch_bounds__(64) void propose_kernel(const EnsDev e, const uint32_t half,
+                                                     const uint32_t step) {
+  const int hw = e.nwalk / 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hw) return;
+  const int k = static_cast<int>(half) * hw + t;       // walker updated
+  const int other = (half == 0) ? hw : 0;              // partner half
+  const Draws d0 = draws(e, step, half, k, 0);
+  int j = other + static_cast<int>(static_cast<double>(hw) * d0.u0);
+  if (j >= other + hw) j = other + hw - 1;
+  const float* xo = e.pos + static_cast<size_t>(k) * e.npar;
+  const float* xp = e.pos + static_cast<size_t>(j) * e.npar;
+  float* y = e.prop + static_cast<size_t>(t) * e.npar;
+  float z = gen_z(e.scalefac, d0.u1);
+  bool ok = false;
+  for (int it = 0; it <= 41 && !ok; ++it) {  // first try + maxiters redraws
+    if (it > 0) z = gen_z(e.scalefac, draws(e, step, half, k, static_cast<uint32_t>(it)).u0);
+    const float omz = 1.0f - z;
+    for (int i = 0; i < e.npar; ++i) {
+      const int st = e.state[i];
+      if (st == kStateFixed)
+        y[i] = xo[i];
+      else if (st == kStateBonus)
+        y[i] = __int_as_float(0x7fc00000);
+      else
+        y[i] = z * xo[i] + omz * xp[i];
+    }
+    ok = params_valid(e, y);
+  }
+  e.z[t] = z;
+  e.pvalid[t] = ok ? 1 : 0;
+}
+
+// diagonal Gaussian test target: L = -1/2 sum ((x - mu)/sigma)^2 over the
+// fit parameters
+__global__ __launch_bounds__(64) void gauss_target_kernel(const EnsDev e, const float* x,
+                                                          const int n, double* ll,
+                                                          int32_t* st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float* p = x + static_cast<size_t>(t) * e.npar;
+  double s = 0.0;
+  for (int i = 0; i < e.npar; ++i) {
+    if (e.state[i] != kStateFit) continue;
+    const double r = (static_cast<double>(p[i]) - e.gmean[i]) * e.gisig[i];
+    s += r * r;
+  }
+  ll[t] = -0.5 * s;
+  st[t] = 0;
+}
+
+__global__ __launch_bounds__(64) void accept_kernel(const EnsDev e, const uint32_t half,
+                                                    const uint32_t step, const double inv_temp,
+                                                    const int store, const uint32_t slot) {
+  const int hw = e.nwalk / 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hw) return;
+  const int k = static_cast<int>(half) * hw + t;
+  float* xo = e.pos + static_cast<size_t>(k) * e.npar;
+  const float* y = e.prop + static_cast<size_t>(t) * e.npar;
+  bool accept = false;
+  if (!e.pvalid[t]) {
+    e.nerr[k] += 1;  // no valid proposal after the redraws
+  } else if (e.stp[t] != 0 && e.stp[t] != kBadLike) {
+    e.nerr[k] += 1;  // the likelihood failed (split, model, mean)
+  } else {
+    const double lnew = e.llp[t], lold = e.ll[k];
+    if (isnan(lnew) || isinf(lnew)) {
+      e.nerr[k] += 1;
+    } else {
+      const double prob = exp(static_cast<double>(e.dexp) * log(static_cast<double>(e.z[t])) +
+                              inv_temp * (lnew - lold));
+      if (prob >= 1.0) {
+        accept = true;
+      } else {
+        const double u = draws(e, step, half, k, 100).u0;
+        accept = u < prob;
+      }
+    }
+  }
+  if (accept) {
+    for (int i = 0; i < e.npar; ++i) xo[i] = y[i];
+    e.ll[k] = e.llp[t];
+    e.naccept[k] += 1;
+  }
+  if (store) {
+    float* c = e.chain + (static_cast<size_t>(slot) * e.nwalk + k) * e.npar;
+    for (int i = 0; i < e.npar; ++i) c[i] = xo[i];
+    e.chainll[static_cast<size_t>(slot) * e.nwalk + k] = e.ll[k];
+  }
+}
+
+// z draws for tests of generateZ's distribution
+__global__ __launch_bounds__(256) void zsample_kernel(const EnsDev e, const uint32_t n,
+                                                      double* out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint4 r = philox(t, 0xFFFFu, 0x2A2Au, 0x0BADF00Du, static_cast<uint32_t>(e.seed),
+                         static_cast<uint32_t>(e.seed >> 32));
+  const double a = e.scalefac, v = (a - 1.0) * u53(r.x, r.y) + 1.0;
+  out[t] = v * v / a;
+}
+
+int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(propose_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_gauss(const EnsDev& e, const float* x, int n, double* ll, int32_t* stv,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(gauss_target_kernel, dim3((n + 63) / 64), dim3(64), 0, st, e, x, n, ll,
+                     stv);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_accept(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp, int store,
+                  uint32_t slot, hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(accept_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
+                     inv_temp, store, slot);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(zsample_kernel, dim3((n + 255) / 256), dim3(256), 0, st, e, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace pofd_ens_dev

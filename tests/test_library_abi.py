@@ -5,6 +5,7 @@ import ctypes
 import os
 
 import numpy as np
+import pytest
 
 import pyoracle as O
 from pofd_affine_amd import _lib, fitsio, synth
@@ -206,3 +207,26 @@ def test_data2_binning_bit_exact():
     o1, o2, om = O2.data2_mean_subtract(d1, d2)
     assert np.array_equal(g1, o1) and np.array_equal(g2, o2)
     assert np.array_equal(gm, om)
+
+
+def test_acor_ar1_closed_form():
+    """affineChainSet::acor (Goodman's ACOR, src/affineChainSet.cc:880-950)
+    on AR(1) series x_t = rho x_{t-1} + e_t, whose integrated
+    autocorrelation time is (1 + rho)/(1 - rho) and whose mean has standard
+    error sqrt(tau var / n).  The reference's own fixture (testvec.txt) is
+    not among its test data, so the closed form pins it; short series are
+    refused (L < minfac * maxlag = 50)."""
+    rng = np.random.default_rng(12)
+    for rho in (0.0, 0.5, 0.9, 0.98):
+        n = 1 << 20
+        e = rng.normal(size=n)
+        from scipy.signal import lfilter
+        x = lfilter([1.0], [1.0, -rho], e)
+        tau, mean, sigma = _lib.acor(x)
+        want = (1 + rho) / (1 - rho)
+        assert tau == pytest.approx(want, rel=0.08), (rho, tau, want)
+        var = 1.0 / (1 - rho * rho)
+        assert sigma == pytest.approx(np.sqrt(want * var / n), rel=0.08)
+        assert abs(mean - x.mean()) < 1e-5 * (1 + abs(x.mean()))
+    with pytest.raises(_lib.PofdError):
+        _lib.acor(np.zeros(49))
