@@ -382,6 +382,57 @@ int pofd_ens_sample_z(pofd_ens *ens, uint32_t n, double *out);
 int pofd_acor(const float *x, uint32_t n, double *mean, double *sigma,
               double *tau);
 
+/* ======================================================================
+   Delayed acceptance with a learned surrogate of the log likelihood.  No
+   reference counterpart: the network is an MLP nin -> H -> H -> H -> 1 with
+   GELU activations, bfloat16 weights and activations, float accumulation
+   and float master weights, trained on the GPU with Adam (Kingma and Ba,
+   2015) on (theta, exact logL) pairs, inputs and targets standardised.  In
+   the sampler it is the first stage of a two-stage acceptance (Christen
+   and Fox, 2005), so the chain keeps the exact posterior; the exact
+   likelihood runs only for the proposals the first stage passes.
+   ====================================================================== */
+typedef struct pofd_mlp pofd_mlp;
+int pofd_mlp_create(pofd_mlp **out, uint32_t nin, uint32_t hidden,
+                    uint64_t seed);
+void pofd_mlp_destroy(pofd_mlp *mlp);
+const char *pofd_mlp_last_error(const pofd_mlp *mlp);
+int pofd_mlp_info(const pofd_mlp *mlp, uint32_t *nin, uint32_t *hidden);
+/* parameter vector: W1[H x nin], b1[H], W2[H x H], b2, W3[H x H], b3,
+   W4[1 x H], b4[1] at offsets[8] (each segment padded to 8 floats) */
+int pofd_mlp_nparams(const pofd_mlp *mlp, uint32_t *n);
+int pofd_mlp_layout(const pofd_mlp *mlp, uint64_t *offsets);
+int pofd_mlp_get_params(pofd_mlp *mlp, float *w);
+int pofd_mlp_set_params(pofd_mlp *mlp, const float *w);
+/* inputs x are standardised as (x - xmean)/xstd, the prediction is
+   out*ystd + ymean */
+int pofd_mlp_set_standardization(pofd_mlp *mlp, const float *xmean,
+                                 const float *xstd, double ymean,
+                                 double ystd);
+/* x[n x nin]; out[n] in log-likelihood units */
+int pofd_mlp_predict(pofd_mlp *mlp, const float *x, uint32_t n, double *out);
+/* the gradient of the mean squared error on standardised targets with
+   respect to every parameter (grad[nparams]) and the loss, for one batch */
+int pofd_mlp_loss_grad(pofd_mlp *mlp, const float *x, const double *y,
+                       uint32_t n, float *grad, double *loss);
+/* Adam training: standardisation from the training rows, the last
+   holdout_frac of the rows (after one shuffle) held out; RMS errors of the
+   trained network in log-likelihood units */
+int pofd_mlp_train(pofd_mlp *mlp, const float *x, const double *y,
+                   uint32_t n, uint32_t epochs, uint32_t batch, float lr,
+                   float holdout_frac, uint64_t seed, double *train_rms,
+                   double *holdout_rms);
+/* attach a trained surrogate to the sampler: its inputs are the ensemble
+   parameters param_index[nidx]; delayed acceptance is switched on only when
+   holdout_rms <= gate (0.5 in log-likelihood units is the intended gate),
+   *enabled tells which; mlp NULL switches it off */
+int pofd_ens_set_surrogate(pofd_ens *ens, pofd_mlp *mlp,
+                           const uint32_t *param_index, uint32_t nidx,
+                           double gate, double holdout_rms, int *enabled);
+/* proposals made and exact likelihood evaluations since set_positions */
+int pofd_ens_counts(const pofd_ens *ens, uint64_t *n_proposed,
+                    uint64_t *n_exact);
+
 #ifdef __cplusplus
 }
 #endif

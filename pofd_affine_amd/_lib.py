@@ -86,6 +86,25 @@ _SIGS = {
     "pofd_ens_acor": (C.c_int, [VP, C.c_uint32, DP, DP, DP]),
     "pofd_ens_sample_z": (C.c_int, [VP, C.c_uint32, DP]),
     "pofd_acor": (C.c_int, [FP, C.c_uint32, DP, DP, DP]),
+    "pofd_ens_set_surrogate": (C.c_int, [VP, VP, U32P, C.c_uint32, D, D,
+                                         C.POINTER(C.c_int)]),
+    "pofd_ens_counts": (C.c_int, [VP, C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)]),
+    "pofd_mlp_create": (C.c_int, [C.POINTER(VP), C.c_uint32, C.c_uint32,
+                                  C.c_uint64]),
+    "pofd_mlp_destroy": (None, [VP]),
+    "pofd_mlp_last_error": (C.c_char_p, [VP]),
+    "pofd_mlp_info": (C.c_int, [VP, U32P, U32P]),
+    "pofd_mlp_nparams": (C.c_int, [VP, U32P]),
+    "pofd_mlp_layout": (C.c_int, [VP, C.POINTER(C.c_uint64)]),
+    "pofd_mlp_get_params": (C.c_int, [VP, FP]),
+    "pofd_mlp_set_params": (C.c_int, [VP, FP]),
+    "pofd_mlp_set_standardization": (C.c_int, [VP, FP, FP, D, D]),
+    "pofd_mlp_predict": (C.c_int, [VP, FP, C.c_uint32, DP]),
+    "pofd_mlp_loss_grad": (C.c_int, [VP, FP, DP, C.c_uint32, FP, DP]),
+    "pofd_mlp_train": (C.c_int, [VP, FP, DP, C.c_uint32, C.c_uint32,
+                                 C.c_uint32, C.c_float, C.c_float, C.c_uint64,
+                                 DP, DP]),
     "pofd1d_stage_times": (C.c_int, [VP, DP, U32P]),
     "pofd_dbeam_create": (C.c_int, [C.POINTER(VP), DP, DP, C.c_uint64, D, D]),
     "pofd_dbeam_make_histogram": (C.c_int, [VP, C.c_uint32]),
@@ -680,3 +699,113 @@ class Ensemble:
         self._check(lib().pofd_ens_sample_z(self.ptr, n, _dp(out)),
                     "pofd_ens_sample_z")
         return out
+
+    def set_surrogate(self, mlp, param_index=None, gate=0.5,
+                      holdout_rms=None):
+        """Delayed acceptance with the network mlp over the parameters
+        param_index; switched on only when holdout_rms <= gate (the
+        network's last training holdout error by default).  mlp None
+        switches it off.  Returns whether it is on."""
+        self._mlp = mlp  # keep the network alive
+        if mlp is None:
+            self._check(lib().pofd_ens_set_surrogate(self.ptr, None, None, 0,
+                                                     gate, 0.0, None),
+                        "pofd_ens_set_surrogate")
+            return False
+        idx = np.ascontiguousarray(param_index, dtype=np.uint32)
+        if holdout_rms is None:
+            holdout_rms = mlp.holdout_rms
+        on = C.c_int()
+        self._check(lib().pofd_ens_set_surrogate(
+            self.ptr, mlp.ptr, idx.ctypes.data_as(U32P), idx.size, gate,
+            holdout_rms, C.byref(on)), "pofd_ens_set_surrogate")
+        return bool(on.value)
+
+    def counts(self):
+        """(proposals, exact likelihood evaluations) since set_positions."""
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(lib().pofd_ens_counts(self.ptr, C.byref(a), C.byref(b)),
+                    "pofd_ens_counts")
+        return a.value, b.value
+
+
+# ------------------------------------------------------------ surrogate
+class Mlp:
+    """Log-likelihood surrogate nin -> H -> H -> H -> 1 (GELU, bfloat16
+    weights and activations, float accumulation), trained on the GPU."""
+
+    def __init__(self, nin, hidden=256, seed=1):
+        self.ptr = VP()
+        self.nin, self.hidden = nin, hidden
+        self.train_rms = self.holdout_rms = float("nan")
+        rc = lib().pofd_mlp_create(C.byref(self.ptr), nin, hidden, seed)
+        if rc:
+            raise PofdError(rc, "pofd_mlp_create failed")
+        n = C.c_uint32()
+        self._check(lib().pofd_mlp_nparams(self.ptr, C.byref(n)),
+                    "pofd_mlp_nparams")
+        self.nparams = n.value
+        off = (C.c_uint64 * 8)()
+        self._check(lib().pofd_mlp_layout(self.ptr, off), "pofd_mlp_layout")
+        self.offsets = [int(v) for v in off]
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().pofd_mlp_destroy(self.ptr)
+            self.ptr = None
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().pofd_mlp_last_error(self.ptr)
+            raise PofdError(rc, "%s: %s" % (what, (msg or b"").decode()))
+
+    def params(self):
+        w = np.zeros(self.nparams, dtype=np.float32)
+        self._check(lib().pofd_mlp_get_params(self.ptr, w.ctypes.data_as(FP)),
+                    "pofd_mlp_get_params")
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.nparams
+        self._check(lib().pofd_mlp_set_params(self.ptr, w.ctypes.data_as(FP)),
+                    "pofd_mlp_set_params")
+
+    def set_standardization(self, xmean, xstd, ymean, ystd):
+        m = np.ascontiguousarray(xmean, dtype=np.float32)
+        s = np.ascontiguousarray(xstd, dtype=np.float32)
+        self._check(lib().pofd_mlp_set_standardization(
+            self.ptr, m.ctypes.data_as(FP), s.ctypes.data_as(FP),
+            float(ymean), float(ystd)), "pofd_mlp_set_standardization")
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.nin)
+        out = np.zeros(x.shape[0])
+        self._check(lib().pofd_mlp_predict(self.ptr, x.ctypes.data_as(FP),
+                                           x.shape[0], _dp(out)),
+                    "pofd_mlp_predict")
+        return out
+
+    def loss_grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.nin)
+        y = _f64(y)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        self._check(lib().pofd_mlp_loss_grad(
+            self.ptr, x.ctypes.data_as(FP), _dp(y), x.shape[0],
+            g.ctypes.data_as(FP), C.byref(loss)), "pofd_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, x, y, epochs=100, batch=256, lr=1e-3, holdout_frac=0.1,
+              seed=1):
+        """Adam training; returns (train RMS, holdout RMS) in the units of
+        y."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.nin)
+        y = _f64(y)
+        tr, ho = C.c_double(), C.c_double()
+        self._check(lib().pofd_mlp_train(
+            self.ptr, x.ctypes.data_as(FP), _dp(y), x.shape[0], epochs, batch,
+            lr, holdout_frac, seed, C.byref(tr), C.byref(ho)),
+            "pofd_mlp_train")
+        self.train_rms, self.holdout_rms = tr.value, ho.value
+        return tr.value, ho.value

@@ -13,6 +13,10 @@
 
 using pofd_ens_dev::EnsDev;
 
+// surrogate predictions on the device (mlp.cpp)
+int pofd_mlp_predict_device(pofd_mlp* s, const float* src, int stride, const int* idx, int rows,
+                            double* out, hipStream_t st);
+
 struct pofd_ens {
   pofd_ctx* ctx = nullptr;
   int target = 0;
@@ -23,6 +27,12 @@ struct pofd_ens {
   uint32_t capacity = 0, nstored = 0, step = 0;
   bool positioned = false;
   bool fit_dimension = false;  // exponent from the fit parameters only
+  // delayed acceptance
+  pofd_mlp* mlp = nullptr;
+  bool use_da = false;
+  int* sidx = nullptr;         // parameter index of each surrogate input
+  void* dabuf[8] = {};
+  uint64_t n_exact = 0, n_proposed = 0;
   std::string err;
   ~pofd_ens() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -30,6 +40,9 @@ struct pofd_ens {
       if (p) (void)hipFree(p);
     if (e.chain) (void)hipFree(e.chain);
     if (e.chainll) (void)hipFree(e.chainll);
+    for (void* p : dabuf)
+      if (p) (void)hipFree(p);
+    if (sidx) (void)hipFree(sidx);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -74,6 +87,41 @@ int evaluate(pofd_ens* s, const float* x, uint32_t n, double* ll, int32_t* st) {
     default:
       return POFD_E_ARG;
   }
+}
+
+// surrogate log likelihoods of all walkers' current positions
+int refresh_surrogate(pofd_ens* s) {
+  if (!s->use_da || !s->positioned) return POFD_OK;
+  if (pofd_mlp_predict_device(s->mlp, s->e.pos, static_cast<int>(s->npar), s->sidx,
+                              static_cast<int>(s->nwalk), s->e.llsur, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens: surrogate prediction failed");
+  return POFD_OK;
+}
+
+// one half-step with the delayed-acceptance first stage
+int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int store) {
+  const uint32_t hw = s->nwalk / 2;
+  if (pofd_mlp_predict_device(s->mlp, s->e.prop, static_cast<int>(s->npar), s->sidx,
+                              static_cast<int>(hw), s->e.llsp, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: surrogate prediction failed");
+  if (pofd_ens_dev::launch_stage1(s->e, half, step, inv_temp, s->stream) ||
+      pofd_ens_dev::launch_compact(s->e, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: first stage launch failed");
+  int32_t npass = 0;
+  hipError_t he = hipMemcpyAsync(&npass, s->e.npass, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 s->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(npass)");
+  if (npass > 0) {
+    int rc = evaluate(s, s->e.cprop, static_cast<uint32_t>(npass), s->e.cll, s->e.cst);
+    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+    if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, s->stream))
+      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
+  }
+  s->n_exact += static_cast<uint64_t>(npass);
+  if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
+  return POFD_OK;
 }
 
 }  // namespace
@@ -205,7 +253,62 @@ int pofd_ens_set_positions(pofd_ens* s, const float* theta) {
   (void)hipMemset(s->e.nerr, 0, W * sizeof(uint32_t));
   s->nstored = 0;
   s->step = 0;
+  s->n_exact = s->n_proposed = 0;
   s->positioned = true;
+  return refresh_surrogate(s);
+}
+
+int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_index,
+                           uint32_t nidx, double gate, double holdout_rms, int* enabled) {
+  if (!s) return POFD_E_ARG;
+  if (enabled) *enabled = 0;
+  s->use_da = false;
+  s->mlp = mlp;
+  if (!mlp) return POFD_OK;
+  uint32_t nin = 0, hid = 0;
+  if (pofd_mlp_info(mlp, &nin, &hid) || !param_index || nidx != nin)
+    return efail(s, POFD_E_ARG, "pofd_ens_set_surrogate: input count differs from the network's");
+  for (uint32_t i = 0; i < nidx; ++i)
+    if (param_index[i] >= s->npar)
+      return efail(s, POFD_E_ARG, "pofd_ens_set_surrogate: parameter index out of range");
+    else if (s->e.state[param_index[i]] == pofd_ens_dev::kStateBonus)
+      return efail(s, POFD_E_ARG, "pofd_ens_set_surrogate: bonus parameters are not inputs");
+  // the held-out error gate: beyond it the plain exact step is kept
+  if (!(holdout_rms <= gate)) return POFD_OK;
+  const size_t W = s->nwalk, H = s->nwalk / 2, P = s->npar;
+  if (!s->dabuf[0]) {
+    const size_t sizes[8] = {W * sizeof(double), H * sizeof(double), H * sizeof(int32_t),
+                             H * sizeof(int32_t), 64, H * P * sizeof(float),
+                             H * sizeof(double), H * sizeof(int32_t)};
+    for (int i = 0; i < 8; ++i) {
+      hipError_t he = hipMalloc(&s->dabuf[i], sizes[i]);
+      if (he == hipSuccess) he = hipMemset(s->dabuf[i], 0, sizes[i]);
+      if (he != hipSuccess) return ehip(s, he, "hipMalloc(delayed acceptance)");
+    }
+    s->e.llsur = static_cast<double*>(s->dabuf[0]);
+    s->e.llsp = static_cast<double*>(s->dabuf[1]);
+    s->e.pass = static_cast<int32_t*>(s->dabuf[2]);
+    s->e.cidx = static_cast<int32_t*>(s->dabuf[3]);
+    s->e.npass = static_cast<int32_t*>(s->dabuf[4]);
+    s->e.cprop = static_cast<float*>(s->dabuf[5]);
+    s->e.cll = static_cast<double*>(s->dabuf[6]);
+    s->e.cst = static_cast<int32_t*>(s->dabuf[7]);
+  }
+  std::vector<int> idx(param_index, param_index + nidx);
+  if (s->sidx) (void)hipFree(s->sidx);
+  s->sidx = nullptr;
+  hipError_t he = hipMalloc(&s->sidx, nidx * sizeof(int));
+  if (he == hipSuccess) he = hipMemcpy(s->sidx, idx.data(), nidx * sizeof(int), hipMemcpyHostToDevice);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(surrogate inputs)");
+  s->use_da = true;
+  if (enabled) *enabled = 1;
+  return refresh_surrogate(s);
+}
+
+int pofd_ens_counts(const pofd_ens* s, uint64_t* n_proposed, uint64_t* n_exact) {
+  if (!s) return POFD_E_ARG;
+  if (n_proposed) *n_proposed = s->n_proposed;
+  if (n_exact) *n_exact = s->n_exact;
   return POFD_OK;
 }
 
@@ -247,6 +350,13 @@ int pofd_ens_run(pofd_ens* s, uint32_t nsteps, double temperature, uint32_t thin
     for (uint32_t half = 0; half < 2; ++half) {
       if (pofd_ens_dev::launch_propose(s->e, half, step, s->stream))
         return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
+      s->n_proposed += hw;
+      if (s->use_da) {
+        int rc = da_half(s, half, step, inv_temp, store);
+        if (rc) return rc;
+        continue;
+      }
+      s->n_exact += hw;
       int rc = evaluate(s, s->e.prop, hw, s->e.llp, s->e.stp);
       if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
       if (pofd_ens_dev::launch_accept(s->e, half, step, inv_temp, store, s->nstored,

@@ -32,6 +32,15 @@ struct EnsDev {
   uint32_t* nerr;    // nwalk          rejected for an invalid proposal or likelihood
   float* chain;      // capacity x nwalk x npar
   double* chainll;   // capacity x nwalk
+  // delayed acceptance (surrogate first stage)
+  double* llsur;     // nwalk          surrogate log likelihood of the walkers
+  double* llsp;      // nwalk/2        surrogate of the proposals
+  int32_t* pass;     // nwalk/2        passed the first stage
+  int32_t* cidx;     // nwalk/2        compact list of the passing proposals
+  int32_t* npass;    // 1
+  float* cprop;      // nwalk/2 x npar compact proposals
+  double* cll;       // nwalk/2        their exact log likelihoods
+  int32_t* cst;      // nwalk/2
 };
 
 int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st);
@@ -40,5 +49,13 @@ int launch_gauss(const EnsDev& e, const float* x, int n, double* ll, int32_t* st
 int launch_accept(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp, int store,
                   uint32_t slot, hipStream_t st);
 int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st);
+// delayed acceptance: first stage on the surrogate, compaction of the
+// passing proposals, second stage on their exact likelihoods, chain store
+int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
+                  hipStream_t st);
+int launch_compact(const EnsDev& e, hipStream_t st);
+int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
+                  hipStream_t st);
+int launch_store_half(const EnsDev& e, uint32_t half, uint32_t slot, hipStream_t st);
 
 }  // namespace pofd_ens_dev

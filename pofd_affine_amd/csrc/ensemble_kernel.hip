@@ -180,6 +180,87 @@ __global__ __launch_bounds__(64) void accept_kernel(const EnsDev e, const uint32
   }
 }
 
+// ---- delayed acceptance (Christen and Fox, J. Comput. Graph. Stat. 14,
+// 795 (2005)): the stretch move's acceptance split into a first stage on the
+// surrogate L~ and a second, exact stage that corrects it, so the chain
+// keeps the exact posterior while proposals the surrogate rejects never
+// reach the exact likelihood.
+//   stage 1: a1 = z^d exp((L~new - L~old)/T)
+//   stage 2: a2 = exp(((Lnew - Lold) - (L~new - L~old))/T)
+__global__ __launch_bounds__(64) void stage1_kernel(const EnsDev e, const uint32_t half,
+                                                    const uint32_t step, const double inv_temp) {
+  const int hw = e.nwalk / 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hw) return;
+  const int k = static_cast<int>(half) * hw + t;
+  int pass = 0;
+  if (!e.pvalid[t]) {
+    e.nerr[k] += 1;
+  } else {
+    const double ls = e.llsp[t];
+    if (!(isnan(ls) || isinf(ls))) {
+      const double a1 = exp(static_cast<double>(e.dexp) * log(static_cast<double>(e.z[t])) +
+                            inv_temp * (ls - e.llsur[k]));
+      pass = (a1 >= 1.0) || (draws(e, step, half, k, 101).u0 < a1);
+    }
+  }
+  e.pass[t] = pass;
+}
+
+// serial compaction of the (at most a few thousand) first-stage survivors
+__global__ void compact_kernel(const EnsDev e) {
+  const int hw = e.nwalk / 2;
+  int n = 0;
+  for (int t = 0; t < hw; ++t)
+    if (e.pass[t]) e.cidx[n++] = t;
+  *e.npass = n;
+}
+
+__global__ __launch_bounds__(64) void gather_pass_kernel(const EnsDev e) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= *e.npass) return;
+  const int t = e.cidx[c];
+  const float* src = e.prop + static_cast<size_t>(t) * e.npar;
+  float* dst = e.cprop + static_cast<size_t>(c) * e.npar;
+  for (int i = 0; i < e.npar; ++i) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(64) void stage2_kernel(const EnsDev e, const uint32_t half,
+                                                    const uint32_t step, const double inv_temp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= *e.npass) return;
+  const int hw = e.nwalk / 2;
+  const int t = e.cidx[c];
+  const int k = static_cast<int>(half) * hw + t;
+  const int st = e.cst[c];
+  const double lnew = e.cll[c];
+  if ((st != 0 && st != kBadLike) || isnan(lnew) || isinf(lnew)) {
+    e.nerr[k] += 1;
+    return;
+  }
+  const double a2 = exp(inv_temp * ((lnew - e.ll[k]) - (e.llsp[t] - e.llsur[k])));
+  const bool accept = (a2 >= 1.0) || (draws(e, step, half, k, 102).u0 < a2);
+  if (!accept) return;
+  float* xo = e.pos + static_cast<size_t>(k) * e.npar;
+  const float* y = e.prop + static_cast<size_t>(t) * e.npar;
+  for (int i = 0; i < e.npar; ++i) xo[i] = y[i];
+  e.ll[k] = lnew;
+  e.llsur[k] = e.llsp[t];
+  e.naccept[k] += 1;
+}
+
+__global__ __launch_bounds__(64) void store_half_kernel(const EnsDev e, const uint32_t half,
+                                                        const uint32_t slot) {
+  const int hw = e.nwalk / 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hw) return;
+  const int k = static_cast<int>(half) * hw + t;
+  const float* xo = e.pos + static_cast<size_t>(k) * e.npar;
+  float* c = e.chain + (static_cast<size_t>(slot) * e.nwalk + k) * e.npar;
+  for (int i = 0; i < e.npar; ++i) c[i] = xo[i];
+  e.chainll[static_cast<size_t>(slot) * e.nwalk + k] = e.ll[k];
+}
+
 // z draws for tests of generateZ's distribution
 __global__ __launch_bounds__(256) void zsample_kernel(const EnsDev e, const uint32_t n,
                                                       double* out) {
@@ -214,6 +295,35 @@ int launch_accept(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp
 
 int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st) {
   hipLaunchKernelGGL(zsample_kernel, dim3((n + 255) / 256), dim3(256), 0, st, e, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
+                  hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(stage1_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
+                     inv_temp);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_compact(const EnsDev& e, hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1), 0, st, e);
+  hipLaunchKernelGGL(gather_pass_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
+                  hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(stage2_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
+                     inv_temp);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_store_half(const EnsDev& e, uint32_t half, uint32_t slot, hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(store_half_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, slot);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

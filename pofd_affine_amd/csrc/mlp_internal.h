@@ -1,0 +1,41 @@
+// Declarations shared by the surrogate network's host code and kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace pofd_mlp_dev {
+
+typedef uint16_t bf16;  // bfloat16 bit pattern
+
+// C[m][n] = sum_k A(m,k) B(k,n) (+ bias[n]) with FP32 accumulation.
+// A(m,k) = ta ? A[k*lda + m] : A[m*lda + k]; B(k,n) likewise with tb.
+// Element types: bf16 (0) or float (1) for A and B; C is float.
+int gemm(int atype, const void* A, int lda, int ta, int btype, const void* B, int ldb, int tb,
+         float* C, int ldc, const float* bias, int M, int N, int K, hipStream_t st);
+
+// out_bf16 = gelu(z) (z already includes the bias)
+int gelu_fwd(const float* z, bf16* out, size_t n, hipStream_t st);
+// dz = dA * gelu'(z), in place on dA
+int gelu_bwd(const float* z, float* dA, size_t n, hipStream_t st);
+// float -> bf16 (round to nearest even)
+int to_bf16(const float* in, bf16* out, size_t n, hipStream_t st);
+// standardised inputs: x[r][i] = (src[r*stride + idx[i]] - mean[i]) / std[i], as bf16
+int gather_inputs(const float* src, int stride, const int* idx, const float* mean,
+                  const float* istd, int nin, int rows, bf16* x, hipStream_t st);
+// column sums: g[n] = sum_m dz[m][n]
+int colsum(const float* dz, int M, int N, float* g, hipStream_t st);
+// output-layer loss gradient: d[m] = 2 (pred[m] - y[m]) / M, and the
+// per-row squared errors into se[m]
+int mse_grad(const float* pred, const float* y, int M, float* d, float* se, hipStream_t st);
+// Adam step on master weights w with gradient g (Kingma and Ba, 2015), and
+// the bf16 copy
+int adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float lr, float b1,
+         float b2, float eps, float c1, float c2, hipStream_t st);
+
+// out[i] = pred[i] * ys + ym (double)
+int unstandardize(const float* pred, int n, double ym, double ys, double* out, hipStream_t st);
+
+}  // namespace pofd_mlp_dev

@@ -1,0 +1,217 @@
+// Kernels of the log-likelihood surrogate (a small MLP): a tiled GEMM with
+// bfloat16 or float operands and float accumulation, GELU forward and
+// backward, the squared-error gradient and an Adam step.  Shapes are small
+// (batch x 256 x 256), so one 64 x 64 output tile per 256-thread workgroup,
+// staged through LDS in K-slices of 16, is ample.
+#include <hip/hip_runtime.h>
+
+#include "mlp_internal.h"
+
+namespace pofd_mlp_dev {
+
+namespace {
+
+__device__ __forceinline__ float bf2f(bf16 v) {
+  return __uint_as_float(static_cast<uint32_t>(v) << 16);
+}
+
+__device__ __forceinline__ bf16 f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<bf16>((u >> 16) | 0x40);  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return static_cast<bf16>(u >> 16);
+}
+
+template <int T>
+__device__ __forceinline__ float ld(const void* p, size_t i) {
+  if (T == 0) return bf2f(static_cast<const bf16*>(p)[i]);
+  return static_cast<const float*>(p)[i];
+}
+
+constexpr int kTile = 64, kKs = 16;
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void gemm_kernel(const void* A, int lda, int ta, const void* B,
+                                                   int ldb, int tb, float* C, int ldc,
+                                                   const float* bias, int M, int N, int K) {
+  __shared__ float As[kKs][kTile + 1];
+  __shared__ float Bs[kKs][kTile + 1];
+  const int tid = threadIdx.x;
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  const int tm = (tid >> 4) * 4, tn = (tid & 15) * 4;  // 4 x 4 outputs per thread
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += kKs) {
+    for (int e = tid; e < kKs * kTile; e += 256) {
+      const int kk = e / kTile, mm = e % kTile;
+      const int m = m0 + mm, k = k0 + kk;
+      float va = 0.0f;
+      if (m < M && k < K)
+        va = ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k);
+      As[kk][mm] = va;
+      const int n = n0 + mm;
+      float vb = 0.0f;
+      if (n < N && k < K)
+        vb = ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n);
+      Bs[kk][mm] = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kKs; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = As[kk][tm + i];
+        b[i] = Bs[kk][tn + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + tm + i;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + tn + j;
+      if (n < N) C[static_cast<size_t>(m) * ldc + n] = acc[i][j] + (bias ? bias[n] : 0.0f);
+    }
+  }
+}
+
+__global__ void gelu_fwd_kernel(const float* z, bf16* out, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = z[i];
+  out[i] = f2bf(0.5f * x * (1.0f + erff(x * 0.70710678118654752f)));
+}
+
+__global__ void gelu_bwd_kernel(const float* z, float* dA, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = z[i];
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+  dA[i] *= cdf + x * pdf;
+}
+
+__global__ void to_bf16_kernel(const float* in, bf16* out, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = f2bf(in[i]);
+}
+
+__global__ void gather_kernel(const float* src, int stride, const int* idx, const float* mean,
+                              const float* istd, int nin, int rows, bf16* x) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * nin) return;
+  const int r = t / nin, i = t % nin;
+  x[t] = f2bf((src[static_cast<size_t>(r) * stride + idx[i]] - mean[i]) * istd[i]);
+}
+
+__global__ void colsum_kernel(const float* dz, int M, int N, float* g) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int m = 0; m < M; ++m) s += dz[static_cast<size_t>(m) * N + n];
+  g[n] = s;
+}
+
+__global__ void mse_grad_kernel(const float* pred, const float* y, int M, float* d, float* se) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const float r = pred[m] - y[m];
+  d[m] = 2.0f * r / static_cast<float>(M);
+  se[m] = r * r;
+}
+
+__global__ void adam_kernel(float* w, const float* g, float* m, float* v, bf16* wb, size_t n,
+                            float lr, float b1, float b2, float eps, float c1, float c2) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  w[i] = wi;
+  wb[i] = f2bf(wi);
+}
+
+__global__ void unstd_kernel(const float* pred, int n, double ym, double ys, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<double>(pred[i]) * ys + ym;
+}
+
+inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
+inline unsigned nb(size_t n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+}  // namespace
+
+int gemm(int atype, const void* A, int lda, int ta, int btype, const void* B, int ldb, int tb,
+         float* C, int ldc, const float* bias, int M, int N, int K, hipStream_t st) {
+  const dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile);
+  if (atype == 0 && btype == 0)
+    hipLaunchKernelGGL((gemm_kernel<0, 0>), grid, dim3(256), 0, st, A, lda, ta, B, ldb, tb, C,
+                       ldc, bias, M, N, K);
+  else if (atype == 1 && btype == 0)
+    hipLaunchKernelGGL((gemm_kernel<1, 0>), grid, dim3(256), 0, st, A, lda, ta, B, ldb, tb, C,
+                       ldc, bias, M, N, K);
+  else if (atype == 0 && btype == 1)
+    hipLaunchKernelGGL((gemm_kernel<0, 1>), grid, dim3(256), 0, st, A, lda, ta, B, ldb, tb, C,
+                       ldc, bias, M, N, K);
+  else
+    hipLaunchKernelGGL((gemm_kernel<1, 1>), grid, dim3(256), 0, st, A, lda, ta, B, ldb, tb, C,
+                       ldc, bias, M, N, K);
+  return ok();
+}
+
+int gelu_fwd(const float* z, bf16* out, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(gelu_fwd_kernel, dim3(nb(n, 256)), dim3(256), 0, st, z, out, n);
+  return ok();
+}
+
+int gelu_bwd(const float* z, float* dA, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(gelu_bwd_kernel, dim3(nb(n, 256)), dim3(256), 0, st, z, dA, n);
+  return ok();
+}
+
+int to_bf16(const float* in, bf16* out, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(nb(n, 256)), dim3(256), 0, st, in, out, n);
+  return ok();
+}
+
+int gather_inputs(const float* src, int stride, const int* idx, const float* mean,
+                  const float* istd, int nin, int rows, bf16* x, hipStream_t st) {
+  const size_t n = static_cast<size_t>(rows) * nin;
+  hipLaunchKernelGGL(gather_kernel, dim3(nb(n, 256)), dim3(256), 0, st, src, stride, idx, mean,
+                     istd, nin, rows, x);
+  return ok();
+}
+
+int colsum(const float* dz, int M, int N, float* g, hipStream_t st) {
+  hipLaunchKernelGGL(colsum_kernel, dim3(nb(N, 256)), dim3(256), 0, st, dz, M, N, g);
+  return ok();
+}
+
+int mse_grad(const float* pred, const float* y, int M, float* d, float* se, hipStream_t st) {
+  hipLaunchKernelGGL(mse_grad_kernel, dim3(nb(M, 256)), dim3(256), 0, st, pred, y, M, d, se);
+  return ok();
+}
+
+int adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float lr, float b1,
+         float b2, float eps, float c1, float c2, hipStream_t st) {
+  hipLaunchKernelGGL(adam_kernel, dim3(nb(n, 256)), dim3(256), 0, st, w, g, m, v, wb, n, lr, b1,
+                     b2, eps, c1, c2);
+  return ok();
+}
+
+int unstandardize(const float* pred, int n, double ym, double ys, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(unstd_kernel, dim3(nb(n, 256)), dim3(256), 0, st, pred, n, ym, ys, out);
+  return ok();
+}
+
+}  // namespace pofd_mlp_dev
