@@ -53,6 +53,15 @@ def main():
     ap.add_argument("--cpu-evals-2d", type=int, default=16)
     ap.add_argument("--only-2d", action="store_true",
                     help="run the 2D leg alone (for profiling it)")
+    ap.add_argument("--da-steps", type=int, default=2000,
+                    help="steps of each of the plain and the delayed-"
+                         "acceptance chains compared (0 skips that leg)")
+    ap.add_argument("--da-walkers", type=int, default=512)
+    ap.add_argument("--da-burn", type=int, default=300)
+    ap.add_argument("--da-train-steps", type=int, default=100)
+    ap.add_argument("--da-epochs", type=int, default=60)
+    ap.add_argument("--only-da", action="store_true",
+                    help="run the delayed-acceptance leg only")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (calibrates "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -85,6 +94,9 @@ def main():
 
     W, N = args.walkers, args.nflux
     inp = workload.make_inputs(npix_side=args.side, seed=42)
+    if args.only_da:
+        print(json.dumps({"da": bench_da(args, inp)}))
+        return
     th = synth.walker_thetas(W, seed=1234)
     stride = th.shape[1]
 
@@ -213,6 +225,8 @@ def main():
         out["copy_calib_bytes"] = (args.copy_calib // 16) * 16
     if args.steps_2d > 0:
         out["twod"] = bench_2d(args, hiprt)
+    if args.da_steps > 0:
+        out["da"] = bench_da(args, inp)
     if not args.no_cpu and args.cpu_evals > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -234,6 +248,112 @@ def main():
         out["parity_max_abs_dlogL"] = float(
             np.max(np.abs(ref[ok] - b["ll"][:ne][ok]))) if ok.any() else None
     print(json.dumps(out))
+
+
+def bench_da(args, inp):
+    """The delayed-acceptance leg: the same binned 1D likelihood (nflux
+    2^17), a 512-walker stretch move.  A plain burn-in, then the (theta,
+    exact logL) pairs of the next 100 stored steps (51,200 pairs) train an
+    8 -> 3x256 -> 1 surrogate on the GPU; from that one state a plain chain
+    and a delayed-acceptance chain of the same length run, and are compared
+    on effective samples per second (walkers x steps / tau of the
+    walker-averaged series, per knot) and on the knot posterior medians."""
+    from pofd_affine_amd import _lib, synth, workload
+    N, W = args.nflux, args.da_walkers
+    ctx = workload.build_context(inp, N, args.ninterp, binned=True)
+    th0 = synth.walker_thetas(W, seed=777, scatter=0.01)
+    npar, nk = th0.shape[1], 8
+    state = np.zeros(npar, dtype=np.uint8)
+    state[nk] = 1          # sigma multiplier fixed
+    state[nk + 1:] = 2     # bonus: the counts integrals
+    knots = list(range(nk))
+
+    def make(pos, seed):
+        e = _lib.Ensemble(ctx, _lib.TARGET_1D, W, npar, seed=seed)
+        e.set_param_state(state)
+        e.set_exponent_mode(True)
+        e.set_positions(pos)
+        return e
+
+    ens = make(th0, 31)
+    t0 = time.perf_counter()
+    ens.run(args.da_burn, thin=0)
+    ens.run(args.da_train_steps, thin=1)
+    t_burn = time.perf_counter() - t0
+    ch, cll = ens.chain()
+    x = ch[:, :, :nk].reshape(-1, nk)
+    y = cll.reshape(-1)
+    start, _, _, _ = ens.state()
+    del ens
+    net = _lib.Mlp(nk, 256, seed=5)
+    t0 = time.perf_counter()
+    tr, ho = net.train(x, y, epochs=args.da_epochs, batch=512, lr=1e-3,
+                       holdout_frac=0.1, seed=6)
+    t_train = time.perf_counter() - t0
+
+    def chain(use_da):
+        e = make(start, 41)
+        on = e.set_surrogate(net, knots) if use_da else False
+        e.run(5, thin=0)  # warm the launch path
+        n0 = e.counts()
+        e.clear_chain()
+        t0 = time.perf_counter()
+        e.run(args.da_steps, thin=1)
+        wall = time.perf_counter() - t0
+        n1 = e.counts()
+        c, _ = e.chain()
+        taus = []
+        for i in knots:
+            try:
+                taus.append(e.acor(i)[0])
+            except _lib.PofdError:
+                taus.append(float("nan"))
+        _, _, nacc, nerr = e.state()
+        return dict(on=on, wall=wall, chain=c[:, :, :nk],
+                    taus=np.array(taus),
+                    nprop=n1[0] - n0[0], nexact=n1[1] - n0[1],
+                    acc=float(nacc.sum()) / (W * (args.da_steps + 5)))
+
+    plain = chain(False)
+    da = chain(True)
+    S = args.da_steps
+
+    def ess_per_s(r):
+        tau = np.nanmean(r["taus"])
+        return W * S / tau / r["wall"], tau
+
+    ep, tp = ess_per_s(plain)
+    ed, td = ess_per_s(da)
+    flat_p = plain["chain"].reshape(-1, nk).astype(np.float64)
+    flat_d = da["chain"].reshape(-1, nk).astype(np.float64)
+    sig = flat_p.std(axis=0)
+    dmed = (np.median(flat_d, axis=0) - np.median(flat_p, axis=0)) / sig
+    return {
+        "workload": "1D P(D) stretch move, nflux=%d, %d walkers, %d steps "
+                    "per chain, binned %dx%d map; surrogate MLP %d -> "
+                    "3x256 GELU -> 1 (bf16, fp32 accumulate) trained on "
+                    "%d (theta, logL) pairs from %d stored steps after %d "
+                    "burn-in steps" % (N, W, S, args.side, args.side, nk,
+                                       x.shape[0], args.da_train_steps,
+                                       args.da_burn),
+        "surrogate": {"train_rms": tr, "holdout_rms": ho, "gate": 0.5,
+                      "enabled": bool(da["on"]), "train_s": t_train,
+                      "epochs": args.da_epochs},
+        "burn_and_collect_s": t_burn,
+        "plain": {"wall_s": plain["wall"], "ms_per_step":
+                  1e3 * plain["wall"] / S, "tau_mean": tp,
+                  "tau": plain["taus"].tolist(), "ess_per_s": ep,
+                  "exact_evals": plain["nexact"],
+                  "acceptance": plain["acc"]},
+        "da": {"wall_s": da["wall"], "ms_per_step": 1e3 * da["wall"] / S,
+               "tau_mean": td, "tau": da["taus"].tolist(), "ess_per_s": ed,
+               "exact_evals": da["nexact"], "proposals": da["nprop"],
+               "acceptance": da["acc"]},
+        "ess_per_s_ratio": ed / ep,
+        "step_time_ratio": plain["wall"] / da["wall"],
+        "median_shift_sigma": dmed.tolist(),
+        "max_abs_median_shift_sigma": float(np.max(np.abs(dmed))),
+    }
 
 
 def bench_2d(args, hiprt):

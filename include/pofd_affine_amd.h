@@ -165,6 +165,15 @@ int pofd1d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
                                 uint32_t n_walkers, uint32_t theta_stride,
                                 double *d_loglike_out, int32_t *d_status_out,
                                 void *stream);
+/* Tuning, no reference counterpart: workgroups per walker of the forward
+   and inverse transforms.  0 (the default) chooses by batch size,
+   min(4, 256 / n_walkers) and at least 1, so that small batches (such as
+   the survivors of a delayed-acceptance first stage) still fill the GPU's
+   256 compute units; 1 to 4 forces it (reduced while walkers x parts
+   exceeds 256).  With more than one part the partial sums
+   are combined in a fixed order, so results depend on the part count only
+   in the last bits. */
+int pofd1d_set_parts(pofd_ctx *ctx, int parts);
 
 /* ---- parity / debug accessors for the most recent batch ---- */
 /* The P(D) as PDFactory::getPD(sigma, pd, setLog) leaves PD::pd_ (rotated,

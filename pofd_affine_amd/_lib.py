@@ -54,6 +54,7 @@ _SIGS = {
                                    C.c_uint32, D, D, D, D,
                                    C.POINTER(C.c_int)]),
     "pofd1d_prepare": (C.c_int, [VP]),
+    "pofd1d_set_parts": (C.c_int, [VP, C.c_int]),
     "pofd1d_set_rrange": (C.c_int, [VP, C.c_int, D, D]),
     "pofd1d_beamset_info": (C.c_int, [VP, C.c_int, DP]),
     "pofd1d_loglike_batch": (C.c_int, [VP, FP, C.c_uint32, C.c_uint32, DP,
@@ -264,6 +265,11 @@ class Context:
     def set_spectral_cut(self, value):
         self._check(lib().pofd_set_option(self.ptr, POFD_OPT_SPECTRAL_CUT,
                                           float(value)), "pofd_set_option")
+
+    def set_parts(self, parts):
+        """Workgroups per walker of the 1D transforms (0: by batch size)."""
+        self._check(lib().pofd1d_set_parts(self.ptr, int(parts)),
+                    "pofd1d_set_parts")
 
     def set_stage_timing(self, on):
         self._check(lib().pofd_set_option(self.ptr, POFD_OPT_STAGE_TIMING,

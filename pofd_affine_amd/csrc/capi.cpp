@@ -79,6 +79,7 @@ struct pofd_ctx {
   double exp_conf = 0.0;
   std::vector<std::unique_ptr<Beamset>> sets;
   uint32_t last_nwalk = 0;
+  int force_parts = 0;  // workgroups per walker: 0 = by batch size
   std::unique_ptr<DevBuf> dtheta, dll, dst;
   size_t cap_theta = 0, cap_out = 0;
   // stage timing: one set of POFD_NSTAGES+1 events per beam-set launch
@@ -522,8 +523,16 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   const size_t n_state = r8(pofd::kState) * nwalk;
   const size_t n_upart = r8(pofd::kPixBlocks) * nf * nwalk;
   const size_t n_bonus = r8(4 * static_cast<size_t>(nwalk));
+  // (walker, part) slots for batches split over several workgroups per
+  // walker: at most 4 parts, and walkers x parts <= 256
+  const uint32_t nslots = std::min<uint32_t>(pofd::kMaxSlots, 4 * nwalk);
+  w.zpart_stride = r8(2 * static_cast<size_t>(d.M)) / 2;
+  const size_t n_zpart = 2 * w.zpart_stride * nslots;
+  const size_t n_mpart = r8(2 * static_cast<size_t>(nslots));
+  const size_t n_ipart = r8(static_cast<size_t>(nslots) * nf * pofd::kIPart);
   const size_t total = n_coef + n_ynode + n_z + n_spec + n_pofd + n_blk +
-                       n_dbg + n_state + n_upart + n_bonus;
+                       n_dbg + n_state + n_upart + n_bonus + n_zpart + n_mpart +
+                       n_ipart;
   s.wbuf.reset();
   int rc = dev_alloc(c, s.wbuf, total * sizeof(double));
   if (rc) return rc;
@@ -555,8 +564,24 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   w.upart = base;
   base += n_upart;
   w.bonus = base;
+  base += n_bonus;
+  w.zpart = reinterpret_cast<double2*>(base);
+  base += n_zpart;
+  w.mpart = base;
+  base += n_mpart;
+  w.ipart = base;
+  w.nslots = nslots;
   s.wcap = nwalk;
   return POFD_OK;
+}
+
+// workgroups per walker of the transforms: a batch that leaves most of the
+// 256 compute units idle gives each walker 2 or 4
+int choose_parts(const pofd_ctx* c, const WorkDev& w, uint32_t nwalk) {
+  int parts = c->force_parts;
+  if (parts == 0) parts = static_cast<int>(std::min<uint32_t>(4, std::max<uint32_t>(1, 256 / nwalk)));
+  while (parts > 1 && nwalk * static_cast<uint32_t>(parts) > w.nslots) --parts;
+  return parts;
 }
 
 int check_ready(pofd_ctx* c) {
@@ -749,6 +774,14 @@ extern "C" int pofd1d_beamset_info(const pofd_ctx* c, int b, double* out) {
   return POFD_OK;
 }
 
+extern "C" int pofd1d_set_parts(pofd_ctx* c, int parts) {
+  if (!c) return POFD_E_ARG;
+  if (parts < 0 || parts > 4)
+    return fail(c, POFD_E_ARG, "pofd1d_set_parts: parts must be 0 to 4");
+  c->force_parts = parts;
+  return POFD_OK;
+}
+
 extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
                                            uint32_t nwalk, uint32_t stride,
                                            double* d_ll, int32_t* d_st,
@@ -769,7 +802,7 @@ extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
     s.dev.bonus = (b == 0) ? 1 : 0;  // the counts integrals once per batch
     rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta,
                                 nwalk, stride, d_ll, d_st, b > 0 ? 1 : 0,
-                                stream, ev);
+                                choose_parts(c, s.wk, nwalk), stream, ev);
     if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: launch failed: " +
                                    std::string(hipGetErrorString(hipGetLastError())));
   }

@@ -187,33 +187,57 @@ __global__ __launch_bounds__(64) void accept_kernel(const EnsDev e, const uint32
 // reach the exact likelihood.
 //   stage 1: a1 = z^d exp((L~new - L~old)/T)
 //   stage 2: a2 = exp(((Lnew - Lold) - (L~new - L~old))/T)
-__global__ __launch_bounds__(64) void stage1_kernel(const EnsDev e, const uint32_t half,
-                                                    const uint32_t step, const double inv_temp) {
-  const int hw = e.nwalk / 2;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= hw) return;
-  const int k = static_cast<int>(half) * hw + t;
-  int pass = 0;
-  if (!e.pvalid[t]) {
-    e.nerr[k] += 1;
-  } else {
-    const double ls = e.llsp[t];
-    if (!(isnan(ls) || isinf(ls))) {
-      const double a1 = exp(static_cast<double>(e.dexp) * log(static_cast<double>(e.z[t])) +
-                            inv_temp * (ls - e.llsur[k]));
-      pass = (a1 >= 1.0) || (draws(e, step, half, k, 101).u0 < a1);
-    }
-  }
-  e.pass[t] = pass;
-}
+// First stage and compaction in one workgroup: pass/fail per proposal,
+// then an order-preserving scan (wave ballots, wave offsets in LDS) that
+// lists the survivors in cidx and counts them in npass.
+constexpr int kCompactThreads = 1024;
 
-// serial compaction of the (at most a few thousand) first-stage survivors
-__global__ void compact_kernel(const EnsDev e) {
+__global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
+                                                                 const uint32_t half,
+                                                                 const uint32_t step,
+                                                                 const double inv_temp) {
+  __shared__ int woff[kCompactThreads / 64 + 1];
   const int hw = e.nwalk / 2;
-  int n = 0;
-  for (int t = 0; t < hw; ++t)
-    if (e.pass[t]) e.cidx[n++] = t;
-  *e.npass = n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int base = 0;
+  for (int t0 = 0; t0 < hw; t0 += kCompactThreads) {
+    const int t = t0 + tid;
+    int pass = 0;
+    if (t < hw) {
+      const int k = static_cast<int>(half) * hw + t;
+      if (!e.pvalid[t]) {
+        e.nerr[k] += 1;
+      } else {
+        const double ls = e.llsp[t];
+        if (!(isnan(ls) || isinf(ls))) {
+          const double a1 = exp(static_cast<double>(e.dexp) * log(static_cast<double>(e.z[t])) +
+                                inv_temp * (ls - e.llsur[k]));
+          pass = (a1 >= 1.0) || (draws(e, step, half, k, 101).u0 < a1);
+        }
+      }
+      e.pass[t] = pass;
+    }
+    const unsigned long long bal = __ballot(pass);
+    if (lane == 0) woff[wave] = __popcll(bal);
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int w = 0; w < kCompactThreads / 64; ++w) {
+        const int c = woff[w];
+        woff[w] = acc;
+        acc += c;
+      }
+      woff[kCompactThreads / 64] = acc;
+    }
+    __syncthreads();
+    if (pass) {
+      const unsigned long long below = (lane == 0) ? 0ull : (bal & (~0ull >> (64 - lane)));
+      e.cidx[base + woff[wave] + __popcll(below)] = t;
+    }
+    base += woff[kCompactThreads / 64];
+    __syncthreads();
+  }
+  if (tid == 0) *e.npass = base;
 }
 
 __global__ __launch_bounds__(64) void gather_pass_kernel(const EnsDev e) {
@@ -300,15 +324,13 @@ int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st) {
 
 int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
                   hipStream_t st) {
-  const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(stage1_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
+  hipLaunchKernelGGL(stage1_kernel, dim3(1), dim3(kCompactThreads), 0, st, e, half, step,
                      inv_temp);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_compact(const EnsDev& e, hipStream_t st) {
   const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1), 0, st, e);
   hipLaunchKernelGGL(gather_pass_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }

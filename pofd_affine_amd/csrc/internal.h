@@ -41,6 +41,8 @@ constexpr int kDbg = 32;         // debug scalars per (walker, field)
 constexpr int kState = 16;       // per-walker scalars handed between kernels
 constexpr int kPixBlocks = 64;   // workgroups per (walker, field), unbinned
 constexpr int kGL = 32;          // Gauss-Legendre points per knot interval
+constexpr int kIPart = 8;        // partial statistics per inverse part
+constexpr int kMaxSlots = 256;   // (walker, part) slots of the split transforms
 
 // One field (dataset) of a beam set, device view.
 struct FieldDev {
@@ -113,8 +115,14 @@ struct WorkDev {
   double* state;     // W * kState         : per-walker scalars
   double* upart;     // W * nfields * kPixBlocks : unbinned partial sums
   double* bonus;     // W * 4: int n dS, int S n dS, int S^2 n dS
+  // walkers split over several workgroups (batches of at most 128): one
+  // slot per (walker, part), nslots = min(256, 4 W), walkers x parts <= 256
+  double2* zpart;    // nslots * M : partial kept rows of the forward transform
+  double* mpart;     // nslots * 2 : partial moments (sum S R, sum S^2 R)
+  double* ipart;     // nslots * nfields * kIPart : partial inverse statistics
+  uint32_t nslots;
   size_t coef_stride, ynode_stride, zrows_stride, spec_stride, pofd_stride,
-      blk_stride;
+      blk_stride, zpart_stride;
 };
 
 }  // namespace pofd
@@ -129,7 +137,7 @@ namespace pofd {
 int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
                      const float* d_theta, uint32_t nwalk, uint32_t stride,
                      double* d_loglike, int32_t* d_status, int accumulate,
-                     void* stream, hipEvent_t* ev);
+                     int parts, void* stream, hipEvent_t* ev);
 int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
                           int is_log, int already_log, double* d_out,
                           void* stream);

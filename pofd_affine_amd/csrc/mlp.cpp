@@ -44,13 +44,14 @@ struct pofd_mlp {
   float* dA = nullptr;             // cap x H
   float* dA2 = nullptr;            // cap x H
   float* raw = nullptr;            // cap x nin raw inputs (host uploads)
+  double* dpred = nullptr;         // cap predictions (log-likelihood units)
   hipStream_t stream = nullptr;
   long adam_t = 0;
   std::string err;
   ~pofd_mlp() {
     if (stream) (void)hipStreamSynchronize(stream);
     void* ps[] = {w,   wb,  g, m, v, xmean, xistd, idx, X, Z[0], Z[1], Z[2], A[0], A[1], A[2],
-                  out, y,   dout, se, dA, dA2, raw};
+                  out, y,   dout, se, dA, dA2, raw, dpred};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -69,14 +70,14 @@ int dalloc(T** p, size_t n) {
 int ensure_cap(pofd_mlp* s, int n) {
   if (n <= s->cap) return 0;
   void* ps[] = {s->X, s->Z[0], s->Z[1], s->Z[2], s->A[0], s->A[1], s->A[2], s->out,
-                s->y, s->dout, s->se,   s->dA,   s->dA2,  s->raw};
+                s->y, s->dout, s->se,   s->dA,   s->dA2,  s->raw, s->dpred};
   (void)hipStreamSynchronize(s->stream);
   for (void* p : ps)
     if (p) (void)hipFree(p);
   const size_t N = static_cast<size_t>(n), H = s->H;
   int rc = dalloc(&s->X, N * s->nin) | dalloc(&s->out, N) | dalloc(&s->y, N) |
            dalloc(&s->dout, N) | dalloc(&s->se, N) | dalloc(&s->dA, N * H) |
-           dalloc(&s->dA2, N * H) | dalloc(&s->raw, N * s->nin);
+           dalloc(&s->dA2, N * H) | dalloc(&s->raw, N * s->nin) | dalloc(&s->dpred, N);
   for (int l = 0; l < 3; ++l) rc |= dalloc(&s->Z[l], N * H) | dalloc(&s->A[l], N * H);
   if (rc) return 1;
   s->cap = n;
@@ -163,14 +164,13 @@ int mfail(pofd_mlp* s, int code, const std::string& msg) {
 // rows of src (stride floats per row, inputs at idx) into out, on stream st
 int pofd_mlp_predict_device(pofd_mlp* s, const float* src, int stride, const int* idx, int rows,
                             double* out, hipStream_t st) {
-  if (ensure_cap(s, rows)) return 1;
-  hipStream_t keep = s->stream;
-  s->stream = st;
-  int rc = md::gather_inputs(src, stride, idx, s->xmean, s->xistd, s->nin, rows, s->X, st);
-  rc |= forward(s, rows);
-  rc |= md::unstandardize(s->out, rows, s->ymean, s->ystd, out, st);
-  s->stream = keep;
-  return rc;
+  md::InferLayout lay;
+  for (int l = 0; l < 4; ++l) {
+    lay.w[l] = s->off[2 * l];
+    lay.b[l] = s->off[2 * l + 1];
+  }
+  return md::infer(src, stride, idx, s->xmean, s->xistd, s->nin, s->H, rows, s->wb, s->w, lay,
+                   s->ymean, s->ystd, out, st);
 }
 
 extern "C" {
@@ -178,7 +178,8 @@ extern "C" {
 int pofd_mlp_create(pofd_mlp** out, uint32_t nin, uint32_t hidden, uint64_t seed) {
   if (!out) return POFD_E_ARG;
   *out = nullptr;
-  if (nin == 0 || nin > 64 || hidden == 0 || hidden > 4096) return POFD_E_ARG;
+  // the fused inference kernel keeps two 16-row activation tiles in LDS
+  if (nin == 0 || nin > 64 || hidden == 0 || hidden > 512) return POFD_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return POFD_E_NOGPU;
   pofd_mlp* s = new pofd_mlp();
@@ -285,13 +286,15 @@ int pofd_mlp_set_standardization(pofd_mlp* s, const float* xmean, const float* x
 int pofd_mlp_predict(pofd_mlp* s, const float* x, uint32_t n, double* out) {
   if (!s || !x || !out) return POFD_E_ARG;
   if (n == 0) return POFD_OK;
-  std::vector<float> pred(n);
-  if (upload_inputs(s, x, static_cast<int>(n)) || forward(s, static_cast<int>(n)) ||
-      hipMemcpyAsync(pred.data(), s->out, n * sizeof(float), hipMemcpyDeviceToHost, s->stream) !=
+  if (ensure_cap(s, static_cast<int>(n)) ||
+      hipMemcpyAsync(s->raw, x, static_cast<size_t>(n) * s->nin * sizeof(float),
+                     hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+      pofd_mlp_predict_device(s, s->raw, s->nin, s->idx, static_cast<int>(n), s->dpred,
+                              s->stream) ||
+      hipMemcpyAsync(out, s->dpred, n * sizeof(double), hipMemcpyDeviceToHost, s->stream) !=
           hipSuccess ||
       hipStreamSynchronize(s->stream) != hipSuccess)
     return mfail(s, POFD_E_INTERNAL, "pofd_mlp_predict: device failure");
-  for (uint32_t i = 0; i < n; ++i) out[i] = static_cast<double>(pred[i]) * s->ystd + s->ymean;
   return POFD_OK;
 }
 

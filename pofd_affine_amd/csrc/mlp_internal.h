@@ -35,6 +35,16 @@ int mse_grad(const float* pred, const float* y, int M, float* d, float* se, hipS
 int adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float lr, float b1,
          float b2, float eps, float c1, float c2, hipStream_t st);
 
+// offsets (elements) of the weight and bias blocks in the parameter vector
+struct InferLayout {
+  size_t w[4], b[4];
+};
+// fused prediction of rows inputs gathered from src (stride floats per row,
+// inputs at idx): out[r] = net(standardised x_r) * ys + ym
+int infer(const float* src, int stride, const int* idx, const float* mean, const float* istd,
+          int nin, int H, int rows, const bf16* wb, const float* w, const InferLayout& lay,
+          double ym, double ys, double* out, hipStream_t st);
+
 // out[i] = pred[i] * ys + ym (double)
 int unstandardize(const float* pred, int n, double ym, double ys, double* out, hipStream_t st);
 

@@ -111,12 +111,18 @@ __global__ void gather_kernel(const float* src, int stride, const int* idx, cons
   x[t] = f2bf((src[static_cast<size_t>(r) * stride + idx[i]] - mean[i]) * istd[i]);
 }
 
-__global__ void colsum_kernel(const float* dz, int M, int N, float* g) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// column sums: a workgroup of 256 threads per 64 columns, 4 row phases
+// reduced through LDS in a fixed order
+__global__ __launch_bounds__(256) void colsum_kernel(const float* dz, int M, int N, float* g) {
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + c;
   float s = 0.0f;
-  for (int m = 0; m < M; ++m) s += dz[static_cast<size_t>(m) * N + n];
-  g[n] = s;
+  if (n < N)
+    for (int m = ph; m < M; m += 4) s += dz[static_cast<size_t>(m) * N + n];
+  part[ph][c] = s;
+  __syncthreads();
+  if (ph == 0 && n < N) g[n] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
 }
 
 __global__ void mse_grad_kernel(const float* pred, const float* y, int M, float* d, float* se) {
@@ -144,6 +150,91 @@ __global__ void adam_kernel(float* w, const float* g, float* m, float* v, bf16* 
 __global__ void unstd_kernel(const float* pred, int n, double ym, double ys, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = static_cast<double>(pred[i]) * ys + ym;
+}
+
+// ---- fused inference: gather + standardise, three GELU layers on the
+// matrix cores, output layer, de-standardisation, in one launch.  A
+// workgroup of four waves takes 16 rows; the activations stay in LDS as
+// bfloat16 and the weights stream from L2.  Each wave owns 16-column tiles
+// of the layer output (tiles w, w+4, ...) and accumulates them with
+// v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)..+3] and
+// B[4(l/16)..+3][l%16] and receives D[4(l/16)..+3][l%16].
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+constexpr int kInfRows = 16;
+
+__global__ __launch_bounds__(256) void infer_kernel(const float* src, int stride, const int* idx,
+                                                    const float* mean, const float* istd, int nin,
+                                                    int H, int rows, const bf16* wb, const float* w,
+                                                    InferLayout lay, double ym, double ys,
+                                                    double* out) {
+  extern __shared__ bf16 act[];  // two buffers of kInfRows x ld bf16
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = blockIdx.x * kInfRows;
+  const int Hp = (H + 15) & ~15;
+  const int Kin = (nin + 15) & ~15;
+  const int ld = (Hp > Kin ? Hp : Kin) + 4;  // row pitch (bf16), 8-byte aligned
+  bf16* bufA = act;
+  bf16* bufB = act + kInfRows * ld;
+  for (int e = tid; e < kInfRows * Kin; e += 256) {
+    const int i = e / Kin, c = e % Kin;
+    const int r = r0 + i;
+    float v = 0.0f;
+    if (c < nin && r < rows) v = (src[static_cast<size_t>(r) * stride + idx[c]] - mean[c]) * istd[c];
+    bufA[i * ld + c] = f2bf(v);
+  }
+  __syncthreads();
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  const int ntile = Hp / 16;
+  for (int l = 0; l < 3; ++l) {
+    const int K = (l == 0) ? nin : H;
+    const int Kp = (l == 0) ? Kin : Hp;
+    const bf16* W = wb + lay.w[l];
+    const float* b = w + lay.b[l];
+    for (int t = wave; t < ntile; t += 4) {
+      const int j = t * 16 + li;  // output column this lane feeds B for
+      float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int k0 = 0; k0 < Kp; k0 += 16) {
+        const int k = k0 + lk;
+        const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + k);
+        short4v bv;
+        if (j < H && k + 3 < K && (K & 3) == 0) {
+          bv = *reinterpret_cast<const short4v*>(W + static_cast<size_t>(j) * K + k);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            bv[q] = (j < H && k + q < K) ? static_cast<short>(W[static_cast<size_t>(j) * K + k + q])
+                                         : static_cast<short>(0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+      }
+      // D rows lk..lk+3, column j
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float v = 0.0f;
+        if (j < H) {
+          const float z = acc[q] + b[j];
+          v = 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+        }
+        bufB[(lk + q) * ld + t * 16 + li] = f2bf(v);
+      }
+    }
+    __syncthreads();
+    bf16* tmp = bufA;
+    bufA = bufB;
+    bufB = tmp;
+  }
+  // output layer: 16 threads per row
+  {
+    const int i = tid >> 4, s = tid & 15;
+    const bf16* W4 = wb + lay.w[3];
+    float acc = 0.0f;
+    for (int j = s; j < H; j += 16) acc += bf2f(bufA[i * ld + j]) * bf2f(W4[j]);
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 16);
+    const int r = r0 + i;
+    if (s == 0 && r < rows) out[r] = static_cast<double>(acc + w[lay.b[3]]) * ys + ym;
+  }
 }
 
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
@@ -193,7 +284,7 @@ int gather_inputs(const float* src, int stride, const int* idx, const float* mea
 }
 
 int colsum(const float* dz, int M, int N, float* g, hipStream_t st) {
-  hipLaunchKernelGGL(colsum_kernel, dim3(nb(N, 256)), dim3(256), 0, st, dz, M, N, g);
+  hipLaunchKernelGGL(colsum_kernel, dim3(nb(N, 64)), dim3(256), 0, st, dz, M, N, g);
   return ok();
 }
 
@@ -206,6 +297,18 @@ int adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float
          float b2, float eps, float c1, float c2, hipStream_t st) {
   hipLaunchKernelGGL(adam_kernel, dim3(nb(n, 256)), dim3(256), 0, st, w, g, m, v, wb, n, lr, b1,
                      b2, eps, c1, c2);
+  return ok();
+}
+
+int infer(const float* src, int stride, const int* idx, const float* mean, const float* istd,
+          int nin, int H, int rows, const bf16* wb, const float* w, const InferLayout& lay,
+          double ym, double ys, double* out, hipStream_t st) {
+  if (rows <= 0) return 0;
+  const int Hp = (H + 15) & ~15, Kin = (nin + 15) & ~15;
+  const int ld = (Hp > Kin ? Hp : Kin) + 4;
+  const size_t lds = 2 * kInfRows * ld * sizeof(bf16);
+  hipLaunchKernelGGL(infer_kernel, dim3((rows + kInfRows - 1) / kInfRows), dim3(256), lds, st,
+                     src, stride, idx, mean, istd, nin, H, rows, wb, w, lay, ym, ys, out);
   return ok();
 }
 

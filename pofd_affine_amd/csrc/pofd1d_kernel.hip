@@ -505,31 +505,47 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
 // forward: R fill fused into the row-pruned forward transform, then the
 // spectrum exp(R^ - R^0) and, per dataset, the noisy rows of the inverse
 // ===========================================================================
+__device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int walker,
+                             XformSmem& sm, const double mn, const double var,
+                             unsigned long long tmark);
+
+// The forward transform of one walker is split over `parts` workgroups
+// (1 to 4) by ranges of the 16-column chunks: with one part the
+// workgroup finishes the spectrum itself; with more, each writes its partial
+// kept rows and partial moments and spectrum1d_kernel sums them in part
+// order and finishes.
 __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs,
-                                                              const WorkDev wk) {
+                                                              const WorkDev wk,
+                                                              const int parts) {
   __shared__ XformSmem sm;
   const int tid = threadIdx.x;
-  const int walker = blockIdx.x;
+  const int walker = blockIdx.x / parts;
+  const int part = blockIdx.x - walker * parts;
   double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
   unsigned long long tmark = wall_clock64();
-  const int M = bs.M, M2 = bs.M2;
+  const int M2 = bs.M2;
   const double dflux = bs.dflux;
   const double sigmult = st[ST_SIGMULT];
   const double* coef = wk.coef + walker * wk.coef_stride;
   double2* zrows = wk.zrows + walker * wk.zrows_stride;
+  double2* zdst = (parts == 1) ? zrows
+                               : wk.zpart + static_cast<size_t>(blockIdx.x) * wk.zpart_stride;
 
   load_twiddles(bs, sm);
   if (tid == 0) {
     int kmaxb;
     sm.isc[0] = spectral_rows(bs, sigmult, sm.rowk2, sm.rowof, kmaxb);
     sm.isc[1] = kmaxb;
-    st[ST_KMAXB] = kmaxb;
-    st[ST_NROWS] = sm.isc[0];
+    if (part == 0) {
+      st[ST_KMAXB] = kmaxb;
+      st[ST_NROWS] = sm.isc[0];
+    }
   }
   __syncthreads();
-  const int nrows = sm.isc[0], kmaxb = sm.isc[1];
+  const int nrows = sm.isc[0];
   const int nchunks = M2 / kChunk;
+  const int c_lo = part * nchunks / parts, c_hi = (part + 1) * nchunks / parts;
   const int row4 = tid >> 2, q = tid & 3;
   double mn_part = 0.0, var_part = 0.0;
 
@@ -538,7 +554,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     double2 acc[kRowsPerPass];
 #pragma unroll
     for (int r = 0; r < kRowsPerPass; ++r) acc[r] = make_double2(0.0, 0.0);
-    for (int c = 0; c < nchunks; ++c) {
+    for (int c = c_lo; c < c_hi; ++c) {
       // fill: thread (n1 = row4, q) makes R at 8 consecutive grid points
       {
         const int j0 = 2 * M2 * row4 + 2 * kChunk * c + 8 * q;
@@ -625,13 +641,71 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       v.y += __shfl_xor(v.y, 1);
       v.x += __shfl_xor(v.x, 2);
       v.y += __shfl_xor(v.y, 2);
-      if (q == 0 && r < np) zrows[(p0 + r) * kM1 + row4] = v;
+      if (q == 0 && r < np) zdst[(p0 + r) * kM1 + row4] = v;
     }
   }
   const double mn = block_sum(mn_part, sm.red);
   const double var = block_sum(var_part, sm.red);
+  if (parts > 1) {
+    if (tid == 0) {
+      wk.mpart[2 * blockIdx.x] = mn;
+      wk.mpart[2 * blockIdx.x + 1] = var;
+    }
+    return;
+  }
   __threadfence_block();  // the row stores, visible to this workgroup
   __syncthreads();
+  forward_tail(bs, wk, walker, sm, mn, var, tmark);
+}
+
+// the sum of the parts' kept rows and moments, in part order, then the
+// spectrum and the noisy rows (one workgroup per walker)
+__global__ __launch_bounds__(kThreads) void spectrum1d_kernel(const BeamsetDev bs,
+                                                               const WorkDev wk,
+                                                               const int parts) {
+  __shared__ XformSmem sm;
+  const int tid = threadIdx.x;
+  const int walker = blockIdx.x;
+  double* st = walker_state(wk, walker);
+  if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
+  const unsigned long long tmark = wall_clock64();
+  if (tid == 0) {
+    int kmaxb;
+    sm.isc[0] = spectral_rows(bs, st[ST_SIGMULT], sm.rowk2, sm.rowof, kmaxb);
+    sm.isc[1] = kmaxb;
+  }
+  __syncthreads();
+  const int nrows = sm.isc[0];
+  const size_t p0 = static_cast<size_t>(walker) * parts;
+  double mn = wk.mpart[2 * p0], var = wk.mpart[2 * p0 + 1];
+  for (int p = 1; p < parts; ++p) {
+    mn += wk.mpart[2 * (p0 + p)];
+    var += wk.mpart[2 * (p0 + p) + 1];
+  }
+  double2* zrows = wk.zrows + walker * wk.zrows_stride;
+  for (int e = tid; e < nrows * kM1; e += kThreads) {
+    double2 v = wk.zpart[p0 * wk.zpart_stride + e];
+    for (int p = 1; p < parts; ++p) v = cadd(v, wk.zpart[(p0 + p) * wk.zpart_stride + e]);
+    zrows[e] = v;
+  }
+  __threadfence_block();
+  __syncthreads();
+  forward_tail(bs, wk, walker, sm, mn, var, tmark);
+}
+
+// moments -> mean shift, R^(0), then the spectrum exp(R^ - R^0) with its
+// phase and each dataset's noisy rows; zrows holds the kept rows and sm the
+// row tables
+__device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int walker,
+                             XformSmem& sm, const double mn, const double var,
+                             unsigned long long tmark) {
+  const int tid = threadIdx.x;
+  const int M = bs.M;
+  const double dflux = bs.dflux;
+  double* st = walker_state(wk, walker);
+  const double sigmult = st[ST_SIGMULT];
+  double2* zrows = wk.zrows + walker * wk.zrows_stride;
+  const int nrows = sm.isc[0], kmaxb = sm.isc[1];
   const double sgpos0 = sqrt(var);
   const bool doshift = (sgpos0 > dflux) && (fabs(mn) > dflux);
   const double shift = doshift ? -mn : 0.0;
@@ -728,10 +802,12 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
 // and the whole-array statistics of the split search
 // ===========================================================================
 __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs,
-                                                              const WorkDev wk) {
+                                                              const WorkDev wk,
+                                                              const int parts) {
   __shared__ XformSmem sm;
   const int tid = threadIdx.x;
-  const int walker = blockIdx.x;
+  const int walker = blockIdx.x / parts;
+  const int part = blockIdx.x - walker * parts;
   const int f = blockIdx.y;
   const double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
@@ -771,16 +847,19 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
     }
     sm.isc[1] = static_cast<int>(bd);
     sm.isc[2] = topidx;
-    dbg[7] = sgpos;
-    dbg[8] = sgneg;
-    dbg[12] = spectral_kmax(sigma, bs.iflux, bs.spectral_cut, M);
-    dbg[13] = sm.isc[0];
-    dbg[19] = kmaxb;
-    dbg[23] = sigma;
+    if (part == 0) {
+      dbg[7] = sgpos;
+      dbg[8] = sgneg;
+      dbg[12] = spectral_kmax(sigma, bs.iflux, bs.spectral_cut, M);
+      dbg[13] = sm.isc[0];
+      dbg[19] = kmaxb;
+      dbg[23] = sigma;
+    }
   }
   __syncthreads();
   const int nrows = sm.isc[0], botidx = sm.isc[1], topidx = sm.isc[2];
   const int nchunks = M2 / kChunk;
+  const int c_lo = part * nchunks / parts, c_hi = (part + 1) * nchunks / parts;
   const int row4 = tid >> 2, q = tid & 3;
 
   double pk = -1.0, mnv = 0.0, s_part = 0.0, sj_part = 0.0;
@@ -790,7 +869,7 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
 #pragma unroll
   for (int r = 0; r < kRowsPerPass; ++r)
     zr[r] = (small_rows && r < nrows) ? zp[r * kM1 + row4] : make_double2(0.0, 0.0);
-  for (int c = 0; c < nchunks; ++c) {
+  for (int c = c_lo; c < c_hi; ++c) {
     {
       const int k1 = row4;
 #pragma unroll
@@ -866,12 +945,21 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
   const double S = block_sum(s_part, sm.red);
   const double SJ = block_sum(sj_part, sm.red);
   if (tid == 0) {
-    dbg[1] = peak;
-    dbg[2] = mni;  // -1: empty search range
-    dbg[3] = mnv;
-    dbg[20] = S;
-    dbg[21] = SJ;
-    dbg[28] = static_cast<double>(wall_clock64() - t0);
+    if (parts == 1) {
+      dbg[1] = peak;
+      dbg[2] = mni;  // -1: empty search range
+      dbg[3] = mnv;
+      dbg[20] = S;
+      dbg[21] = SJ;
+    } else {
+      double* ip = wk.ipart + (static_cast<size_t>(blockIdx.x) * bs.nfields + f) * kIPart;
+      ip[0] = peak;
+      ip[1] = mni;
+      ip[2] = mnv;
+      ip[3] = S;
+      ip[4] = SJ;
+    }
+    if (part == 0) dbg[28] = static_cast<double>(wall_clock64() - t0);
   }
 }
 
@@ -881,7 +969,8 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
 // log2 of its normalised P(D) in place for pixels1d_kernel)
 // ===========================================================================
 __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev bs,
-                                                               const WorkDev wk) {
+                                                               const WorkDev wk,
+                                                               const int parts) {
   __shared__ double red[kWaves * 2];
   __shared__ int isc[4];
   __shared__ double sc[2];
@@ -899,6 +988,33 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
   const double* bmin = wk.bmin + wf * wk.blk_stride;
   const double* bsum = wk.bsum + wf * wk.blk_stride;
   double* dbg = wk.dbg + wf * kDbg;
+  if (parts > 1) {
+    // the inverse transform's parts, combined in part (= index) order
+    if (tid == 0) {
+      const double* ip0 =
+          wk.ipart + (static_cast<size_t>(walker) * parts * bs.nfields + f) * kIPart;
+      double pk = ip0[0], bv = ip0[2], s = ip0[3], sj = ip0[4];
+      int bi = static_cast<int>(ip0[1]);
+      for (int p = 1; p < parts; ++p) {
+        const double* ip = ip0 + static_cast<size_t>(p) * bs.nfields * kIPart;
+        pk = (ip[0] > pk) ? ip[0] : pk;
+        const int ui = static_cast<int>(ip[1]);
+        const double u = ip[2];
+        const bool take = (ui >= 0) && (bi < 0 || u < bv || (u == bv && ui < bi));
+        bv = take ? u : bv;
+        bi = take ? ui : bi;
+        s += ip[3];
+        sj += ip[4];
+      }
+      dbg[1] = pk;
+      dbg[2] = bi;
+      dbg[3] = bv;
+      dbg[20] = s;
+      dbg[21] = sj;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
   const double sgpos = dbg[7], sgneg = dbg[8];
   const double peak = dbg[1];
   const double S = dbg[20], SJ = dbg[21];
@@ -1259,7 +1375,9 @@ __global__ void materialize_pd_kernel(const double* pofd, int N, int split,
 int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
                      const float* d_theta, uint32_t nwalk, uint32_t stride,
                      double* d_loglike, int32_t* d_status, int accumulate,
-                     void* stream, hipEvent_t* ev) {
+                     int parts, void* stream, hipEvent_t* ev) {
+  if (parts < 1 || parts > 4) return POFD_E_ARG;
+  if (parts > 1 && nwalk * static_cast<uint32_t>(parts) > wk.nslots) return POFD_E_ARG;
   (void)beamset_idx;
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto mark = [&](int i) {
@@ -1269,13 +1387,16 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
   hipLaunchKernelGGL(prep1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk,
                      d_theta, stride, d_status, accumulate);
   mark(POFD_STAGE_FORWARD);
-  hipLaunchKernelGGL(forward1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk);
+  hipLaunchKernelGGL(forward1d_kernel, dim3(nwalk * parts), dim3(kThreads), 0, s, bs, wk,
+                     parts);
+  if (parts > 1)
+    hipLaunchKernelGGL(spectrum1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk, parts);
   mark(POFD_STAGE_INVERSE);
-  hipLaunchKernelGGL(inverse1d_kernel, dim3(nwalk, bs.nfields), dim3(kThreads), 0,
-                     s, bs, wk);
+  hipLaunchKernelGGL(inverse1d_kernel, dim3(nwalk * parts, bs.nfields), dim3(kThreads), 0,
+                     s, bs, wk, parts);
   mark(POFD_STAGE_FINALIZE);
   hipLaunchKernelGGL(finalize1d_kernel, dim3(nwalk, bs.nfields), dim3(kThreads), 0,
-                     s, bs, wk);
+                     s, bs, wk, parts);
   mark(POFD_STAGE_PIXELS);
   bool any_unbinned = false;
   for (int f = 0; f < bs.nfields; ++f) any_unbinned |= !bs.fields[f].binned;

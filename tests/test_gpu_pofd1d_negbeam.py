@@ -102,6 +102,9 @@ def test_negbeam_pd(negbeam, w):
     assert int(dbg["split"]) == ref["split"]
     pdl, mf, df = ctx.get_pd(w, is_log=False)
     assert df == ref["dflux"]
-    assert abs(mf - ref["minflux"]) < 1e-9 * df
+    # the P(D) mean (minflux) comes from sums over the whole grid, added in
+    # another order when a walker's transform is split over workgroups:
+    # rounding of ~1e-13 relative, 1e-8 of a flux bin at most
+    assert abs(mf - ref["minflux"]) < 1e-8 * df
     np.testing.assert_allclose(pdl, ref["pd"], rtol=0,
                                atol=1e-10 * ref["pd"].max())

@@ -1,0 +1,93 @@
+"""The 1D transforms split over 1 to 4 workgroups per walker
+(pofd1d_set_parts).  The split only changes the order in which partial sums
+are added, so every part count gives the same log likelihood to the
+rounding of double-precision transforms.  On the 10^6-pixel map, whose logL
+is -1.9e6, reordering the partial sums moves it by up to ~1e-6 (relative
+5e-13, the noise the full-size parity tests allow the reference's own FFT),
+so there every part count is held to the exact-math value of the
+reference's algorithm (the extended-precision oracle) within the 1e-6
+parity bar; on the small map part counts agree within 1e-8, and the P(D)
+within 1e-12 of its peak."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REFDATA
+from oracle_side import build_oracle
+from pofd_affine_amd import fitsio, synth, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[gpu-parts-case] " + msg, flush=True)
+
+
+def _by_parts(ctx, th):
+    out = {}
+    for parts in (1, 2, 3, 4):
+        ctx.set_parts(parts)
+        ll, st = ctx.loglike_batch(th)
+        assert np.all(st == 0), (parts, st)
+        # unbinned fields keep only log2 of the P(D)
+        pd = 2.0 ** ctx.get_pd(len(th) - 1, is_log=True)[0]
+        out[parts] = (ll, pd)
+    ctx.set_parts(0)
+    return out
+
+
+@pytest.mark.parametrize("binned", [True, False])
+def test_parts_full_size(binned):
+    _say("full size by parts, binned=%s" % binned)
+    # sources inside the knot range (<= 450 mJy), as test_config1d_in_support
+    inp = workload.make_inputs(npix_side=1000, seed=42, smax=0.45)
+    th = synth.walker_thetas(24, seed=99)
+    ctx = workload.build_context(inp, 131072, 2048, binned=binned)
+    out = _by_parts(ctx, th)
+    for parts in (2, 3, 4):
+        pk = np.max(out[1][1])
+        np.testing.assert_allclose(out[parts][1], out[1][1], rtol=0,
+                                   atol=1e-12 * pk)
+    L = build_oracle(inp, 131072, 2048, binned, th[0])
+    L.set_precise(1)
+    ext, rst = L.loglike_batch(th, 16)
+    assert np.all(rst == 0)
+    for parts in (1, 2, 3, 4):
+        np.testing.assert_allclose(out[parts][0], ext, rtol=0, atol=1e-6)
+
+
+def test_parts_negative_beam():
+    _say("negative beam by parts")
+    pix, ps = fitsio.read_beam(os.path.join(REFDATA,
+                                            "band12_beam_f100.fits[0]"))
+    inp = dict(workload.make_inputs(npix_side=200, seed=7), beam=pix,
+               pixsize=ps)
+    th = synth.walker_thetas(6, seed=13)
+    ctx = workload.build_context(inp, 32768, 512, binned=True)
+    out = _by_parts(ctx, th)
+    L = build_oracle(inp, 32768, 512, True, th[0])
+    ref, rst = L.loglike_batch(th, 6)
+    assert np.all(rst == 0)
+    for parts in (1, 2, 3, 4):
+        np.testing.assert_allclose(out[parts][0], ref, rtol=0, atol=1e-6)
+
+
+def test_parts_auto_choice_and_limits():
+    """The automatic choice splits small batches; a forced count that does
+    not fit the workspace's slots is reduced, and a bad count refused."""
+    _say("parts auto choice")
+    inp = workload.make_inputs(npix_side=200, seed=7)
+    ctx = workload.build_context(inp, 32768, 512, binned=True)
+    th = synth.walker_thetas(200, seed=5)
+    ll_big, st = ctx.loglike_batch(th)               # 1 part per walker
+    assert np.all(st == 0)
+    ll_small, st = ctx.loglike_batch(th[:40])        # 4 parts per walker
+    assert np.all(st == 0)
+    np.testing.assert_allclose(ll_small, ll_big[:40], rtol=0, atol=1e-8)
+    ctx.set_parts(4)
+    ll_forced, st = ctx.loglike_batch(th)            # reduced to fit
+    assert np.all(st == 0)
+    np.testing.assert_allclose(ll_forced, ll_big, rtol=0, atol=1e-8)
+    with pytest.raises(Exception):
+        ctx.set_parts(5)
