@@ -165,6 +165,17 @@ int pofd1d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
                                 uint32_t n_walkers, uint32_t theta_stride,
                                 double *d_loglike_out, int32_t *d_status_out,
                                 void *stream);
+/* As pofd1d_loglike_batch_device for the first *d_count rows (an int32 in
+   device memory, 0 <= *d_count <= max_walkers, read when the kernels run):
+   a caller whose batch size is produced on the GPU (the delayed-acceptance
+   sampler's first-stage survivors) launches without reading it back.  No
+   reference counterpart.  Outputs past *d_count are left unwritten. */
+int pofd1d_loglike_batch_device_count(pofd_ctx *ctx, const float *d_theta,
+                                      uint32_t max_walkers,
+                                      const int32_t *d_count,
+                                      uint32_t theta_stride,
+                                      double *d_loglike_out,
+                                      int32_t *d_status_out, void *stream);
 /* Tuning, no reference counterpart: workgroups per walker of the forward
    and inverse transforms.  0 (the default) chooses by batch size,
    min(4, 256 / n_walkers) and at least 1, so that small batches (such as
@@ -439,8 +450,7 @@ int pofd_ens_set_surrogate(pofd_ens *ens, pofd_mlp *mlp,
                            const uint32_t *param_index, uint32_t nidx,
                            double gate, double holdout_rms, int *enabled);
 /* proposals made and exact likelihood evaluations since set_positions */
-int pofd_ens_counts(const pofd_ens *ens, uint64_t *n_proposed,
-                    uint64_t *n_exact);
+int pofd_ens_counts(pofd_ens *ens, uint64_t *n_proposed, uint64_t *n_exact);
 
 #ifdef __cplusplus
 }

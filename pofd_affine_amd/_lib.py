@@ -55,6 +55,8 @@ _SIGS = {
                                    C.POINTER(C.c_int)]),
     "pofd1d_prepare": (C.c_int, [VP]),
     "pofd1d_set_parts": (C.c_int, [VP, C.c_int]),
+    "pofd1d_loglike_batch_device_count": (C.c_int, [VP, VP, C.c_uint32, VP,
+                                                    C.c_uint32, VP, VP, VP]),
     "pofd1d_set_rrange": (C.c_int, [VP, C.c_int, D, D]),
     "pofd1d_beamset_info": (C.c_int, [VP, C.c_int, DP]),
     "pofd1d_loglike_batch": (C.c_int, [VP, FP, C.c_uint32, C.c_uint32, DP,
@@ -350,6 +352,15 @@ class Context:
         self._check(lib().pofd1d_loglike_batch_device(
             self.ptr, VP(d_theta), nwalk, stride, VP(d_ll), VP(d_st),
             VP(stream)), "pofd1d_loglike_batch_device")
+
+    def loglike_batch_device_count(self, d_theta, max_walkers, d_count,
+                                   stride, d_ll, d_st, stream=0):
+        """As loglike_batch_device for the first *d_count rows, the count
+        an int32 in device memory."""
+        self._check(lib().pofd1d_loglike_batch_device_count(
+            self.ptr, VP(d_theta), max_walkers, VP(d_count), stride,
+            VP(d_ll), VP(d_st), VP(stream)),
+            "pofd1d_loglike_batch_device_count")
 
     def get_pd(self, walker, b=0, f=0, is_log=True):
         """is_log True/False: PD::pd_ after getPD; "raw": the clamped,

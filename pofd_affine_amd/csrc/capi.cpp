@@ -575,14 +575,6 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   return POFD_OK;
 }
 
-// workgroups per walker of the transforms: a batch that leaves most of the
-// 256 compute units idle gives each walker 2 or 4
-int choose_parts(const pofd_ctx* c, const WorkDev& w, uint32_t nwalk) {
-  int parts = c->force_parts;
-  if (parts == 0) parts = static_cast<int>(std::min<uint32_t>(4, std::max<uint32_t>(1, 256 / nwalk)));
-  while (parts > 1 && nwalk * static_cast<uint32_t>(parts) > w.nslots) --parts;
-  return parts;
-}
 
 int check_ready(pofd_ctx* c) {
   if (!c) return POFD_E_ARG;
@@ -782,10 +774,10 @@ extern "C" int pofd1d_set_parts(pofd_ctx* c, int parts) {
   return POFD_OK;
 }
 
-extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
-                                           uint32_t nwalk, uint32_t stride,
-                                           double* d_ll, int32_t* d_st,
-                                           void* stream) {
+namespace {
+// d_count: nullptr, or the batch size on the device (at most nwalk)
+int batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwalk, const int32_t* d_count,
+                 uint32_t stride, double* d_ll, int32_t* d_st, void* stream) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (nwalk == 0) return POFD_OK;
@@ -800,14 +792,30 @@ extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
     hipEvent_t* ev = nullptr;
     if ((rc = pofd2::ctx_stage_events(c, &ev))) return rc;
     s.dev.bonus = (b == 0) ? 1 : 0;  // the counts integrals once per batch
-    rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta,
-                                nwalk, stride, d_ll, d_st, b > 0 ? 1 : 0,
-                                choose_parts(c, s.wk, nwalk), stream, ev);
+    rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta, nwalk, d_count,
+                                stride, d_ll, d_st, b > 0 ? 1 : 0, c->force_parts, stream,
+                                ev);
     if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: launch failed: " +
                                    std::string(hipGetErrorString(hipGetLastError())));
   }
   c->last_nwalk = nwalk;
   return POFD_OK;
+}
+}  // namespace
+
+extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
+                                           uint32_t nwalk, uint32_t stride,
+                                           double* d_ll, int32_t* d_st,
+                                           void* stream) {
+  return batch_device(c, d_theta, nwalk, nullptr, stride, d_ll, d_st, stream);
+}
+
+extern "C" int pofd1d_loglike_batch_device_count(pofd_ctx* c, const float* d_theta,
+                                                 uint32_t max_walkers, const int32_t* d_count,
+                                                 uint32_t stride, double* d_ll, int32_t* d_st,
+                                                 void* stream) {
+  if (!d_count) return fail(c, POFD_E_ARG, "pofd1d_loglike_batch_device_count: null count");
+  return batch_device(c, d_theta, max_walkers, d_count, stride, d_ll, d_st, stream);
 }
 
 extern "C" int pofd1d_loglike_batch(pofd_ctx* c, const float* theta,

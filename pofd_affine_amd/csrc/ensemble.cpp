@@ -31,7 +31,7 @@ struct pofd_ens {
   pofd_mlp* mlp = nullptr;
   bool use_da = false;
   int* sidx = nullptr;         // parameter index of each surrogate input
-  void* dabuf[8] = {};
+  void* dabuf[9] = {};
   uint64_t n_exact = 0, n_proposed = 0;
   std::string err;
   ~pofd_ens() {
@@ -107,6 +107,17 @@ int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int stor
   if (pofd_ens_dev::launch_stage1(s->e, half, step, inv_temp, s->stream) ||
       pofd_ens_dev::launch_compact(s->e, s->stream))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: first stage launch failed");
+  if (s->target == POFD_TARGET_1D) {
+    // the survivors' count stays on the device: no host round trip
+    int rc = pofd1d_loglike_batch_device_count(s->ctx, s->e.cprop, hw, s->e.npass,
+                                               s->npar, s->e.cll, s->e.cst, s->stream);
+    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+    if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, s->stream))
+      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
+    if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))
+      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
+    return POFD_OK;
+  }
   int32_t npass = 0;
   hipError_t he = hipMemcpyAsync(&npass, s->e.npass, sizeof(int32_t), hipMemcpyDeviceToHost,
                                  s->stream);
@@ -118,7 +129,6 @@ int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int stor
     if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, s->stream))
       return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
   }
-  s->n_exact += static_cast<uint64_t>(npass);
   if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
   return POFD_OK;
@@ -254,6 +264,7 @@ int pofd_ens_set_positions(pofd_ens* s, const float* theta) {
   s->nstored = 0;
   s->step = 0;
   s->n_exact = s->n_proposed = 0;
+  if (s->e.nexact) (void)hipMemset(s->e.nexact, 0, sizeof(unsigned long long));
   s->positioned = true;
   return refresh_surrogate(s);
 }
@@ -277,10 +288,10 @@ int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_ind
   if (!(holdout_rms <= gate)) return POFD_OK;
   const size_t W = s->nwalk, H = s->nwalk / 2, P = s->npar;
   if (!s->dabuf[0]) {
-    const size_t sizes[8] = {W * sizeof(double), H * sizeof(double), H * sizeof(int32_t),
+    const size_t sizes[9] = {W * sizeof(double), H * sizeof(double), H * sizeof(int32_t),
                              H * sizeof(int32_t), 64, H * P * sizeof(float),
-                             H * sizeof(double), H * sizeof(int32_t)};
-    for (int i = 0; i < 8; ++i) {
+                             H * sizeof(double), H * sizeof(int32_t), 64};
+    for (int i = 0; i < 9; ++i) {
       hipError_t he = hipMalloc(&s->dabuf[i], sizes[i]);
       if (he == hipSuccess) he = hipMemset(s->dabuf[i], 0, sizes[i]);
       if (he != hipSuccess) return ehip(s, he, "hipMalloc(delayed acceptance)");
@@ -293,6 +304,7 @@ int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_ind
     s->e.cprop = static_cast<float*>(s->dabuf[5]);
     s->e.cll = static_cast<double*>(s->dabuf[6]);
     s->e.cst = static_cast<int32_t*>(s->dabuf[7]);
+    s->e.nexact = static_cast<unsigned long long*>(s->dabuf[8]);
   }
   std::vector<int> idx(param_index, param_index + nidx);
   if (s->sidx) (void)hipFree(s->sidx);
@@ -305,10 +317,17 @@ int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_ind
   return refresh_surrogate(s);
 }
 
-int pofd_ens_counts(const pofd_ens* s, uint64_t* n_proposed, uint64_t* n_exact) {
+int pofd_ens_counts(pofd_ens* s, uint64_t* n_proposed, uint64_t* n_exact) {
   if (!s) return POFD_E_ARG;
+  unsigned long long dev = 0;
+  if (s->e.nexact) {
+    hipError_t he = hipStreamSynchronize(s->stream);
+    if (he == hipSuccess)
+      he = hipMemcpy(&dev, s->e.nexact, sizeof(dev), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return ehip(s, he, "hipMemcpy(counts)");
+  }
   if (n_proposed) *n_proposed = s->n_proposed;
-  if (n_exact) *n_exact = s->n_exact;
+  if (n_exact) *n_exact = s->n_exact + dev;
   return POFD_OK;
 }
 

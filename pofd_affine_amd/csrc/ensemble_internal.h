@@ -41,6 +41,7 @@ struct EnsDev {
   float* cprop;      // nwalk/2 x npar compact proposals
   double* cll;       // nwalk/2        their exact log likelihoods
   int32_t* cst;      // nwalk/2
+  unsigned long long* nexact;  // 1: exact evaluations requested by stage 1
 };
 
 int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st);

@@ -237,7 +237,10 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
     base += woff[kCompactThreads / 64];
     __syncthreads();
   }
-  if (tid == 0) *e.npass = base;
+  if (tid == 0) {
+    *e.npass = base;
+    *e.nexact += static_cast<unsigned long long>(base);
+  }
 }
 
 __global__ __launch_bounds__(64) void gather_pass_kernel(const EnsDev e) {

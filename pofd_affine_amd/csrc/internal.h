@@ -123,7 +123,25 @@ struct WorkDev {
   uint32_t nslots;
   size_t coef_stride, ynode_stride, zrows_stride, spec_stride, pofd_stride,
       blk_stride, zpart_stride;
+  // the batch of one launch: n walkers, read from dcount on the device when
+  // it is set (the grids then cover the largest possible batch and the
+  // workgroups past it return), else nhost; force_parts as pofd1d_set_parts
+  const int32_t* dcount;
+  int nhost;
+  int force_parts;
 };
+
+// workgroups per walker of the split transforms for a batch of n walkers
+// (the same rule on the host and in the kernels)
+__host__ __device__ inline int batch_parts(int n, int force, uint32_t nslots) {
+  int parts = force;
+  if (parts == 0) {
+    parts = (n > 0) ? 256 / n : 4;
+    parts = parts < 1 ? 1 : (parts > 4 ? 4 : parts);
+  }
+  while (parts > 1 && static_cast<uint32_t>(n * parts) > nslots) --parts;
+  return parts;
+}
 
 }  // namespace pofd
 
@@ -134,10 +152,11 @@ struct pofd_beam {
 // launcher implemented in pofd1d_kernel.hip
 namespace pofd {
 // ev: nullptr, or POFD_NSTAGES+1 events recorded around the stages
+// nwalk: the batch, or with d_count its largest possible size
 int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
-                     const float* d_theta, uint32_t nwalk, uint32_t stride,
-                     double* d_loglike, int32_t* d_status, int accumulate,
-                     int parts, void* stream, hipEvent_t* ev);
+                     const float* d_theta, uint32_t nwalk, const int32_t* d_count,
+                     uint32_t stride, double* d_loglike, int32_t* d_status,
+                     int accumulate, int force_parts, void* stream, hipEvent_t* ev);
 int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
                           int is_log, int already_log, double* d_out,
                           void* stream);

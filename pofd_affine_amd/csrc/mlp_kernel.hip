@@ -154,20 +154,36 @@ __global__ void unstd_kernel(const float* pred, int n, double ym, double ys, dou
 
 // ---- fused inference: gather + standardise, three GELU layers on the
 // matrix cores, output layer, de-standardisation, in one launch.  A
-// workgroup of four waves takes 16 rows; the activations stay in LDS as
+// workgroup of sixteen waves takes 16 rows; the activations stay in LDS as
 // bfloat16 and the weights stream from L2.  Each wave owns 16-column tiles
-// of the layer output (tiles w, w+4, ...) and accumulates them with
+// of the layer output (tiles w, w+16, ...) and accumulates them with
 // v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)..+3] and
-// B[4(l/16)..+3][l%16] and receives D[4(l/16)..+3][l%16].
+// B[4(l/16)..+3][l%16] and receives D[4(l/16)..+3][l%16].  The weight
+// fragments of four k-steps are loaded before their four products, so a
+// wave waits on L2 once per 64 inputs rather than once per 16.
 typedef short short4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 constexpr int kInfRows = 16;
+constexpr int kInfThreads = 1024;
+constexpr int kInfWaves = kInfThreads / 64;
 
-__global__ __launch_bounds__(256) void infer_kernel(const float* src, int stride, const int* idx,
-                                                    const float* mean, const float* istd, int nin,
-                                                    int H, int rows, const bf16* wb, const float* w,
-                                                    InferLayout lay, double ym, double ys,
-                                                    double* out) {
+__device__ __forceinline__ short4v wfrag(const bf16* W, int j, int k, int H, int K) {
+  short4v bv;
+  if (j < H && k + 3 < K && (K & 3) == 0) {
+    bv = *reinterpret_cast<const short4v*>(W + static_cast<size_t>(j) * K + k);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      bv[q] = (j < H && k + q < K) ? static_cast<short>(W[static_cast<size_t>(j) * K + k + q])
+                                   : static_cast<short>(0);
+  }
+  return bv;
+}
+
+__global__ __launch_bounds__(kInfThreads) void infer_kernel(
+    const float* src, int stride, const int* idx, const float* mean, const float* istd, int nin,
+    int H, int rows, const bf16* wb, const float* w, InferLayout lay, double ym, double ys,
+    double* out) {
   extern __shared__ bf16 act[];  // two buffers of kInfRows x ld bf16
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = blockIdx.x * kInfRows;
@@ -176,7 +192,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const float* src, int stride
   const int ld = (Hp > Kin ? Hp : Kin) + 4;  // row pitch (bf16), 8-byte aligned
   bf16* bufA = act;
   bf16* bufB = act + kInfRows * ld;
-  for (int e = tid; e < kInfRows * Kin; e += 256) {
+  for (int e = tid; e < kInfRows * Kin; e += kInfThreads) {
     const int i = e / Kin, c = e % Kin;
     const int r = r0 + i;
     float v = 0.0f;
@@ -191,22 +207,25 @@ __global__ __launch_bounds__(256) void infer_kernel(const float* src, int stride
     const int Kp = (l == 0) ? Kin : Hp;
     const bf16* W = wb + lay.w[l];
     const float* b = w + lay.b[l];
-    for (int t = wave; t < ntile; t += 4) {
+    for (int t = wave; t < ntile; t += kInfWaves) {
       const int j = t * 16 + li;  // output column this lane feeds B for
       float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int k0 = 0; k0 < Kp; k0 += 16) {
-        const int k = k0 + lk;
-        const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + k);
-        short4v bv;
-        if (j < H && k + 3 < K && (K & 3) == 0) {
-          bv = *reinterpret_cast<const short4v*>(W + static_cast<size_t>(j) * K + k);
-        } else {
+      int k0 = 0;
+      for (; k0 + 64 <= Kp; k0 += 64) {
+        short4v bq[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            bv[q] = (j < H && k + q < K) ? static_cast<short>(W[static_cast<size_t>(j) * K + k + q])
-                                         : static_cast<short>(0);
+        for (int u = 0; u < 4; ++u) bq[u] = wfrag(W, j, k0 + 16 * u + lk, H, K);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const short4v a =
+              *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + 16 * u + lk);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u], acc, 0, 0, 0);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+      }
+      for (; k0 < Kp; k0 += 16) {
+        const short4v bq = wfrag(W, j, k0 + lk, H, K);
+        const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + lk);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
       }
       // D rows lk..lk+3, column j
 #pragma unroll
@@ -224,14 +243,14 @@ __global__ __launch_bounds__(256) void infer_kernel(const float* src, int stride
     bufA = bufB;
     bufB = tmp;
   }
-  // output layer: 16 threads per row
+  // output layer: 64 threads (one wave) per row
   {
-    const int i = tid >> 4, s = tid & 15;
+    const int i = wave, s = lane;
     const bf16* W4 = wb + lay.w[3];
     float acc = 0.0f;
-    for (int j = s; j < H; j += 16) acc += bf2f(bufA[i * ld + j]) * bf2f(W4[j]);
+    for (int j = s; j < H; j += 64) acc += bf2f(bufA[i * ld + j]) * bf2f(W4[j]);
 #pragma unroll
-    for (int o = 8; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 16);
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
     const int r = r0 + i;
     if (s == 0 && r < rows) out[r] = static_cast<double>(acc + w[lay.b[3]]) * ys + ym;
   }
@@ -307,7 +326,7 @@ int infer(const float* src, int stride, const int* idx, const float* mean, const
   const int Hp = (H + 15) & ~15, Kin = (nin + 15) & ~15;
   const int ld = (Hp > Kin ? Hp : Kin) + 4;
   const size_t lds = 2 * kInfRows * ld * sizeof(bf16);
-  hipLaunchKernelGGL(infer_kernel, dim3((rows + kInfRows - 1) / kInfRows), dim3(256), lds, st,
+  hipLaunchKernelGGL(infer_kernel, dim3((rows + kInfRows - 1) / kInfRows), dim3(kInfThreads), lds, st,
                      src, stride, idx, mean, istd, nin, H, rows, wb, w, lay, ym, ys, out);
   return ok();
 }

@@ -237,6 +237,10 @@ __device__ int spectral_rows(const BeamsetDev& bs, double sigmult, short* rowk2,
   return nrows;
 }
 
+__device__ __forceinline__ int batch_n(const WorkDev& wk) {
+  return wk.dcount ? *wk.dcount : wk.nhost;
+}
+
 __device__ __forceinline__ double* walker_state(const WorkDev& wk, int w) {
   return wk.state + static_cast<size_t>(w) * kState;
 }
@@ -289,6 +293,7 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
   __shared__ PrepSmem sm;
   const int tid = threadIdx.x;
   const int walker = blockIdx.x;
+  if (walker >= batch_n(wk)) return;
   const int ninterp = bs.ninterp, nk = bs.nknots;
   double* st = walker_state(wk, walker);
   const unsigned long long t0 = wall_clock64();
@@ -515,12 +520,14 @@ __device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int 
 // kept rows and partial moments and spectrum1d_kernel sums them in part
 // order and finishes.
 __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs,
-                                                              const WorkDev wk,
-                                                              const int parts) {
+                                                              const WorkDev wk) {
   __shared__ XformSmem sm;
   const int tid = threadIdx.x;
+  const int n = batch_n(wk);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
   const int walker = blockIdx.x / parts;
   const int part = blockIdx.x - walker * parts;
+  if (walker >= n) return;
   double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
   unsigned long long tmark = wall_clock64();
@@ -661,11 +668,13 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
 // the sum of the parts' kept rows and moments, in part order, then the
 // spectrum and the noisy rows (one workgroup per walker)
 __global__ __launch_bounds__(kThreads) void spectrum1d_kernel(const BeamsetDev bs,
-                                                               const WorkDev wk,
-                                                               const int parts) {
+                                                               const WorkDev wk) {
   __shared__ XformSmem sm;
   const int tid = threadIdx.x;
   const int walker = blockIdx.x;
+  const int n = batch_n(wk);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
+  if (walker >= n || parts == 1) return;  // one part: the forward finished
   double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
   const unsigned long long tmark = wall_clock64();
@@ -802,13 +811,15 @@ __device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int 
 // and the whole-array statistics of the split search
 // ===========================================================================
 __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs,
-                                                              const WorkDev wk,
-                                                              const int parts) {
+                                                              const WorkDev wk) {
   __shared__ XformSmem sm;
   const int tid = threadIdx.x;
+  const int n = batch_n(wk);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
   const int walker = blockIdx.x / parts;
   const int part = blockIdx.x - walker * parts;
   const int f = blockIdx.y;
+  if (walker >= n) return;
   const double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
   const unsigned long long t0 = wall_clock64();
@@ -969,14 +980,16 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
 // log2 of its normalised P(D) in place for pixels1d_kernel)
 // ===========================================================================
 __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev bs,
-                                                               const WorkDev wk,
-                                                               const int parts) {
+                                                               const WorkDev wk) {
   __shared__ double red[kWaves * 2];
   __shared__ int isc[4];
   __shared__ double sc[2];
   const int tid = threadIdx.x;
   const int walker = blockIdx.x;
   const int f = blockIdx.y;
+  const int n = batch_n(wk);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
+  if (walker >= n) return;
   const double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
   unsigned long long tmark = wall_clock64();
@@ -1258,7 +1271,7 @@ __global__ __launch_bounds__(kPixThreads) void pixels1d_kernel(const BeamsetDev 
   const int walker = blockIdx.y;
   const int f = blockIdx.z;
   const FieldDev& fd = bs.fields[f];
-  if (fd.binned) return;
+  if (fd.binned || walker >= batch_n(wk)) return;
   const size_t wf = static_cast<size_t>(walker) * bs.nfields + f;
   double* out = wk.upart + wf * kPixBlocks + blk;
   const double* dbg = wk.dbg + wf * kDbg;
@@ -1311,11 +1324,10 @@ __global__ __launch_bounds__(kPixThreads) void pixels1d_kernel(const BeamsetDev 
 // datasets of the beam set in order, added to the batch outputs
 // ===========================================================================
 __global__ void combine1d_kernel(const BeamsetDev bs, const WorkDev wk,
-                                 const int accumulate, const uint32_t nwalk,
-                                 double* __restrict__ loglike,
+                                 const int accumulate, double* __restrict__ loglike,
                                  int32_t* __restrict__ status) {
-  const uint32_t walker = blockIdx.x * blockDim.x + threadIdx.x;
-  if (walker >= nwalk) return;
+  const int walker = blockIdx.x * blockDim.x + threadIdx.x;
+  if (walker >= batch_n(wk)) return;
   const double* st = walker_state(wk, walker);
   const int s = static_cast<int>(st[ST_STATUS]);
   const double prev = accumulate ? loglike[walker] : 0.0;
@@ -1372,12 +1384,27 @@ __global__ void materialize_pd_kernel(const double* pofd, int N, int split,
   }
 }
 
-int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
-                     const float* d_theta, uint32_t nwalk, uint32_t stride,
-                     double* d_loglike, int32_t* d_status, int accumulate,
-                     int parts, void* stream, hipEvent_t* ev) {
-  if (parts < 1 || parts > 4) return POFD_E_ARG;
-  if (parts > 1 && nwalk * static_cast<uint32_t>(parts) > wk.nslots) return POFD_E_ARG;
+int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk_in, int beamset_idx,
+                     const float* d_theta, uint32_t nwalk, const int32_t* d_count,
+                     uint32_t stride, double* d_loglike, int32_t* d_status,
+                     int accumulate, int force_parts, void* stream, hipEvent_t* ev) {
+  if (force_parts < 0 || force_parts > 4) return POFD_E_ARG;
+  WorkDev wk = wk_in;
+  wk.dcount = d_count;
+  wk.nhost = static_cast<int>(nwalk);
+  wk.force_parts = force_parts;
+  // workgroups of the split transforms: exactly n x parts for a host count;
+  // for a device count every batch up to nwalk fits in max(nwalk, nslots)
+  uint32_t nxf;
+  bool split;
+  if (d_count) {
+    nxf = (nwalk > wk.nslots) ? nwalk : wk.nslots;
+    split = true;
+  } else {
+    const int parts = batch_parts(static_cast<int>(nwalk), force_parts, wk.nslots);
+    nxf = nwalk * static_cast<uint32_t>(parts);
+    split = parts > 1;
+  }
   (void)beamset_idx;
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto mark = [&](int i) {
@@ -1387,16 +1414,14 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
   hipLaunchKernelGGL(prep1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk,
                      d_theta, stride, d_status, accumulate);
   mark(POFD_STAGE_FORWARD);
-  hipLaunchKernelGGL(forward1d_kernel, dim3(nwalk * parts), dim3(kThreads), 0, s, bs, wk,
-                     parts);
-  if (parts > 1)
-    hipLaunchKernelGGL(spectrum1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk, parts);
+  hipLaunchKernelGGL(forward1d_kernel, dim3(nxf), dim3(kThreads), 0, s, bs, wk);
+  if (split)
+    hipLaunchKernelGGL(spectrum1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk);
   mark(POFD_STAGE_INVERSE);
-  hipLaunchKernelGGL(inverse1d_kernel, dim3(nwalk * parts, bs.nfields), dim3(kThreads), 0,
-                     s, bs, wk, parts);
+  hipLaunchKernelGGL(inverse1d_kernel, dim3(nxf, bs.nfields), dim3(kThreads), 0, s, bs, wk);
   mark(POFD_STAGE_FINALIZE);
   hipLaunchKernelGGL(finalize1d_kernel, dim3(nwalk, bs.nfields), dim3(kThreads), 0,
-                     s, bs, wk, parts);
+                     s, bs, wk);
   mark(POFD_STAGE_PIXELS);
   bool any_unbinned = false;
   for (int f = 0; f < bs.nfields; ++f) any_unbinned |= !bs.fields[f].binned;
@@ -1405,7 +1430,7 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
                        dim3(kPixThreads), 0, s, bs, wk);
   mark(POFD_STAGE_COMBINE);
   hipLaunchKernelGGL(combine1d_kernel, dim3((nwalk + 63) / 64), dim3(64), 0, s,
-                     bs, wk, accumulate, nwalk, d_loglike, d_status);
+                     bs, wk, accumulate, d_loglike, d_status);
   mark(POFD_NSTAGES);
   return hipGetLastError() == hipSuccess ? POFD_OK : POFD_E_INTERNAL;
 }

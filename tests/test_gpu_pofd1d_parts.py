@@ -15,7 +15,7 @@ import pytest
 
 from conftest import REFDATA
 from oracle_side import build_oracle
-from pofd_affine_amd import fitsio, synth, workload
+from pofd_affine_amd import fitsio, hiprt, synth, workload
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +91,35 @@ def test_parts_auto_choice_and_limits():
     np.testing.assert_allclose(ll_forced, ll_big, rtol=0, atol=1e-8)
     with pytest.raises(Exception):
         ctx.set_parts(5)
+
+
+@pytest.mark.parametrize("count", [0, 1, 37, 70, 150])
+def test_device_count_batch(count):
+    """pofd1d_loglike_batch_device_count: the batch size read from device
+    memory when the kernels run; rows past it are left untouched."""
+    _say("device count %d" % count)
+    inp = workload.make_inputs(npix_side=200, seed=7)
+    ctx = workload.build_context(inp, 32768, 512, binned=True)
+    nmax = 160
+    th = synth.walker_thetas(nmax, seed=17)
+    ref, st = ctx.loglike_batch(th[:max(count, 1)])
+    assert np.all(st == 0)
+    d_th = hiprt.DeviceBuffer(th.nbytes)
+    d_th.upload(th)
+    d_n = hiprt.DeviceBuffer(4)
+    d_n.upload(np.array([count], dtype=np.int32))
+    d_ll = hiprt.DeviceBuffer(nmax * 8)
+    d_st = hiprt.DeviceBuffer(nmax * 4)
+    d_ll.upload(np.full(nmax, 123.0))
+    d_st.upload(np.full(nmax, -7, dtype=np.int32))
+    ctx.loglike_batch_device_count(d_th.value, nmax, d_n.value, th.shape[1],
+                                   d_ll.value, d_st.value)
+    hiprt.device_synchronize()
+    ll = np.zeros(nmax)
+    stv = np.zeros(nmax, dtype=np.int32)
+    d_ll.download(ll)
+    d_st.download(stv)
+    assert np.all(stv[:count] == 0)
+    # the same part count as a host batch of this size
+    np.testing.assert_array_equal(ll[:count], ref[:count])
+    assert np.all(ll[count:] == 123.0) and np.all(stv[count:] == -7)
