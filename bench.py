@@ -57,7 +57,7 @@ def main():
                     help="steps of each of the plain and the delayed-"
                          "acceptance chains compared (0 skips that leg)")
     ap.add_argument("--da-walkers", type=int, default=512)
-    ap.add_argument("--da-burn", type=int, default=300)
+    ap.add_argument("--da-burn", type=int, default=500)
     ap.add_argument("--da-train-steps", type=int, default=100)
     ap.add_argument("--da-epochs", type=int, default=60)
     ap.add_argument("--only-da", action="store_true",
@@ -252,12 +252,15 @@ def main():
 
 def bench_da(args, inp):
     """The delayed-acceptance leg: the same binned 1D likelihood (nflux
-    2^17), a 512-walker stretch move.  A plain burn-in, then the (theta,
+    2^17), a 512-walker stretch move.  A plain burn-in of 500 steps from a
+    0.01-dex ball around the generating knots, then the (theta,
     exact logL) pairs of the next 100 stored steps (51,200 pairs) train an
     8 -> 3x256 -> 1 surrogate on the GPU; from that one state a plain chain
     and a delayed-acceptance chain of the same length run, and are compared
     on effective samples per second (walkers x steps / tau of the
-    walker-averaged series, per knot) and on the knot posterior medians."""
+    walker-averaged series, per knot) and on the knot posterior medians; a
+    second plain chain with another seed gives the Monte Carlo spread the
+    median shift is to be read against."""
     from pofd_affine_amd import _lib, synth, workload
     N, W = args.nflux, args.da_walkers
     ctx = workload.build_context(inp, N, args.ninterp, binned=True)
@@ -291,8 +294,8 @@ def bench_da(args, inp):
                        holdout_frac=0.1, seed=6)
     t_train = time.perf_counter() - t0
 
-    def chain(use_da):
-        e = make(start, 41)
+    def chain(use_da, seed=41):
+        e = make(start, seed)
         on = e.set_surrogate(net, knots) if use_da else False
         e.run(5, thin=0)  # warm the launch path
         n0 = e.counts()
@@ -316,6 +319,7 @@ def bench_da(args, inp):
 
     plain = chain(False)
     da = chain(True)
+    control = chain(False, seed=43)  # the Monte Carlo spread of the medians
     S = args.da_steps
 
     def ess_per_s(r):
@@ -326,8 +330,10 @@ def bench_da(args, inp):
     ed, td = ess_per_s(da)
     flat_p = plain["chain"].reshape(-1, nk).astype(np.float64)
     flat_d = da["chain"].reshape(-1, nk).astype(np.float64)
+    flat_c = control["chain"].reshape(-1, nk).astype(np.float64)
     sig = flat_p.std(axis=0)
     dmed = (np.median(flat_d, axis=0) - np.median(flat_p, axis=0)) / sig
+    dctl = (np.median(flat_c, axis=0) - np.median(flat_p, axis=0)) / sig
     return {
         "workload": "1D P(D) stretch move, nflux=%d, %d walkers, %d steps "
                     "per chain, binned %dx%d map; surrogate MLP %d -> "
@@ -353,6 +359,11 @@ def bench_da(args, inp):
         "step_time_ratio": plain["wall"] / da["wall"],
         "median_shift_sigma": dmed.tolist(),
         "max_abs_median_shift_sigma": float(np.max(np.abs(dmed))),
+        # a second plain chain with another seed: how far two exact chains
+        # of this length differ in their medians
+        "control_median_shift_sigma": dctl.tolist(),
+        "control_max_abs_median_shift_sigma": float(np.max(np.abs(dctl))),
+        "control_tau_mean": float(np.nanmean(control["taus"])),
     }
 
 
