@@ -413,6 +413,7 @@ int pofd_acor(const float *x, uint32_t n, double *mean, double *sigma,
    likelihood runs only for the proposals the first stage passes.
    ====================================================================== */
 typedef struct pofd_mlp pofd_mlp;
+/* nin <= 64 inputs, hidden <= 512 units */
 int pofd_mlp_create(pofd_mlp **out, uint32_t nin, uint32_t hidden,
                     uint64_t seed);
 void pofd_mlp_destroy(pofd_mlp *mlp);
@@ -424,8 +425,11 @@ int pofd_mlp_nparams(const pofd_mlp *mlp, uint32_t *n);
 int pofd_mlp_layout(const pofd_mlp *mlp, uint64_t *offsets);
 int pofd_mlp_get_params(pofd_mlp *mlp, float *w);
 int pofd_mlp_set_params(pofd_mlp *mlp, const float *w);
-/* inputs x are standardised as (x - xmean)/xstd, the prediction is
-   out*ystd + ymean */
+/* inputs x are whitened as z = W (x - xmean), W row-major nin x nin, before
+   their bfloat16 rounding; the prediction is out*ystd + ymean */
+int pofd_mlp_set_whitening(pofd_mlp *mlp, const float *xmean,
+                           const float *wmat, double ymean, double ystd);
+/* the diagonal case: z_i = (x_i - xmean_i) / xstd_i */
 int pofd_mlp_set_standardization(pofd_mlp *mlp, const float *xmean,
                                  const float *xstd, double ymean,
                                  double ystd);
@@ -435,9 +439,11 @@ int pofd_mlp_predict(pofd_mlp *mlp, const float *x, uint32_t n, double *out);
    respect to every parameter (grad[nparams]) and the loss, for one batch */
 int pofd_mlp_loss_grad(pofd_mlp *mlp, const float *x, const double *y,
                        uint32_t n, float *grad, double *loss);
-/* Adam training: standardisation from the training rows, the last
-   holdout_frac of the rows (after one shuffle) held out; RMS errors of the
-   trained network in log-likelihood units */
+/* Adam training (step size decayed along a cosine to 5% of lr): the
+   whitening from the mean and covariance of the training rows (W = L^-1,
+   L L^T the covariance), targets standardised, the last holdout_frac of the
+   rows (after one shuffle) held out; RMS errors of the trained network in
+   log-likelihood units */
 int pofd_mlp_train(pofd_mlp *mlp, const float *x, const double *y,
                    uint32_t n, uint32_t epochs, uint32_t batch, float lr,
                    float holdout_frac, uint64_t seed, double *train_rms,

@@ -64,6 +64,14 @@ def standardize(x, xmean, xstd):
         np.float64)
 
 
+def whiten(x, xmean, wmat):
+    """W (x - mean) for each row: the difference in float32 as the GPU
+    forms it, the product in float64 (the GPU accumulates it in float32)."""
+    d = (np.asarray(x, dtype=np.float32) -
+         np.asarray(xmean, dtype=np.float32)).astype(np.float64)
+    return d @ np.asarray(wmat, dtype=np.float32).astype(np.float64).T
+
+
 def forward(w, offsets, nin, H, xs, emulate=True):
     """Standardised prediction of standardised inputs xs[n x nin] and the
     intermediates the backward pass needs."""

@@ -103,6 +103,7 @@ _SIGS = {
     "pofd_mlp_get_params": (C.c_int, [VP, FP]),
     "pofd_mlp_set_params": (C.c_int, [VP, FP]),
     "pofd_mlp_set_standardization": (C.c_int, [VP, FP, FP, D, D]),
+    "pofd_mlp_set_whitening": (C.c_int, [VP, FP, FP, D, D]),
     "pofd_mlp_predict": (C.c_int, [VP, FP, C.c_uint32, DP]),
     "pofd_mlp_loss_grad": (C.c_int, [VP, FP, DP, C.c_uint32, FP, DP]),
     "pofd_mlp_train": (C.c_int, [VP, FP, DP, C.c_uint32, C.c_uint32,
@@ -794,6 +795,14 @@ class Mlp:
         self._check(lib().pofd_mlp_set_standardization(
             self.ptr, m.ctypes.data_as(FP), s.ctypes.data_as(FP),
             float(ymean), float(ystd)), "pofd_mlp_set_standardization")
+
+    def set_whitening(self, xmean, wmat, ymean, ystd):
+        m = np.ascontiguousarray(xmean, dtype=np.float32)
+        w = np.ascontiguousarray(wmat, dtype=np.float32).reshape(self.nin,
+                                                                 self.nin)
+        self._check(lib().pofd_mlp_set_whitening(
+            self.ptr, m.ctypes.data_as(FP), w.ctypes.data_as(FP),
+            float(ymean), float(ystd)), "pofd_mlp_set_whitening")
 
     def predict(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.nin)

@@ -28,8 +28,8 @@ struct pofd_mlp {
   float* g = nullptr;   // gradient
   float* m = nullptr;   // Adam moments
   float* v = nullptr;
-  float* xmean = nullptr;  // standardisation (device)
-  float* xistd = nullptr;
+  float* xmean = nullptr;  // input whitening (device): z = wz (x - xmean)
+  float* xw = nullptr;     // nin x nin, row-major
   int* idx = nullptr;      // identity column map for host inputs
   double ymean = 0.0, ystd = 1.0;
   // batch scratch
@@ -50,7 +50,7 @@ struct pofd_mlp {
   std::string err;
   ~pofd_mlp() {
     if (stream) (void)hipStreamSynchronize(stream);
-    void* ps[] = {w,   wb,  g, m, v, xmean, xistd, idx, X, Z[0], Z[1], Z[2], A[0], A[1], A[2],
+    void* ps[] = {w,   wb,  g, m, v, xmean, xw, idx, X, Z[0], Z[1], Z[2], A[0], A[1], A[2],
                   out, y,   dout, se, dA, dA2, raw, dpred};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -140,7 +140,7 @@ int upload_inputs(pofd_mlp* s, const float* x, int n) {
   if (hipMemcpyAsync(s->raw, x, static_cast<size_t>(n) * s->nin * sizeof(float),
                      hipMemcpyHostToDevice, s->stream) != hipSuccess)
     return 1;
-  return md::gather_inputs(s->raw, s->nin, s->idx, s->xmean, s->xistd, s->nin, n, s->X,
+  return md::gather_inputs(s->raw, s->nin, s->idx, s->xmean, s->xw, s->nin, n, s->X,
                            s->stream);
 }
 
@@ -169,7 +169,7 @@ int pofd_mlp_predict_device(pofd_mlp* s, const float* src, int stride, const int
     lay.w[l] = s->off[2 * l];
     lay.b[l] = s->off[2 * l + 1];
   }
-  return md::infer(src, stride, idx, s->xmean, s->xistd, s->nin, s->H, rows, s->wb, s->w, lay,
+  return md::infer(src, stride, idx, s->xmean, s->xw, s->nin, s->H, rows, s->wb, s->w, lay,
                    s->ymean, s->ystd, out, st);
 }
 
@@ -195,7 +195,7 @@ int pofd_mlp_create(pofd_mlp** out, uint32_t nin, uint32_t hidden, uint64_t seed
   s->npar = o;
   if (hipStreamCreate(&s->stream) != hipSuccess ||
       (dalloc(&s->w, o) | dalloc(&s->wb, o) | dalloc(&s->g, o) | dalloc(&s->m, o) |
-       dalloc(&s->v, o) | dalloc(&s->xmean, I) | dalloc(&s->xistd, I) | dalloc(&s->idx, I))) {
+       dalloc(&s->v, o) | dalloc(&s->xmean, I) | dalloc(&s->xw, I * I) | dalloc(&s->idx, I))) {
     delete s;
     return POFD_E_INTERNAL;
   }
@@ -208,7 +208,8 @@ int pofd_mlp_create(pofd_mlp** out, uint32_t nin, uint32_t hidden, uint64_t seed
     const size_t n = (l == 0) ? H * I : (l == 3 ? H : H * H);
     for (size_t i = 0; i < n; ++i) w[s->off[2 * l] + i] = static_cast<float>(nd(rng));
   }
-  std::vector<float> zeros(o, 0.0f), mean(I, 0.0f), istd(I, 1.0f);
+  std::vector<float> zeros(o, 0.0f), mean(I, 0.0f), eye(I * I, 0.0f);
+  for (size_t i = 0; i < I; ++i) eye[i * I + i] = 1.0f;
   std::vector<int> idx(I);
   std::iota(idx.begin(), idx.end(), 0);
   hipError_t he = hipMemcpy(s->w, w.data(), o * sizeof(float), hipMemcpyHostToDevice);
@@ -216,7 +217,7 @@ int pofd_mlp_create(pofd_mlp** out, uint32_t nin, uint32_t hidden, uint64_t seed
   if (he == hipSuccess) he = hipMemcpy(s->v, zeros.data(), o * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(s->g, zeros.data(), o * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(s->xmean, mean.data(), I * sizeof(float), hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = hipMemcpy(s->xistd, istd.data(), I * sizeof(float), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(s->xw, eye.data(), I * I * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(s->idx, idx.data(), I * sizeof(int), hipMemcpyHostToDevice);
   if (he != hipSuccess || md::to_bf16(s->w, s->wb, o, s->stream) ||
       hipStreamSynchronize(s->stream) != hipSuccess) {
@@ -266,21 +267,31 @@ int pofd_mlp_set_params(pofd_mlp* s, const float* w) {
   return POFD_OK;
 }
 
-int pofd_mlp_set_standardization(pofd_mlp* s, const float* xmean, const float* xstd,
-                                 double ymean, double ystd) {
-  if (!s || !xmean || !xstd || !(ystd > 0.0)) return POFD_E_ARG;
-  std::vector<float> istd(s->nin);
-  for (int i = 0; i < s->nin; ++i) {
-    if (!(xstd[i] > 0.0f)) return mfail(s, POFD_E_ARG, "pofd_mlp_set_standardization: std <= 0");
-    istd[i] = 1.0f / xstd[i];
-  }
-  if (hipMemcpy(s->xmean, xmean, s->nin * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->xistd, istd.data(), s->nin * sizeof(float), hipMemcpyHostToDevice) !=
-          hipSuccess)
+int pofd_mlp_set_whitening(pofd_mlp* s, const float* xmean, const float* wmat, double ymean,
+                           double ystd) {
+  if (!s || !xmean || !wmat || !(ystd > 0.0)) return POFD_E_ARG;
+  const size_t I = s->nin;
+  for (size_t i = 0; i < I * I; ++i)
+    if (!std::isfinite(wmat[i]))
+      return mfail(s, POFD_E_ARG, "pofd_mlp_set_whitening: non-finite matrix");
+  if (hipStreamSynchronize(s->stream) != hipSuccess ||
+      hipMemcpy(s->xmean, xmean, I * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->xw, wmat, I * I * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return POFD_E_INTERNAL;
   s->ymean = ymean;
   s->ystd = ystd;
   return POFD_OK;
+}
+
+int pofd_mlp_set_standardization(pofd_mlp* s, const float* xmean, const float* xstd,
+                                 double ymean, double ystd) {
+  if (!s || !xmean || !xstd || !(ystd > 0.0)) return POFD_E_ARG;
+  std::vector<float> wz(static_cast<size_t>(s->nin) * s->nin, 0.0f);
+  for (int i = 0; i < s->nin; ++i) {
+    if (!(xstd[i] > 0.0f)) return mfail(s, POFD_E_ARG, "pofd_mlp_set_standardization: std <= 0");
+    wz[static_cast<size_t>(i) * s->nin + i] = 1.0f / xstd[i];
+  }
+  return pofd_mlp_set_whitening(s, xmean, wz.data(), ymean, ystd);
 }
 
 int pofd_mlp_predict(pofd_mlp* s, const float* x, uint32_t n, double* out) {
@@ -333,8 +344,12 @@ int pofd_mlp_train(pofd_mlp* s, const float* x, const double* y, uint32_t n, uin
   std::shuffle(perm.begin(), perm.end(), rng);
   const uint32_t nho = static_cast<uint32_t>(holdout_frac * n);
   const uint32_t ntr = n - nho;
-  // standardisation from the training rows
-  std::vector<double> mu(nin, 0.0), var(nin, 0.0);
+  // whitening from the training rows: z = L^-1 (x - mean) with L L^T the
+  // sample covariance (Cholesky), so that the inputs are uncorrelated with
+  // unit variance before their bfloat16 rounding -- strongly correlated
+  // parameters would otherwise reach the network as small differences of
+  // rounded numbers
+  std::vector<double> mu(nin, 0.0), cov(static_cast<size_t>(nin) * nin, 0.0);
   double ym = 0.0, yv = 0.0;
   for (uint32_t r = 0; r < ntr; ++r) {
     const uint32_t i = perm[r];
@@ -345,45 +360,125 @@ int pofd_mlp_train(pofd_mlp* s, const float* x, const double* y, uint32_t n, uin
   ym /= ntr;
   for (uint32_t r = 0; r < ntr; ++r) {
     const uint32_t i = perm[r];
-    for (int c = 0; c < nin; ++c) {
-      const double d = x[static_cast<size_t>(i) * nin + c] - mu[c];
-      var[c] += d * d;
-    }
+    const float* xr = x + static_cast<size_t>(i) * nin;
+    for (int a = 0; a < nin; ++a)
+      for (int b = 0; b <= a; ++b) cov[a * nin + b] += (xr[a] - mu[a]) * (xr[b] - mu[b]);
     yv += (y[i] - ym) * (y[i] - ym);
   }
-  std::vector<float> xm(nin), xs(nin);
-  for (int c = 0; c < nin; ++c) {
-    xm[c] = static_cast<float>(mu[c]);
-    const double sd = std::sqrt(var[c] / ntr);
-    xs[c] = static_cast<float>(sd > 0 ? sd : 1.0);
+  std::vector<float> xm(nin), wz(static_cast<size_t>(nin) * nin, 0.0f);
+  for (int c = 0; c < nin; ++c) xm[c] = static_cast<float>(mu[c]);
+  {
+    double tr = 0.0;
+    for (int a = 0; a < nin; ++a) tr += cov[a * nin + a] / ntr;
+    const double jitter = 1e-12 * (tr > 0 ? tr / nin : 1.0);
+    std::vector<double> L(static_cast<size_t>(nin) * nin, 0.0);
+    bool ok = true;
+    for (int a = 0; a < nin && ok; ++a) {
+      for (int b = 0; b <= a; ++b) {
+        double v = cov[a * nin + b] / ntr + (a == b ? jitter : 0.0);
+        for (int k = 0; k < b; ++k) v -= L[a * nin + k] * L[b * nin + k];
+        if (a == b) {
+          if (!(v > 0.0)) {
+            ok = false;
+            break;
+          }
+          L[a * nin + a] = std::sqrt(v);
+        } else {
+          L[a * nin + b] = v / L[b * nin + b];
+        }
+      }
+    }
+    if (ok) {
+      // wz = L^-1 by forward substitution, column by column
+      std::vector<double> Li(static_cast<size_t>(nin) * nin, 0.0);
+      for (int c = 0; c < nin; ++c)
+        for (int a = c; a < nin; ++a) {
+          double v = (a == c) ? 1.0 : 0.0;
+          for (int k = c; k < a; ++k) v -= L[a * nin + k] * Li[k * nin + c];
+          Li[a * nin + c] = v / L[a * nin + a];
+        }
+      for (size_t i = 0; i < Li.size(); ++i) wz[i] = static_cast<float>(Li[i]);
+    } else {  // a degenerate input: per-input standardisation
+      for (int c = 0; c < nin; ++c) {
+        const double sd = std::sqrt(cov[c * nin + c] / ntr);
+        wz[c * nin + c] = static_cast<float>(sd > 0 ? 1.0 / sd : 1.0);
+      }
+    }
   }
   const double ysd = std::sqrt(yv / ntr);
-  int rc = pofd_mlp_set_standardization(s, xm.data(), xs.data(), ym, ysd > 0 ? ysd : 1.0);
+  int rc = pofd_mlp_set_whitening(s, xm.data(), wz.data(), ym, ysd > 0 ? ysd : 1.0);
   if (rc) return rc;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  std::vector<float> bx(static_cast<size_t>(batch) * nin);
-  std::vector<double> by(batch);
-  std::vector<uint32_t> order(perm.begin(), perm.begin() + ntr);
+  // the training rows stay on the device; each epoch uploads one shuffled
+  // order and the batches are gathered there
+  std::vector<float> hx(static_cast<size_t>(ntr) * nin), hy(ntr);
+  for (uint32_t r = 0; r < ntr; ++r) {
+    const uint32_t i = perm[r];
+    std::memcpy(&hx[static_cast<size_t>(r) * nin], &x[static_cast<size_t>(i) * nin],
+                nin * sizeof(float));
+    hy[r] = static_cast<float>((y[i] - s->ymean) / s->ystd);
+  }
+  float* tx = nullptr;
+  float* ty = nullptr;
+  int* dorder = nullptr;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(s->stream);
+    if (tx) (void)hipFree(tx);
+    if (ty) (void)hipFree(ty);
+    if (dorder) (void)hipFree(dorder);
+  };
+  std::vector<int> order(ntr);
+  std::iota(order.begin(), order.end(), 0);
+  if (ensure_cap(s, static_cast<int>(std::min(batch, ntr))) || dalloc(&tx, hx.size()) ||
+      dalloc(&ty, ntr) || dalloc(&dorder, static_cast<size_t>(ntr) * epochs) ||
+      hipMemcpyAsync(tx, hx.data(), hx.size() * sizeof(float), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess ||
+      hipMemcpyAsync(ty, hy.data(), ntr * sizeof(float), hipMemcpyHostToDevice, s->stream) !=
+          hipSuccess) {
+    release();
+    return mfail(s, POFD_E_INTERNAL, "pofd_mlp_train: device allocation failed");
+  }
+  // all epochs' orders up front (one copy, no reuse of a host buffer while
+  // a copy may still read it)
+  std::vector<int> orders(static_cast<size_t>(ntr) * epochs);
   for (uint32_t ep = 0; ep < epochs; ++ep) {
     std::shuffle(order.begin(), order.end(), rng);
-    for (uint32_t b0 = 0; b0 < ntr; b0 += batch) {
+    std::copy(order.begin(), order.end(), orders.begin() + static_cast<size_t>(ep) * ntr);
+  }
+  if (epochs > 0 &&
+      hipMemcpyAsync(dorder, orders.data(), orders.size() * sizeof(int), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess) {
+    release();
+    return mfail(s, POFD_E_INTERNAL, "pofd_mlp_train: order upload failed");
+  }
+  // cosine decay of the step size to 5% of lr over the run
+  const uint32_t nbatch = (ntr + batch - 1) / batch;
+  const double total = static_cast<double>(nbatch) * epochs;
+  uint64_t it = 0;
+  for (uint32_t ep = 0; ep < epochs; ++ep) {
+    for (uint32_t b0 = 0; b0 < ntr; b0 += batch, ++it) {
       const uint32_t nb = std::min(batch, ntr - b0);
-      for (uint32_t r = 0; r < nb; ++r) {
-        const uint32_t i = order[b0 + r];
-        std::memcpy(&bx[static_cast<size_t>(r) * nin], &x[static_cast<size_t>(i) * nin],
-                    nin * sizeof(float));
-        by[r] = y[i];
-      }
-      if (upload_inputs(s, bx.data(), nb) || upload_targets(s, by.data(), nb) ||
-          forward(s, nb) || backward(s, nb))
+      if (md::gather_rows(tx, ty, dorder + static_cast<size_t>(ep) * ntr, static_cast<int>(b0),
+                          static_cast<int>(nb), nin, s->xmean, s->xw, s->X, s->y,
+                          s->stream) ||
+          forward(s, nb) || backward(s, nb)) {
+        release();
         return mfail(s, POFD_E_INTERNAL, "pofd_mlp_train: device failure");
+      }
       ++s->adam_t;
       const float c1 = 1.0f / (1.0f - std::pow(b1, static_cast<float>(s->adam_t)));
       const float c2 = 1.0f / (1.0f - std::pow(b2, static_cast<float>(s->adam_t)));
-      if (md::adam(s->w, s->g, s->m, s->v, s->wb, s->npar, lr, b1, b2, eps, c1, c2, s->stream))
+      const double frac = static_cast<double>(it) / total;
+      const float lrt =
+          static_cast<float>(lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * frac))));
+      if (md::adam(s->w, s->g, s->m, s->v, s->wb, s->npar, lrt, b1, b2, eps, c1, c2,
+                   s->stream)) {
+        release();
         return mfail(s, POFD_E_INTERNAL, "pofd_mlp_train: adam launch failed");
+      }
     }
   }
+  release();
   // errors in log-likelihood units
   auto rms = [&](uint32_t first, uint32_t count, double* res) -> int {
     if (!res) return 0;

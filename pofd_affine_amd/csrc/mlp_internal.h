@@ -22,9 +22,14 @@ int gelu_fwd(const float* z, bf16* out, size_t n, hipStream_t st);
 int gelu_bwd(const float* z, float* dA, size_t n, hipStream_t st);
 // float -> bf16 (round to nearest even)
 int to_bf16(const float* in, bf16* out, size_t n, hipStream_t st);
-// standardised inputs: x[r][i] = (src[r*stride + idx[i]] - mean[i]) / std[i], as bf16
+// whitened inputs: x[r][i] = sum_j wz[i][j] (src[r*stride + idx[j]] - mean[j]),
+// as bf16 (wz row-major nin x nin; diagonal for plain standardisation)
 int gather_inputs(const float* src, int stride, const int* idx, const float* mean,
-                  const float* istd, int nin, int rows, bf16* x, hipStream_t st);
+                  const float* wz, int nin, int rows, bf16* x, hipStream_t st);
+// training batch from a device-resident set: x[r] = whitened
+// tx[order[b0 + r]] (bf16), y[r] = ty[order[b0 + r]]
+int gather_rows(const float* tx, const float* ty, const int* order, int b0, int nb, int nin,
+                const float* mean, const float* wz, bf16* x, float* y, hipStream_t st);
 // column sums: g[n] = sum_m dz[m][n]
 int colsum(const float* dz, int M, int N, float* g, hipStream_t st);
 // output-layer loss gradient: d[m] = 2 (pred[m] - y[m]) / M, and the
@@ -40,8 +45,8 @@ struct InferLayout {
   size_t w[4], b[4];
 };
 // fused prediction of rows inputs gathered from src (stride floats per row,
-// inputs at idx): out[r] = net(standardised x_r) * ys + ym
-int infer(const float* src, int stride, const int* idx, const float* mean, const float* istd,
+// inputs at idx): out[r] = net(whitened x_r) * ys + ym
+int infer(const float* src, int stride, const int* idx, const float* mean, const float* wz,
           int nin, int H, int rows, const bf16* wb, const float* w, const InferLayout& lay,
           double ym, double ys, double* out, hipStream_t st);
 

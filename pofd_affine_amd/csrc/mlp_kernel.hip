@@ -22,6 +22,16 @@ __device__ __forceinline__ bf16 f2bf(float f) {
   return static_cast<bf16>(u >> 16);
 }
 
+// whitened input c of one row: sum_j wz[c][j] (x_j - mean_j), float, j in
+// order (a diagonal wz gives wz[c][c] (x_c - mean_c) exactly)
+template <typename F>
+__device__ __forceinline__ float whiten(F xj, const float* mean, const float* wz, int nin,
+                                        int c) {
+  float acc = 0.0f;
+  for (int j = 0; j < nin; ++j) acc = fmaf(wz[c * nin + j], xj(j) - mean[j], acc);
+  return acc;
+}
+
 template <int T>
 __device__ __forceinline__ float ld(const void* p, size_t i) {
   if (T == 0) return bf2f(static_cast<const bf16*>(p)[i]);
@@ -104,11 +114,27 @@ __global__ void to_bf16_kernel(const float* in, bf16* out, size_t n) {
 }
 
 __global__ void gather_kernel(const float* src, int stride, const int* idx, const float* mean,
-                              const float* istd, int nin, int rows, bf16* x) {
+                              const float* wz, int nin, int rows, bf16* x) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= rows * nin) return;
   const int r = t / nin, i = t % nin;
-  x[t] = f2bf((src[static_cast<size_t>(r) * stride + idx[i]] - mean[i]) * istd[i]);
+  const float* row = src + static_cast<size_t>(r) * stride;
+  x[t] = f2bf(whiten([&](int j) { return row[idx[j]]; }, mean, wz, nin, i));
+}
+
+// a training batch: rows order[b0..b0+nb) of the device-resident set
+__global__ void gather_rows_kernel(const float* tx, const float* ty, const int* order, int b0,
+                                   int nb, int nin, const float* mean, const float* wz,
+                                   bf16* x, float* y) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * (nin + 1)) return;
+  const int r = t / (nin + 1), c = t % (nin + 1);
+  const int row = order[b0 + r];
+  const float* xr = tx + static_cast<size_t>(row) * nin;
+  if (c < nin)
+    x[static_cast<size_t>(r) * nin + c] = f2bf(whiten([&](int j) { return xr[j]; }, mean, wz, nin, c));
+  else
+    y[r] = ty[row];
 }
 
 // column sums: a workgroup of 256 threads per 64 columns, 4 row phases
@@ -181,7 +207,7 @@ __device__ __forceinline__ short4v wfrag(const bf16* W, int j, int k, int H, int
 }
 
 __global__ __launch_bounds__(kInfThreads) void infer_kernel(
-    const float* src, int stride, const int* idx, const float* mean, const float* istd, int nin,
+    const float* src, int stride, const int* idx, const float* mean, const float* wz, int nin,
     int H, int rows, const bf16* wb, const float* w, InferLayout lay, double ym, double ys,
     double* out) {
   extern __shared__ bf16 act[];  // two buffers of kInfRows x ld bf16
@@ -196,7 +222,10 @@ __global__ __launch_bounds__(kInfThreads) void infer_kernel(
     const int i = e / Kin, c = e % Kin;
     const int r = r0 + i;
     float v = 0.0f;
-    if (c < nin && r < rows) v = (src[static_cast<size_t>(r) * stride + idx[c]] - mean[c]) * istd[c];
+    if (c < nin && r < rows) {
+      const float* row = src + static_cast<size_t>(r) * stride;
+      v = whiten([&](int j) { return row[idx[j]]; }, mean, wz, nin, c);
+    }
     bufA[i * ld + c] = f2bf(v);
   }
   __syncthreads();
@@ -295,10 +324,18 @@ int to_bf16(const float* in, bf16* out, size_t n, hipStream_t st) {
 }
 
 int gather_inputs(const float* src, int stride, const int* idx, const float* mean,
-                  const float* istd, int nin, int rows, bf16* x, hipStream_t st) {
+                  const float* wz, int nin, int rows, bf16* x, hipStream_t st) {
   const size_t n = static_cast<size_t>(rows) * nin;
   hipLaunchKernelGGL(gather_kernel, dim3(nb(n, 256)), dim3(256), 0, st, src, stride, idx, mean,
-                     istd, nin, rows, x);
+                     wz, nin, rows, x);
+  return ok();
+}
+
+int gather_rows(const float* tx, const float* ty, const int* order, int b0, int nrows, int nin,
+                const float* mean, const float* wz, bf16* x, float* y, hipStream_t st) {
+  const size_t n = static_cast<size_t>(nrows) * (nin + 1);
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(nb(n, 256)), dim3(256), 0, st, tx, ty, order, b0,
+                     nrows, nin, mean, wz, x, y);
   return ok();
 }
 
@@ -319,7 +356,7 @@ int adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float
   return ok();
 }
 
-int infer(const float* src, int stride, const int* idx, const float* mean, const float* istd,
+int infer(const float* src, int stride, const int* idx, const float* mean, const float* wz,
           int nin, int H, int rows, const bf16* wb, const float* w, const InferLayout& lay,
           double ym, double ys, double* out, hipStream_t st) {
   if (rows <= 0) return 0;
@@ -327,7 +364,7 @@ int infer(const float* src, int stride, const int* idx, const float* mean, const
   const int ld = (Hp > Kin ? Hp : Kin) + 4;
   const size_t lds = 2 * kInfRows * ld * sizeof(bf16);
   hipLaunchKernelGGL(infer_kernel, dim3((rows + kInfRows - 1) / kInfRows), dim3(kInfThreads), lds, st,
-                     src, stride, idx, mean, istd, nin, H, rows, wb, w, lay, ym, ys, out);
+                     src, stride, idx, mean, wz, nin, H, rows, wb, w, lay, ym, ys, out);
   return ok();
 }
 

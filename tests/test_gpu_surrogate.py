@@ -64,6 +64,14 @@ def test_loss_and_gradient_match_oracle(nin, H, M):
     out, _ = mlp_ref.forward(w, net.offsets, nin, H, xs)
     np.testing.assert_allclose(net.predict(x), out * ystd + ymean,
                                rtol=0, atol=1e-4 * ystd * np.max(np.abs(out)))
+    # a full whitening matrix in place of the diagonal standardisation
+    wmat = np.tril(rng.normal(size=(nin, nin))) + 2.0 * np.eye(nin)
+    net.set_whitening(xmean, wmat, ymean, ystd)
+    zs = mlp_ref.whiten(x, xmean, wmat)
+    outw, _ = mlp_ref.forward(w, net.offsets, nin, H, zs)
+    np.testing.assert_allclose(net.predict(x), outw * ystd + ymean, rtol=0,
+                               atol=2e-3 * ystd * np.max(np.abs(outw)))
+    net.set_standardization(xmean, xstd, ymean, ystd)
     # a plain gradient step lowers the loss
     net.set_params(w - 0.01 * g / scale)
     assert net.loss_grad(x, y)[0] < loss
@@ -91,6 +99,28 @@ def test_training_reaches_small_holdout_error():
     pred = net.predict(x[:512])
     assert np.sqrt(np.mean((pred - y[:512]) ** 2)) < 0.05 * y.std()
     assert loss0[0] > 0.5  # it did start untrained
+
+
+def test_training_whitens_correlated_inputs():
+    """Inputs correlated at 0.999 whose target depends on their small
+    difference: the training's whitening (Cholesky of the covariance) puts
+    that direction at unit scale before the bfloat16 rounding."""
+    _say("training on correlated inputs")
+    rng = np.random.default_rng(8)
+    n = 8192
+    u = rng.normal(size=n)
+    v = rng.normal(size=n)
+    a = 5.0 + u
+    b = 5.0 + u + 0.0447 * v          # corr(a, b) = 0.999
+    c = rng.normal(size=n)
+    x = np.stack([a, b, c], axis=1)
+    r = (b - a) / 0.0447               # the narrow direction, unit width
+    y = -0.5 * (r * r + c * c + u * u)
+    net = _lib.Mlp(3, 128, seed=4)
+    tr, ho = net.train(x.astype(np.float32), y, epochs=120, batch=256,
+                       lr=2e-3, holdout_frac=0.1, seed=9)
+    _say("train RMS %.4f holdout RMS %.4f (std y %.2f)" % (tr, ho, y.std()))
+    assert ho < 0.05 * y.std() and ho < 0.1
 
 
 def _gauss_da(seed, surrogate_mean_shift, surrogate_width, nwalk=64):

@@ -78,3 +78,19 @@ def test_forward_is_the_composed_network():
     want = vals[6] * a + vals[7]
     out, _ = mlp_ref.forward(w, off, 1, 1, x, emulate=False)
     assert abs(out[0] - want) < 1e-15
+
+
+def test_whiten_diagonal_is_standardize():
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(50, 4)).astype(np.float32)
+    m = rng.normal(size=4).astype(np.float32)
+    sd = (0.3 + rng.random(4)).astype(np.float32)
+    w = np.diag(np.float32(1.0) / sd)
+    # equal to float32 rounding (whiten keeps the product in float64)
+    np.testing.assert_allclose(mlp_ref.whiten(x, m, w),
+                               mlp_ref.standardize(x, m, sd), rtol=1e-7)
+    # a full matrix: the rows of W (x - m)
+    W = rng.normal(size=(4, 4))
+    z = mlp_ref.whiten(x, m, W)
+    d = x.astype(np.float64) - m
+    np.testing.assert_allclose(z[7], W.astype(np.float32) @ d[7], rtol=1e-6)
