@@ -185,11 +185,13 @@ __global__ void unstd_kernel(const float* pred, int n, double ym, double ys, dou
 // of the layer output (tiles w, w+16, ...) and accumulates them with
 // v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)..+3] and
 // B[4(l/16)..+3][l%16] and receives D[4(l/16)..+3][l%16].  The weight
-// fragments of four k-steps are loaded before their four products, so a
-// wave waits on L2 once per 64 inputs rather than once per 16.
+// fragments of sixteen k-steps are loaded before their products, so a wave
+// waits on memory once per 256 inputs (the weights are usually evicted
+// from L2 by the likelihood kernels between two predictions).
 typedef short short4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 constexpr int kInfRows = 16;
+constexpr int kMaxIn = 64;  // network inputs (pofd_mlp_create's limit)
 constexpr int kInfThreads = 1024;
 constexpr int kInfWaves = kInfThreads / 64;
 
@@ -218,14 +220,28 @@ __global__ __launch_bounds__(kInfThreads) void infer_kernel(
   const int ld = (Hp > Kin ? Hp : Kin) + 4;  // row pitch (bf16), 8-byte aligned
   bf16* bufA = act;
   bf16* bufB = act + kInfRows * ld;
+  // inputs: the column map, the whitening and the rows' raw values staged
+  // in LDS first (two dependent global reads in all), then whitened there
+  __shared__ int sidx[kMaxIn];
+  __shared__ float smean[kMaxIn];
+  __shared__ float swz[kMaxIn * kMaxIn];
+  __shared__ float sx[kInfRows * kMaxIn];
+  for (int e = tid; e < nin * nin; e += kInfThreads) swz[e] = wz[e];
+  if (tid < nin) {
+    sidx[tid] = idx[tid];
+    smean[tid] = mean[tid];
+  }
+  __syncthreads();
+  for (int e = tid; e < kInfRows * nin; e += kInfThreads) {
+    const int i = e / nin, j = e % nin;
+    const int r = r0 + i;
+    sx[e] = (r < rows) ? src[static_cast<size_t>(r) * stride + sidx[j]] : 0.0f;
+  }
+  __syncthreads();
   for (int e = tid; e < kInfRows * Kin; e += kInfThreads) {
     const int i = e / Kin, c = e % Kin;
-    const int r = r0 + i;
     float v = 0.0f;
-    if (c < nin && r < rows) {
-      const float* row = src + static_cast<size_t>(r) * stride;
-      v = whiten([&](int j) { return row[idx[j]]; }, mean, wz, nin, c);
-    }
+    if (c < nin && r0 + i < rows) v = whiten([&](int j) { return sx[i * nin + j]; }, smean, swz, nin, c);
     bufA[i * ld + c] = f2bf(v);
   }
   __syncthreads();
@@ -240,6 +256,18 @@ __global__ __launch_bounds__(kInfThreads) void infer_kernel(
       const int j = t * 16 + li;  // output column this lane feeds B for
       float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
       int k0 = 0;
+      // 256 inputs at a time: sixteen fragments in flight, one wait
+      for (; k0 + 256 <= Kp; k0 += 256) {
+        short4v bq[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) bq[u] = wfrag(W, j, k0 + 16 * u + lk, H, K);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const short4v a =
+              *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + 16 * u + lk);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u], acc, 0, 0, 0);
+        }
+      }
       for (; k0 + 64 <= Kp; k0 += 64) {
         short4v bq[4];
 #pragma unroll

@@ -213,9 +213,8 @@ def test_acor_ar1_closed_form():
     """affineChainSet::acor (Goodman's ACOR, src/affineChainSet.cc:880-950)
     on AR(1) series x_t = rho x_{t-1} + e_t, whose integrated
     autocorrelation time is (1 + rho)/(1 - rho) and whose mean has standard
-    error sqrt(tau var / n).  The reference's own fixture (testvec.txt) is
-    not among its test data, so the closed form pins it; short series are
-    refused (L < minfac * maxlag = 50)."""
+    error sqrt(tau var / n); short series are refused (L < minfac * maxlag
+    = 50).  The reference's own fixture is the next test."""
     rng = np.random.default_rng(12)
     for rho in (0.0, 0.5, 0.9, 0.98):
         n = 1 << 20
@@ -230,3 +229,18 @@ def test_acor_ar1_closed_form():
         assert abs(mean - x.mean()) < 1e-5 * (1 + abs(x.mean()))
     with pytest.raises(_lib.PofdError):
         _lib.acor(np.zeros(49))
+
+
+def test_acor_reference_vector():
+    """The reference's known answer (test/chain_test.cc:461-487): ACOR of
+    the 8815-value series testdata/testvec.txt read as floats gives
+    tau 130.052 +- 0.002, mean 5.82089 +- 0.0002, sigma 0.0134451 +- 0.0002
+    ("values computed using original acor code")."""
+    import os
+    from conftest import REFDATA
+    x = np.loadtxt(os.path.join(REFDATA, "testvec.txt"), dtype=np.float32)
+    assert x.size == 8815
+    tau, mean, sigma = _lib.acor(x)
+    assert abs(tau - 130.052) < 0.002, tau
+    assert abs(mean - 5.82089) < 0.0002, mean
+    assert abs(sigma - 0.0134451) < 0.0002, sigma
