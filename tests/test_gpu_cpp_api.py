@@ -129,3 +129,44 @@ def test_cpp_getlike2d(tmp_path):
         assert err == "error PDFactoryDouble::initPD"
         pd = [l for l in lines if l.startswith("pd")][0].split()
         assert int(pd[5]) == 512 and int(pd[6]) == 512
+
+
+EXE3 = os.path.join(ROOT, "tests", "cpp", "_build", "speclike_check")
+
+
+def test_cpp_getlike_from_spec_and_init_files(tmp_path):
+    """pofd_affine_getLike's inputs: a spec file (dataset, fftsize,
+    ninterp, binning, beam histogram) and an init file of knot positions
+    and values, read by include/pofd_affine_amd_io.hpp; the log likelihood
+    equals the oracle's for the same settings."""
+    print("[gpu-case] C++ getLike from spec and init files", flush=True)
+    assert os.path.exists(EXE3), "build tests/cpp first (__graft_entry__.build)"
+    inp = workload.make_inputs(npix_side=300, seed=5)
+    fitsio.write_image(str(tmp_path / "beam.fits"), inp["beam"].reshape(27, 27),
+                       extra={"PIXSIZE": inp["pixsize"]})
+    fitsio.write_image(str(tmp_path / "map.fits"),
+                       inp["data"].reshape(300, 300), extname="image")
+    th = synth.walker_thetas(2, seed=21)[:, :9]
+    knots = inp["knots"]
+    with open(tmp_path / "init.txt", "w") as f:
+        f.write("# knot position (Jy), log10 value\n")
+        for k, v in zip(knots, th[0][:len(knots)]):
+            f.write("%.9g\t%.9g\n" % (k, v))
+    with open(tmp_path / "spec.txt", "w") as f:
+        f.write("dataset= %s 0.0025 %s\n" % (tmp_path / "map.fits",
+                                             tmp_path / "beam.fits"))
+        f.write("fftsize= 32768\nninterp= 512\nbin_data= 1000\n"
+                "minbeamval= 1e-6 #comment\nbeam_histogram= yes\n"
+                "nbeamhist= 120\nmean_sub= no\n")
+    out = subprocess.run([EXE3, str(tmp_path / "spec.txt"),
+                          str(tmp_path / "init.txt"), "%.9g" % th[0][8]],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    line = [l for l in out.stdout.split("\n") if l.startswith("loglike")][0]
+    got, rej = float(line.split()[1]), int(line.split()[2])
+    assert rej == 0
+    L = build_oracle(inp, 32768, 512, True, th[0])
+    ref, rst = L.loglike_batch(np.hstack([th[:1], np.zeros((1, 2),
+                                                           np.float32)]), 1)
+    assert rst[0] == 0
+    assert got == pytest.approx(ref[0], abs=1e-6)
