@@ -176,6 +176,20 @@ int pofd1d_loglike_batch_device_count(pofd_ctx *ctx, const float *d_theta,
                                       uint32_t theta_stride,
                                       double *d_loglike_out,
                                       int32_t *d_status_out, void *stream);
+/* Priors and regularisation of calcLike::getLogLike (src/calcLike.cc:
+   923-952), added on the device to every batch's log likelihood after the
+   dataset sum (walkers whose datasets failed are left as they are):
+   POFD_PRIOR_SIGMA   -1/2 ln 2pi - ln w - ((sigma_mult - 1)/w)^2 / 2
+   POFD_PRIOR_CFIRB   the same in (cfirb_mean - int S n dS) / cfirb_sigma
+   POFD_PRIOR_POISSON the same in (poisson_mean - int S^2 n dS) / poisson_sigma
+   and, when reg_alpha > 0, differenceRegularize(reg_alpha)
+   (src/numberCountsKnotsSpline.cc:540-559).  flags 0 and reg_alpha 0 turn
+   them off (the default). */
+enum { POFD_PRIOR_SIGMA = 1, POFD_PRIOR_CFIRB = 2, POFD_PRIOR_POISSON = 4 };
+int pofd1d_set_priors(pofd_ctx *ctx, int flags, double sigma_width,
+                      double cfirb_mean, double cfirb_sigma,
+                      double poisson_mean, double poisson_sigma,
+                      double reg_alpha);
 /* Tuning, no reference counterpart: workgroups per walker of the forward
    and inverse transforms.  0 (the default) chooses by batch size,
    min(4, 256 / n_walkers) and at least 1, so that small batches (such as
@@ -455,6 +469,9 @@ int pofd_mlp_train(pofd_mlp *mlp, const float *x, const double *y,
 int pofd_ens_set_surrogate(pofd_ens *ens, pofd_mlp *mlp,
                            const uint32_t *param_index, uint32_t nidx,
                            double gate, double holdout_rms, int *enabled);
+/* zero the per-walker acceptance and rejection counters (the end of a
+   burn-in) */
+int pofd_ens_reset_counts(pofd_ens *ens);
 /* proposals made and exact likelihood evaluations since set_positions */
 int pofd_ens_counts(pofd_ens *ens, uint64_t *n_proposed, uint64_t *n_exact);
 

@@ -687,7 +687,6 @@ class calcLike {
      bonus (optional, 3 per walker): getNS, getFluxPerArea, getFluxSqPerArea */
   void getLogLikeBatch(const float* theta, unsigned int nwalk, unsigned int stride,
                        double* loglike, int* status, double* bonus = nullptr) const {
-    const double half_log_2pi = 0.918938517570495605469;
     if (sets.empty()) {
       for (unsigned w = 0; w < nwalk; ++w) loglike[w] = NAN;
       return;
@@ -697,40 +696,38 @@ class calcLike {
     if (nk == 0) throw affineExcept("calcLike", "getLogLike", "Model knot positions not loaded");
     if (nk > stride - 1) throw affineExcept("calcLike", "getLogLike", "Not enough elements in paramSet");
     if (!prepared) throw affineExcept("calcLikeSingle", "getNegLogLike", "R range not set");
-    int rc = pofd1d_loglike_batch(ctx.get(), theta, nwalk, stride, loglike,
-                                  reinterpret_cast<int32_t*>(status));
+    // the priors (src/calcLike.cc:923-952) are added on the device after
+    // the dataset sum, for walkers whose datasets did not fail
+    int rc = pushPriors();
+    if (!rc)
+      rc = pofd1d_loglike_batch(ctx.get(), theta, nwalk, stride, loglike,
+                                reinterpret_cast<int32_t*>(status));
     if (rc) pofd_amd_detail::throw_status(ctx.get(), rc, "calcLike", "getLogLike");
-    std::vector<double> bo;
-    const bool need_bonus = bonus || has_cfirb_prior || has_poisson_prior;
-    if (need_bonus) {
-      bo.resize(3 * static_cast<size_t>(nwalk));
-      rc = pofd1d_get_bonus(ctx.get(), nwalk, bo.data());
+    if (bonus) {
+      rc = pofd1d_get_bonus(ctx.get(), nwalk, bonus);
       if (rc) pofd_amd_detail::throw_status(ctx.get(), rc, "calcLike", "getLogLike");
-      if (bonus) std::copy(bo.begin(), bo.end(), bonus);
     }
-    for (unsigned w = 0; w < nwalk; ++w) {
-      // the priors follow the dataset sum unless it threw (bad_like does not)
-      if (status[w] != 0 && status[w] != POFD_E_BADLIKE) continue;
-      const float* t = theta + static_cast<size_t>(w) * stride;
-      const double sigmult = t[nk];
-      if (has_sigma_prior) {
-        const double val = (sigmult - 1.0) / sigma_prior_width;
-        loglike[w] -= half_log_2pi + log(sigma_prior_width) + 0.5 * val * val;
-      }
-      if (has_cfirb_prior) {  // src/calcLike.cc:932-939
-        const double val = (cfirb_prior_mean - bo[3 * w + 1]) / cfirb_prior_sigma;
-        loglike[w] -= half_log_2pi + log(cfirb_prior_sigma) + 0.5 * val * val;
-      }
-      if (has_poisson_prior) {  // :941-947
-        const double val = (poisson_prior_mean - bo[3 * w + 2]) / poisson_prior_sigma;
-        loglike[w] -= half_log_2pi + log(poisson_prior_sigma) + 0.5 * val * val;
-      }
-      if (regularization_alpha > 0.0) {
-        numberCountsKnotsSpline m = model;
-        m.setParams(paramSet(stride, t));
-        loglike[w] += m.differenceRegularize(regularization_alpha);
-      }
-    }
+  }
+
+  /* the prior settings onto the context (pofd1d_set_priors) */
+  int pushPriors() const {
+    if (!ctx) return POFD_E_ARG;
+    int flags = 0;
+    if (has_sigma_prior) flags |= POFD_PRIOR_SIGMA;
+    if (has_cfirb_prior) flags |= POFD_PRIOR_CFIRB;
+    if (has_poisson_prior) flags |= POFD_PRIOR_POISSON;
+    return pofd1d_set_priors(ctx.get(), flags, sigma_prior_width, cfirb_prior_mean,
+                             cfirb_prior_sigma, poisson_prior_mean, poisson_prior_sigma,
+                             regularization_alpha);
+  }
+
+  /* the device context (after setRRanges), for the sampler: its batches
+     include the priors set here */
+  pofd_ctx* context() const {
+    if (!prepared) throw affineExcept("calcLikeSingle", "getNegLogLike", "R range not set");
+    const int rc = pushPriors();
+    if (rc) pofd_amd_detail::throw_status(ctx.get(), rc, "calcLike", "context");
+    return ctx.get();
   }
 };
 

@@ -55,6 +55,7 @@ _SIGS = {
                                    C.POINTER(C.c_int)]),
     "pofd1d_prepare": (C.c_int, [VP]),
     "pofd1d_set_parts": (C.c_int, [VP, C.c_int]),
+    "pofd1d_set_priors": (C.c_int, [VP, C.c_int, D, D, D, D, D, D]),
     "pofd1d_loglike_batch_device_count": (C.c_int, [VP, VP, C.c_uint32, VP,
                                                     C.c_uint32, VP, VP, VP]),
     "pofd1d_set_rrange": (C.c_int, [VP, C.c_int, D, D]),
@@ -268,6 +269,25 @@ class Context:
     def set_spectral_cut(self, value):
         self._check(lib().pofd_set_option(self.ptr, POFD_OPT_SPECTRAL_CUT,
                                           float(value)), "pofd_set_option")
+
+    def set_priors(self, sigma_width=None, cfirb=None, poisson=None,
+                   reg_alpha=0.0):
+        """Priors added on the device to every batch (calcLike::getLogLike):
+        sigma_width, cfirb=(mean, sigma), poisson=(mean, sigma)."""
+        flags = 0
+        sw = cm = cs = pm = ps = 0.0
+        if sigma_width is not None:
+            flags |= 1
+            sw = sigma_width
+        if cfirb is not None:
+            flags |= 2
+            cm, cs = cfirb
+        if poisson is not None:
+            flags |= 4
+            pm, ps = poisson
+        self._check(lib().pofd1d_set_priors(self.ptr, flags, sw, cm, cs, pm,
+                                            ps, reg_alpha),
+                    "pofd1d_set_priors")
 
     def set_parts(self, parts):
         """Workgroups per walker of the 1D transforms (0: by batch size)."""

@@ -80,6 +80,7 @@ struct pofd_ctx {
   std::vector<std::unique_ptr<Beamset>> sets;
   uint32_t last_nwalk = 0;
   int force_parts = 0;  // workgroups per walker: 0 = by batch size
+  pofd::PriorDev priors{};  // added on the device after the beam sets
   std::unique_ptr<DevBuf> dtheta, dll, dst;
   size_t cap_theta = 0, cap_out = 0;
   // stage timing: one set of POFD_NSTAGES+1 events per beam-set launch
@@ -798,10 +799,38 @@ int batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwalk, const int32_
     if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: launch failed: " +
                                    std::string(hipGetErrorString(hipGetLastError())));
   }
+  if (c->priors.flags || c->priors.reg_alpha > 0.0) {
+    Beamset& s0 = *c->sets[0];
+    rc = pofd::launch_prior1d(s0.dev, s0.wk, c->priors, d_theta, stride, nwalk, d_count, d_ll,
+                              d_st, stream);
+    if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: prior launch failed");
+  }
   c->last_nwalk = nwalk;
   return POFD_OK;
 }
 }  // namespace
+
+// the counts integrals of the last batch (beam set 0's workspace; 4 per
+// batch row), for the sampler's bonus parameters; null before any batch
+const double* pofd1d_bonus_device(pofd_ctx* c) {
+  if (!c || c->sets.empty() || !c->sets[0]->wbuf) return nullptr;
+  return c->sets[0]->wk.bonus;
+}
+
+extern "C" int pofd1d_set_priors(pofd_ctx* c, int flags, double sigma_width, double cfirb_mean,
+                                 double cfirb_sigma, double poisson_mean, double poisson_sigma,
+                                 double reg_alpha) {
+  if (!c) return POFD_E_ARG;
+  if (flags & ~(pofd::kPriorSigma | pofd::kPriorCfirb | pofd::kPriorPoisson))
+    return fail(c, POFD_E_ARG, "pofd1d_set_priors: unknown prior flag");
+  if (((flags & pofd::kPriorSigma) && !(sigma_width > 0.0)) ||
+      ((flags & pofd::kPriorCfirb) && !(cfirb_sigma > 0.0)) ||
+      ((flags & pofd::kPriorPoisson) && !(poisson_sigma > 0.0)) || reg_alpha < 0.0)
+    return fail(c, POFD_E_ARG, "pofd1d_set_priors: invalid prior width or alpha");
+  c->priors = pofd::PriorDev{flags, sigma_width, cfirb_mean, cfirb_sigma, poisson_mean,
+                             poisson_sigma, reg_alpha};
+  return POFD_OK;
+}
 
 extern "C" int pofd1d_loglike_batch_device(pofd_ctx* c, const float* d_theta,
                                            uint32_t nwalk, uint32_t stride,

@@ -13,6 +13,9 @@
 
 using pofd_ens_dev::EnsDev;
 
+// counts integrals of the last 1D batch on the device (capi.cpp)
+const double* pofd1d_bonus_device(pofd_ctx* c);
+
 // surrogate predictions on the device (mlp.cpp)
 int pofd_mlp_predict_device(pofd_mlp* s, const float* src, int stride, const int* idx, int rows,
                             double* out, hipStream_t st);
@@ -76,8 +79,11 @@ void update_dexp(pofd_ens* s) {
 // evaluate n parameter rows x (device) into ll / st (device) on the stream
 int evaluate(pofd_ens* s, const float* x, uint32_t n, double* ll, int32_t* st) {
   switch (s->target) {
-    case POFD_TARGET_1D:
-      return pofd1d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
+    case POFD_TARGET_1D: {
+      const int rc = pofd1d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
+      s->e.bonus_src = pofd1d_bonus_device(s->ctx);  // the workspace may have moved
+      return rc;
+    }
     case POFD_TARGET_2D:
       return pofd2d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
     case POFD_TARGET_GAUSS:
@@ -112,6 +118,7 @@ int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int stor
     int rc = pofd1d_loglike_batch_device_count(s->ctx, s->e.cprop, hw, s->e.npass,
                                                s->npar, s->e.cll, s->e.cst, s->stream);
     if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+    s->e.bonus_src = pofd1d_bonus_device(s->ctx);
     if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, s->stream))
       return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
     if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))
@@ -259,10 +266,13 @@ int pofd_ens_set_positions(pofd_ens* s, const float* theta) {
     if (hst[w] != POFD_OK && hst[w] != POFD_E_BADLIKE)
       return efail(s, hst[w], "pofd_ens_set_positions: initial walker " + std::to_string(w) +
                                   " has no valid likelihood");
+  if (pofd_ens_dev::launch_bonus_init(s->e, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_set_positions: bonus launch failed");
   (void)hipMemset(s->e.naccept, 0, W * sizeof(uint32_t));
   (void)hipMemset(s->e.nerr, 0, W * sizeof(uint32_t));
   s->nstored = 0;
-  s->step = 0;
+  // the step counter (the random streams' index) keeps running, so new
+  // positions never reuse an earlier step's draws
   s->n_exact = s->n_proposed = 0;
   if (s->e.nexact) (void)hipMemset(s->e.nexact, 0, sizeof(unsigned long long));
   s->positioned = true;
@@ -315,6 +325,16 @@ int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_ind
   s->use_da = true;
   if (enabled) *enabled = 1;
   return refresh_surrogate(s);
+}
+
+int pofd_ens_reset_counts(pofd_ens* s) {
+  if (!s) return POFD_E_ARG;
+  const size_t W = s->nwalk;
+  hipError_t he = hipStreamSynchronize(s->stream);
+  if (he == hipSuccess) he = hipMemset(s->e.naccept, 0, W * sizeof(uint32_t));
+  if (he == hipSuccess) he = hipMemset(s->e.nerr, 0, W * sizeof(uint32_t));
+  if (he != hipSuccess) return ehip(s, he, "hipMemset(counts)");
+  return POFD_OK;
 }
 
 int pofd_ens_counts(pofd_ens* s, uint64_t* n_proposed, uint64_t* n_exact) {

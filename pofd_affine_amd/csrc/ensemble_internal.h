@@ -42,6 +42,10 @@ struct EnsDev {
   double* cll;       // nwalk/2        their exact log likelihoods
   int32_t* cst;      // nwalk/2
   unsigned long long* nexact;  // 1: exact evaluations requested by stage 1
+  // counts integrals of the last likelihood batch (4 per batch row: N, int S n,
+  // int S^2 n): the first two bonus parameters of an accepted step take the
+  // mean flux and flux^2 per area (pofdMCMC::fillBonusParams); null: NaN
+  const double* bonus_src;
 };
 
 int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st);
@@ -58,5 +62,7 @@ int launch_compact(const EnsDev& e, hipStream_t st);
 int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
                   hipStream_t st);
 int launch_store_half(const EnsDev& e, uint32_t half, uint32_t slot, hipStream_t st);
+// bonus parameters of all walkers from the batch that evaluated them in order
+int launch_bonus_init(const EnsDev& e, hipStream_t st);
 
 }  // namespace pofd_ens_dev

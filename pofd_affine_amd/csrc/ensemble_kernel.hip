@@ -85,6 +85,18 @@ __device__ __forceinline__ bool params_valid(const EnsDev& e, const float* p) {
   return true;
 }
 
+// the bonus parameters of walker position x from batch row r
+__device__ __forceinline__ void fill_bonus(const EnsDev& e, float* x, int r) {
+  int nb = 0;
+  for (int i = 0; i < e.npar; ++i) {
+    if (e.state[i] != kStateBonus) continue;
+    x[i] = (e.bonus_src && nb < 2)
+               ? static_cast<float>(e.bonus_src[static_cast<size_t>(r) * 4 + 1 + nb])
+               : __int_as_float(0x7fc00000);
+    ++nb;
+  }
+}
+
 }  // namespace
 
 // one thread per walker of half `half`
@@ -170,6 +182,7 @@ __global__ __launch_bounds__(64) void accept_kernel(const EnsDev e, const uint32
   }
   if (accept) {
     for (int i = 0; i < e.npar; ++i) xo[i] = y[i];
+    fill_bonus(e, xo, t);
     e.ll[k] = e.llp[t];
     e.naccept[k] += 1;
   }
@@ -271,9 +284,15 @@ __global__ __launch_bounds__(64) void stage2_kernel(const EnsDev e, const uint32
   float* xo = e.pos + static_cast<size_t>(k) * e.npar;
   const float* y = e.prop + static_cast<size_t>(t) * e.npar;
   for (int i = 0; i < e.npar; ++i) xo[i] = y[i];
+  fill_bonus(e, xo, c);
   e.ll[k] = lnew;
   e.llsur[k] = e.llsp[t];
   e.naccept[k] += 1;
+}
+
+__global__ __launch_bounds__(64) void bonus_init_kernel(const EnsDev e) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < e.nwalk) fill_bonus(e, e.pos + static_cast<size_t>(k) * e.npar, k);
 }
 
 __global__ __launch_bounds__(64) void store_half_kernel(const EnsDev e, const uint32_t half,
@@ -343,6 +362,11 @@ int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp
   const int hw = e.nwalk / 2;
   hipLaunchKernelGGL(stage2_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
                      inv_temp);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_bonus_init(const EnsDev& e, hipStream_t st) {
+  hipLaunchKernelGGL(bonus_init_kernel, dim3((e.nwalk + 63) / 64), dim3(64), 0, st, e);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

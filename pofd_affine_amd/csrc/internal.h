@@ -157,6 +157,19 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
                      const float* d_theta, uint32_t nwalk, const int32_t* d_count,
                      uint32_t stride, double* d_loglike, int32_t* d_status,
                      int accumulate, int force_parts, void* stream, hipEvent_t* ev);
+// priors and regularisation added to the summed log likelihood
+// (calcLike::getLogLike, src/calcLike.cc:923-952); flags: kPriorSigma,
+// kPriorCfirb, kPriorPoisson; regularisation when reg_alpha > 0
+enum { kPriorSigma = 1, kPriorCfirb = 2, kPriorPoisson = 4 };
+struct PriorDev {
+  int flags;
+  double sigma_width, cfirb_mean, cfirb_sigma, poisson_mean, poisson_sigma, reg_alpha;
+};
+// after every beam set of a batch: one thread per walker; bs0/wk0 are
+// beam set 0's (the counts integrals), the batch size as for the batch
+int launch_prior1d(const BeamsetDev& bs0, const WorkDev& wk0, const PriorDev& pr,
+                   const float* d_theta, uint32_t stride, uint32_t nwalk, const int32_t* d_count,
+                   double* d_loglike, const int32_t* d_status, void* stream);
 int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
                           int is_log, int already_log, double* d_out,
                           void* stream);

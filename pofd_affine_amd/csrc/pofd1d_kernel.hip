@@ -1435,6 +1435,67 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk_in, int beamset_idx
   return hipGetLastError() == hipSuccess ? POFD_OK : POFD_E_INTERNAL;
 }
 
+// calcLike::getLogLike's prior terms (src/calcLike.cc:923-952): Gaussian
+// priors on the sigma multiplier (mean 1), on the mean flux per area (the
+// CFIRB) and on the mean flux^2 per area (the Poisson noise), and the
+// difference regularisation of the knot slopes in log2 space
+// (numberCountsKnotsSpline::differenceRegularize, src/numberCountsKnotsSpline.cc:540-559);
+// walkers whose datasets failed keep their status and NaN
+__global__ void prior1d_kernel(const BeamsetDev bs, const WorkDev wk, const PriorDev pr,
+                               const float* __restrict__ theta, const uint32_t stride,
+                               double* __restrict__ loglike, const int32_t* __restrict__ status) {
+  const int walker = blockIdx.x * blockDim.x + threadIdx.x;
+  if (walker >= batch_n(wk)) return;
+  const int s = status[walker];
+  if (s != POFD_OK && s != POFD_E_BADLIKE) return;
+  const double h = 0.918938517570495605469;  // log(2 pi) / 2
+  const float* t = theta + static_cast<size_t>(walker) * stride;
+  const double* bo = wk.bonus + static_cast<size_t>(walker) * 4;
+  double add = 0.0;
+  if (pr.flags & kPriorSigma) {
+    const double v = (static_cast<double>(t[bs.nknots]) - 1.0) / pr.sigma_width;
+    add -= h + log(pr.sigma_width) + 0.5 * v * v;
+  }
+  if (pr.flags & kPriorCfirb) {
+    const double v = (pr.cfirb_mean - bo[1]) / pr.cfirb_sigma;
+    add -= h + log(pr.cfirb_sigma) + 0.5 * v * v;
+  }
+  if (pr.flags & kPriorPoisson) {
+    const double v = (pr.poisson_mean - bo[2]) / pr.poisson_sigma;
+    add -= h + log(pr.poisson_sigma) + 0.5 * v * v;
+  }
+  if (pr.reg_alpha > 0.0 && bs.nknots > 1) {
+    const int n = bs.nknots;
+    double mean = 0.0;
+    for (int i = 1; i < n; ++i)
+      mean += (bs.logfac * static_cast<double>(t[i]) - bs.logfac * static_cast<double>(t[i - 1])) /
+              (bs.logknots[i] - bs.logknots[i - 1]);
+    mean /= static_cast<double>(n - 1);
+    double pen = 0.0;
+    for (int i = 1; i < n; ++i) {
+      const double d = (bs.logfac * static_cast<double>(t[i]) -
+                        bs.logfac * static_cast<double>(t[i - 1])) /
+                           (bs.logknots[i] - bs.logknots[i - 1]) -
+                       mean;
+      pen -= d * d;
+    }
+    add += pr.reg_alpha * pen;
+  }
+  loglike[walker] += add;
+}
+
+int launch_prior1d(const BeamsetDev& bs0, const WorkDev& wk0, const PriorDev& pr,
+                   const float* d_theta, uint32_t stride, uint32_t nwalk, const int32_t* d_count,
+                   double* d_loglike, const int32_t* d_status, void* stream) {
+  WorkDev wk = wk0;
+  wk.dcount = d_count;
+  wk.nhost = static_cast<int>(nwalk);
+  hipLaunchKernelGGL(prior1d_kernel, dim3((nwalk + 63) / 64), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), bs0, wk, pr, d_theta, stride, d_loglike,
+                     d_status);
+  return hipGetLastError() == hipSuccess ? POFD_OK : POFD_E_INTERNAL;
+}
+
 int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
                           int is_log, int already_log, double* d_out,
                           void* stream) {

@@ -170,3 +170,65 @@ def test_cpp_getlike_from_spec_and_init_files(tmp_path):
                                                            np.float32)]), 1)
     assert rst[0] == 0
     assert got == pytest.approx(ref[0], abs=1e-6)
+
+
+EXE4 = os.path.join(ROOT, "tests", "cpp", "_build", "mcmc_check")
+
+
+def test_cpp_pofdmcmc_from_files(tmp_path):
+    """pofdMCMC (include/pofd_affine_amd_mcmc.hpp) from an init file with
+    ranges and a limit and a spec file: burn-in to 5 autocorrelation times,
+    200 stored steps of 32 walkers on the GPU; every stored log likelihood
+    is calcLike's for its parameters, the bonus parameters are filled, the
+    fixed sigma multiplier stays 1, the knot means lie within 5 posterior
+    sigma of the values the map was made from, and the text chain has the
+    reference's layout."""
+    print("[gpu-case] C++ pofdMCMC from init and spec files", flush=True)
+    assert os.path.exists(EXE4), "build tests/cpp first (__graft_entry__.build)"
+    inp = workload.make_inputs(npix_side=300, seed=5)
+    fitsio.write_image(str(tmp_path / "beam.fits"), inp["beam"].reshape(27, 27),
+                       extra={"PIXSIZE": inp["pixsize"]})
+    fitsio.write_image(str(tmp_path / "map.fits"),
+                       inp["data"].reshape(300, 300), extname="image")
+    truth = synth.true_theta()[:8].astype(np.float64)
+    with open(tmp_path / "init.txt", "w") as f:
+        for i, (k, v) in enumerate(zip(inp["knots"], truth)):
+            lim = " %.6g" % (v - 1.0) if i == 2 else ""   # one lower limit
+            f.write("%.9g %.9g 0.02%s\n" % (k, v, lim))
+    with open(tmp_path / "spec.txt", "w") as f:
+        f.write("dataset= %s 0.0025 %s\n" % (tmp_path / "map.fits",
+                                             tmp_path / "beam.fits"))
+        f.write("fftsize= 32768\nninterp= 512\nbin_data= 1000\n"
+                "minbeamval= 1e-6\nnbeamhist= 120\nfit_sigma= no\n")
+    chain = tmp_path / "chain.txt"
+    out = subprocess.run([EXE4, str(tmp_path / "init.txt"),
+                          str(tmp_path / "spec.txt"), "32", "200", "40",
+                          str(chain)], capture_output=True, text=True,
+                         timeout=240)
+    assert out.returncode == 0, out.stderr
+    v = {}
+    stats = {}
+    for line in out.stdout.splitlines():
+        w = line.split()
+        if not w:
+            continue
+        if w[0] == "stat":
+            stats[int(w[1])] = [float(x) for x in w[2:6]] + [int(w[6])]
+        elif w[0] in ("npar", "fit", "bonus", "acc", "recheck", "maxlike",
+                      "acor"):
+            v[w[0]] = w[1:]
+    print(out.stdout[-1500:], flush=True)
+    assert v["npar"] == ["11"] and v["fit"] == ["8"] and v["bonus"] == ["2"]
+    assert 0.05 < float(v["acc"][0]) < 0.9
+    a, b = float(v["recheck"][0]), float(v["recheck"][1])
+    assert a == pytest.approx(b, abs=1e-6)
+    for i in range(8):
+        mn, var = stats[i][0], stats[i][1]
+        assert abs(mn - truth[i]) < 5 * np.sqrt(var) + 1e-3, (i, mn, var)
+    assert stats[8][0] == 1.0 and stats[8][4] == 1          # fixed sigma
+    assert stats[9][0] > 0 and stats[10][0] > 0             # bonus filled
+    rows = np.loadtxt(chain)
+    assert rows.shape == (200 * 32, 13)
+    assert np.array_equal(np.unique(rows[:, 12]), np.arange(32))
+    assert np.all(np.isfinite(rows[:, 9:12]))
+    assert rows[:, 2].min() >= truth[2] - 1.0                # the limit

@@ -141,3 +141,46 @@ def test_reference_exponent_with_bonus_widens():
     ch, _ = ens.chain()
     widths = np.std(ch[:, :, [0, 1, 3]], axis=(0, 1)) / sig[[0, 1, 3]]
     assert np.all(widths > 1.04), widths
+
+
+def test_pofd1d_priors_and_bonus_in_chain():
+    """With priors set on the context, the sampler's log likelihoods include
+    them (calcLike::getLogLike), and accepted steps carry the mean flux and
+    flux^2 per area in the two bonus parameters (pofdMCMC::fillBonusParams)."""
+    _say("1D target with priors and bonus parameters")
+    inp = workload.make_inputs(npix_side=200, seed=7)
+    ctx = workload.build_context(inp, 32768, 512, binned=True)
+    th = synth.walker_thetas(32, seed=31)
+    ref0, _ = ctx.loglike_batch(th)
+    b0 = ctx.get_bonus(32)
+    ctx.set_priors(sigma_width=0.1, cfirb=(1.5 * b0[0, 1], 0.3 * b0[0, 1]),
+                   poisson=(b0[0, 2], 0.5 * b0[0, 2]), reg_alpha=0.01)
+    ref1, st = ctx.loglike_batch(th)
+    assert np.all(st == 0)
+    # the added terms, from calcLike::getLogLike (src/calcLike.cc:923-952)
+    h = 0.5 * np.log(2 * np.pi)
+    knots = np.log2(np.asarray(inp["knots"]))
+    lk = np.log2(10.0) * th[:, :8].astype(np.float64)
+    slope = np.diff(lk, axis=1) / np.diff(knots)
+    reg = -0.01 * np.sum((slope - slope.mean(axis=1, keepdims=True)) ** 2,
+                         axis=1)
+    cm, cs, pm, ps = 1.5 * b0[0, 1], 0.3 * b0[0, 1], b0[0, 2], 0.5 * b0[0, 2]
+    want = (-(h + np.log(0.1) + 0.5 * ((th[:, 8] - 1.0) / 0.1) ** 2)
+            - (h + np.log(cs) + 0.5 * ((cm - b0[:, 1]) / cs) ** 2)
+            - (h + np.log(ps) + 0.5 * ((pm - b0[:, 2]) / ps) ** 2) + reg)
+    np.testing.assert_allclose(ref1 - ref0, want, rtol=1e-9, atol=1e-6)
+    ens = _lib.Ensemble(ctx, _lib.TARGET_1D, 32, th.shape[1], seed=3)
+    state = np.zeros(th.shape[1], dtype=np.uint8)
+    state[9:] = 2
+    ens.set_param_state(state)
+    ens.set_positions(th)
+    ens.run(15, thin=1)
+    cur, ll, nacc, nerr = ens.state()
+    assert nacc.sum() > 0 and np.all(nerr == 0)
+    ref, st = ctx.loglike_batch(cur)
+    np.testing.assert_array_equal(ll, ref)
+    bo = ctx.get_bonus(32)
+    np.testing.assert_allclose(cur[:, 9], bo[:, 1], rtol=1e-6)
+    np.testing.assert_allclose(cur[:, 10], bo[:, 2], rtol=1e-6)
+    ch, _ = ens.chain()
+    assert np.all(np.isfinite(ch[:, :, 9:]))
