@@ -256,6 +256,28 @@ __global__ __launch_bounds__(kInfThreads) void infer_kernel(
       const int j = t * 16 + li;  // output column this lane feeds B for
       float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
       int k0 = 0;
+      if ((H & 15) == 0 && (K & 15) == 0) {
+        // every lane in range (a uniform test): unguarded loads, so that the
+        // sixteen fragments of 256 inputs are all in flight before one wait
+        const bf16* wr = W + static_cast<size_t>(j) * K + lk;
+        for (; k0 + 256 <= Kp; k0 += 256) {
+          short4v bq[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            bq[u] = *reinterpret_cast<const short4v*>(wr + k0 + 16 * u);
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const short4v a =
+                *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + 16 * u + lk);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u], acc, 0, 0, 0);
+          }
+        }
+        for (; k0 < Kp; k0 += 16) {
+          const short4v bq = *reinterpret_cast<const short4v*>(wr + k0);
+          const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + lk);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
+        }
+      }
       // 256 inputs at a time: sixteen fragments in flight, one wait
       for (; k0 + 256 <= Kp; k0 += 256) {
         short4v bq[16];
