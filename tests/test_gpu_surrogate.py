@@ -245,3 +245,47 @@ def test_delayed_acceptance_on_pofd1d():
     assert n1[1] - n0[1] < n1[0] - n0[0]
     ch2, cll2 = ens.chain()
     assert ch2.shape[0] == 250 and np.all(np.isfinite(cll2))
+
+
+def test_delayed_acceptance_on_pofd2d():
+    """Delayed acceptance on the two-band likelihood: a 12-input network
+    (band-1 knots, colour sigma and offset knots) trained on the chain's own
+    pairs; afterwards every walker's log likelihood is still the exact one
+    and fewer exact evaluations were spent than proposals made."""
+    _say("DA on the 2D P(D) likelihood")
+    from pofd_affine_amd import synth2d, workload2d
+    from oracle2d_side import make_inputs2d
+    inp = make_inputs2d(npix_side=120, seed=5, nbins=60)
+    ctx = workload2d.build_context2d(inp, 512, synth2d.true_theta(),
+                                     nedge=64, binned=True)
+    nw = 32
+    th = synth2d.walker_thetas(nw, seed=41, scatter=0.01)
+    npar = th.shape[1]
+    nfit = 12
+    state = np.zeros(npar, dtype=np.uint8)
+    state[nfit:nfit + 2] = 1      # sigma multipliers fixed
+    state[nfit + 2:] = 2          # bonus
+    ens = _lib.Ensemble(ctx, _lib.TARGET_2D, nw, npar, seed=19)
+    ens.set_param_state(state)
+    ens.set_exponent_mode(True)
+    ens.set_positions(th)
+    ens.run(160, thin=1)
+    ch, cll = ens.chain()
+    x = ch[40:, :, :nfit].reshape(-1, nfit)
+    y = cll[40:].reshape(-1)
+    net = _lib.Mlp(nfit, 128, seed=7)
+    tr, ho = net.train(x, y, epochs=150, batch=128, lr=1e-3,
+                       holdout_frac=0.1, seed=8)
+    _say("2D surrogate train RMS %.3f holdout RMS %.3f" % (tr, ho))
+    assert ens.set_surrogate(net, list(range(nfit)), gate=np.inf,
+                             holdout_rms=0.0)
+    n0 = ens.counts()
+    ens.run(60, thin=1)
+    n1 = ens.counts()
+    cur, ll, nacc, nerr = ens.state()
+    ref, st = ctx.loglike2d_batch(cur)
+    assert np.all(st == 0)
+    np.testing.assert_allclose(ll, ref, rtol=0, atol=1e-9)
+    _say("2D exact %d of %d proposals" % (n1[1] - n0[1], n1[0] - n0[0]))
+    assert n1[1] - n0[1] < n1[0] - n0[0]
+    assert np.all(ch[:, :, nfit:nfit + 2] == th[0, nfit:nfit + 2])
