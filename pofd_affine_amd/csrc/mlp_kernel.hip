@@ -2,7 +2,7 @@
 // bfloat16 or float operands and float accumulation, GELU forward and
 // backward, the squared-error gradient and an Adam step.  Shapes are small
 // (batch x 256 x 256), so one 64 x 64 output tile per 256-thread workgroup,
-// staged through LDS in K-slices of 16, is ample.
+// staged through LDS in K-slices of 64, is ample.
 #include <hip/hip_runtime.h>
 
 #include "mlp_internal.h"
@@ -38,7 +38,7 @@ __device__ __forceinline__ float ld(const void* p, size_t i) {
   return static_cast<const float*>(p)[i];
 }
 
-constexpr int kTile = 64, kKs = 16;
+constexpr int kTile = 64, kKs = 64;
 
 template <int TA, int TB>
 __global__ __launch_bounds__(256) void gemm_kernel(const void* A, int lda, int ta, const void* B,
@@ -50,19 +50,40 @@ __global__ __launch_bounds__(256) void gemm_kernel(const void* A, int lda, int t
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
   const int tm = (tid >> 4) * 4, tn = (tid & 15) * 4;  // 4 x 4 outputs per thread
   float acc[4][4] = {};
+  const bool inner_mn = (m0 + kTile <= M) && (n0 + kTile <= N);
   for (int k0 = 0; k0 < K; k0 += kKs) {
-    for (int e = tid; e < kKs * kTile; e += 256) {
-      const int kk = e / kTile, mm = e % kTile;
-      const int m = m0 + mm, k = k0 + kk;
-      float va = 0.0f;
-      if (m < M && k < K)
-        va = ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k);
-      As[kk][mm] = va;
-      const int n = n0 + mm;
-      float vb = 0.0f;
-      if (n < N && k < K)
-        vb = ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n);
-      Bs[kk][mm] = vb;
+    if (inner_mn && k0 + kKs <= K) {
+      // the whole slice inside both matrices (uniform over the workgroup):
+      // unguarded loads, all in flight before the first use
+      float va[kKs * kTile / 256], vb[kKs * kTile / 256];
+#pragma unroll
+      for (int q = 0; q < kKs * kTile / 256; ++q) {
+        const int e = tid + 256 * q;
+        const int kk = e / kTile, mm = e % kTile;
+        const int m = m0 + mm, n = n0 + mm, k = k0 + kk;
+        va[q] = ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k);
+        vb[q] = ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n);
+      }
+#pragma unroll
+      for (int q = 0; q < kKs * kTile / 256; ++q) {
+        const int e = tid + 256 * q;
+        As[e / kTile][e % kTile] = va[q];
+        Bs[e / kTile][e % kTile] = vb[q];
+      }
+    } else {
+      for (int e = tid; e < kKs * kTile; e += 256) {
+        const int kk = e / kTile, mm = e % kTile;
+        const int m = m0 + mm, k = k0 + kk;
+        float va = 0.0f;
+        if (m < M && k < K)
+          va = ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k);
+        As[kk][mm] = va;
+        const int n = n0 + mm;
+        float vb = 0.0f;
+        if (n < N && k < K)
+          vb = ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n);
+        Bs[kk][mm] = vb;
+      }
     }
     __syncthreads();
 #pragma unroll
