@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--da-epochs", type=int, default=60)
     ap.add_argument("--only-da", action="store_true",
                     help="run the delayed-acceptance leg only")
+    ap.add_argument("--multifield-steps", type=int, default=20,
+                    help="steps of the uneven multi-field graph leg (0 skips)")
+    ap.add_argument("--only-multifield", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="first run a plain device copy of BYTES (calibrates "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -96,6 +99,9 @@ def main():
     inp = workload.make_inputs(npix_side=args.side, seed=42)
     if args.only_da:
         print(json.dumps({"da": bench_da(args, inp)}))
+        return
+    if args.only_multifield:
+        print(json.dumps({"multifield": bench_multifield(args, hiprt)}))
         return
     th = synth.walker_thetas(W, seed=1234)
     stride = th.shape[1]
@@ -227,6 +233,8 @@ def main():
         out["twod"] = bench_2d(args, hiprt)
     if args.da_steps > 0:
         out["da"] = bench_da(args, inp)
+    if args.multifield_steps > 0:
+        out["multifield"] = bench_multifield(args, hiprt)
     if not args.no_cpu and args.cpu_evals > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -248,6 +256,92 @@ def main():
         out["parity_max_abs_dlogL"] = float(
             np.max(np.abs(ref[ok] - b["ll"][:ne][ok]))) if ok.any() else None
     print(json.dumps(out))
+
+
+def bench_multifield(args, hiprt):
+    """Uneven fields sharing one theta: 8 synthetic maps of 2e5-4e6 pixels
+    (log-uniform), noise 1.5-4 mJy, nflux 2^15-2^18 per field, unbinned,
+    256 walkers; the eight fields' batches captured as one HIP graph
+    (pofd1d_graph_create) whose launch gives the summed log likelihood.
+    Per-step latency percentiles beside evals/s, and the same work launched
+    eagerly field by field for comparison."""
+    from pofd_affine_amd import _lib, synth, workload
+    rng = np.random.default_rng(2024)
+    nfield = 8
+    npix = np.exp(rng.uniform(np.log(2e5), np.log(4e6), nfield))
+    sides = np.maximum(64, np.round(np.sqrt(npix)).astype(int))
+    sigmas = rng.uniform(0.0015, 0.004, nfield)
+    nfluxes = (2 ** rng.integers(15, 19, nfield)).astype(int)
+    ctxs = []
+    for i in range(nfield):
+        inp = workload.make_inputs(npix_side=int(sides[i]), sigma=float(sigmas[i]),
+                                   seed=300 + i)
+        ctxs.append(workload.build_context(inp, int(nfluxes[i]), args.ninterp,
+                                           binned=False))
+    W = args.walkers
+    th = synth.walker_thetas(W, seed=77)
+    stride = th.shape[1]
+    d_th = hiprt.DeviceBuffer(th.nbytes)
+    d_th.upload(th)
+    d_ll = hiprt.DeviceBuffer(W * 8)
+    d_st = hiprt.DeviceBuffer(W * 4)
+    stream = hiprt.Stream()
+    g = _lib.FieldGraph(ctxs, d_th.value, W, stride, d_ll.value, d_st.value,
+                        stream.ptr.value)
+    for _ in range(2):
+        g.launch(stream.ptr.value)
+    stream.synchronize()
+    K = args.multifield_steps
+    evs = [hiprt.Event() for _ in range(K + 1)]
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for i in range(K):
+        g.launch(stream.ptr.value)
+        evs[i + 1].record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    per = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(K)])
+    ll = np.zeros(W)
+    st = np.zeros(W, dtype=np.int32)
+    d_ll.download(ll)
+    d_st.download(st)
+    # eager: the same batches launched field by field
+    outs = [(hiprt.DeviceBuffer(W * 8), hiprt.DeviceBuffer(W * 4))
+            for _ in range(nfield)]
+
+    def eager():
+        for c, (a, b) in zip(ctxs, outs):
+            c.loglike_batch_device(d_th.value, W, stride, a.value, b.value,
+                                   stream.ptr.value)
+    eager()
+    stream.synchronize()
+    eevs = [hiprt.Event() for _ in range(K + 1)]
+    eevs[0].record(stream)
+    for i in range(K):
+        eager()
+        eevs[i + 1].record(stream)
+    stream.synchronize()
+    eper = np.array([eevs[i].elapsed_ms(eevs[i + 1]) for i in range(K)])
+    want = np.zeros(W)
+    for a, _ in outs:
+        v = np.zeros(W)
+        a.download(v)
+        want += v
+    return {
+        "workload": "%d unbinned fields sharing one theta, %d walkers, "
+                    "pixels %s, sigma %s mJy, nflux %s; one captured HIP "
+                    "graph per step" % (
+                        nfield, W, [int(x) ** 2 for x in sides],
+                        [round(1e3 * float(x), 2) for x in sigmas],
+                        [int(x) for x in nfluxes]),
+        "value": W / (wall / K), "unit": "evals/s", "steps": K,
+        "step_ms_p50": float(np.percentile(per, 50)),
+        "step_ms_p99": float(np.percentile(per, 99)),
+        "eager_step_ms_p50": float(np.percentile(eper, 50)),
+        "eager_step_ms_p99": float(np.percentile(eper, 99)),
+        "status_ok": int(np.sum(st == 0)),
+        "graph_vs_eager_max_abs_dlogL": float(np.max(np.abs(ll - want))),
+    }
 
 
 def bench_da(args, inp):

@@ -190,6 +190,22 @@ int pofd1d_set_priors(pofd_ctx *ctx, int flags, double sigma_width,
                       double cfirb_mean, double cfirb_sigma,
                       double poisson_mean, double poisson_sigma,
                       double reg_alpha);
+/* Several fields with their own nflux (one context each) sharing theta, as
+   one captured HIP graph, no reference counterpart: the contexts' batches
+   run in order on `stream`, each adding its log likelihood (and keeping an
+   earlier failure's status) to d_loglike_out, so one pofd1d_graph_launch
+   evaluates the sum over all the fields.  The workspaces are sized before
+   the capture; d_theta, d_loglike_out and d_status_out are bound to the
+   graph (write new theta rows into d_theta between launches).  Stage
+   timing must be off.  Priors set on any of the contexts are added once by
+   that context. */
+typedef struct pofd_graph pofd_graph;
+int pofd1d_graph_create(pofd_ctx **ctxs, uint32_t n_ctx, const float *d_theta,
+                        uint32_t n_walkers, uint32_t theta_stride,
+                        double *d_loglike_out, int32_t *d_status_out,
+                        void *stream, pofd_graph **out);
+int pofd1d_graph_launch(pofd_graph *graph, void *stream);
+void pofd1d_graph_destroy(pofd_graph *graph);
 /* Tuning, no reference counterpart: workgroups per walker of the forward
    and inverse transforms.  0 (the default) chooses by batch size,
    min(4, 256 / n_walkers) and at least 1, so that small batches (such as

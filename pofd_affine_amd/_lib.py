@@ -55,6 +55,11 @@ _SIGS = {
                                    C.POINTER(C.c_int)]),
     "pofd1d_prepare": (C.c_int, [VP]),
     "pofd1d_set_parts": (C.c_int, [VP, C.c_int]),
+    "pofd1d_graph_create": (C.c_int, [C.POINTER(VP), C.c_uint32, VP,
+                                      C.c_uint32, C.c_uint32, VP, VP, VP,
+                                      C.POINTER(VP)]),
+    "pofd1d_graph_launch": (C.c_int, [VP, VP]),
+    "pofd1d_graph_destroy": (None, [VP]),
     "pofd1d_set_priors": (C.c_int, [VP, C.c_int, D, D, D, D, D, D]),
     "pofd1d_loglike_batch_device_count": (C.c_int, [VP, VP, C.c_uint32, VP,
                                                     C.c_uint32, VP, VP, VP]),
@@ -615,6 +620,32 @@ class Context2(Context):
                 "split2", "sgpos1", "sgpos2"]
         return pd.reshape(n, n), dict(zip(keys, info))
 
+
+
+# ------------------------------------------------------------ field groups
+class FieldGraph:
+    """Several 1D contexts (fields of different nflux sharing theta) as one
+    captured HIP graph whose launch writes the summed log likelihood."""
+
+    def __init__(self, ctxs, d_theta, nwalk, stride, d_ll, d_st, stream):
+        self.ctxs = list(ctxs)  # keep the contexts alive
+        arr = (VP * len(self.ctxs))(*[c.ptr for c in self.ctxs])
+        self.ptr = VP()
+        rc = lib().pofd1d_graph_create(arr, len(self.ctxs), VP(d_theta),
+                                       nwalk, stride, VP(d_ll), VP(d_st),
+                                       VP(stream), C.byref(self.ptr))
+        if rc:
+            raise PofdError(rc, "pofd1d_graph_create failed")
+
+    def launch(self, stream):
+        rc = lib().pofd1d_graph_launch(self.ptr, VP(stream))
+        if rc:
+            raise PofdError(rc, "pofd1d_graph_launch failed")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().pofd1d_graph_destroy(self.ptr)
+            self.ptr = None
 
 
 # ------------------------------------------------------------ sampler

@@ -776,9 +776,11 @@ extern "C" int pofd1d_set_parts(pofd_ctx* c, int parts) {
 }
 
 namespace {
-// d_count: nullptr, or the batch size on the device (at most nwalk)
+// d_count: nullptr, or the batch size on the device (at most nwalk);
+// add_to: add to d_ll (and keep d_st's failures) instead of overwriting
 int batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwalk, const int32_t* d_count,
-                 uint32_t stride, double* d_ll, int32_t* d_st, void* stream) {
+                 uint32_t stride, double* d_ll, int32_t* d_st, void* stream,
+                 bool add_to = false) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (nwalk == 0) return POFD_OK;
@@ -794,8 +796,8 @@ int batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwalk, const int32_
     if ((rc = pofd2::ctx_stage_events(c, &ev))) return rc;
     s.dev.bonus = (b == 0) ? 1 : 0;  // the counts integrals once per batch
     rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta, nwalk, d_count,
-                                stride, d_ll, d_st, b > 0 ? 1 : 0, c->force_parts, stream,
-                                ev);
+                                stride, d_ll, d_st, (b > 0 || add_to) ? 1 : 0,
+                                c->force_parts, stream, ev);
     if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: launch failed: " +
                                    std::string(hipGetErrorString(hipGetLastError())));
   }
@@ -846,6 +848,64 @@ extern "C" int pofd1d_loglike_batch_device_count(pofd_ctx* c, const float* d_the
   if (!d_count) return fail(c, POFD_E_ARG, "pofd1d_loglike_batch_device_count: null count");
   return batch_device(c, d_theta, max_walkers, d_count, stride, d_ll, d_st, stream);
 }
+
+// ---- several contexts (fields of different nflux sharing theta) as one
+// captured HIP graph: the contexts' batches in order on one stream, each
+// adding its log likelihood to the previous ones'
+struct pofd_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~pofd_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+extern "C" int pofd1d_graph_create(pofd_ctx** ctxs, uint32_t nctx, const float* d_theta,
+                                   uint32_t nwalk, uint32_t stride, double* d_ll,
+                                   int32_t* d_st, void* stream, pofd_graph** out) {
+  if (!out || !ctxs || nctx == 0 || nwalk == 0 || !stream) return POFD_E_ARG;
+  *out = nullptr;
+  for (uint32_t i = 0; i < nctx; ++i) {
+    pofd_ctx* c = ctxs[i];
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (c->stage_timing)
+      return fail(c, POFD_E_ARG, "pofd1d_graph_create: stage timing must be off");
+    // every workspace exists before the capture: nothing allocates inside it
+    for (auto& s : c->sets)
+      if ((rc = ensure_workspace(c, *s, nwalk))) return rc;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto* g = new pofd_graph();
+  hipError_t he = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  int rc = (he == hipSuccess) ? POFD_OK : POFD_E_INTERNAL;
+  for (uint32_t i = 0; rc == POFD_OK && i < nctx; ++i)
+    rc = batch_device(ctxs[i], d_theta, nwalk, nullptr, stride, d_ll, d_st, stream, i > 0);
+  if (he == hipSuccess) {
+    const hipError_t he2 = hipStreamEndCapture(st, &g->graph);
+    if (he2 != hipSuccess && rc == POFD_OK) rc = POFD_E_INTERNAL;
+  }
+  if (rc == POFD_OK &&
+      hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = POFD_E_INTERNAL;
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return POFD_OK;
+}
+
+extern "C" int pofd1d_graph_launch(pofd_graph* g, void* stream) {
+  if (!g || !g->exec) return POFD_E_ARG;
+  return hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) == hipSuccess
+             ? POFD_OK
+             : POFD_E_INTERNAL;
+}
+
+extern "C" void pofd1d_graph_destroy(pofd_graph* g) { delete g; }
+
 
 extern "C" int pofd1d_loglike_batch(pofd_ctx* c, const float* theta,
                                     uint32_t nwalk, uint32_t stride,
