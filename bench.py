@@ -451,6 +451,11 @@ def bench_da(args, inp):
                "acceptance": da["acc"]},
         "ess_per_s_ratio": ed / ep,
         "step_time_ratio": plain["wall"] / da["wall"],
+        # steps after which the one-off training (not the burn-in, which
+        # both chains need) is paid back by the faster DA steps
+        "breakeven_steps_incl_training": (
+            t_train / ((plain["wall"] - da["wall"]) / S)
+            if plain["wall"] > da["wall"] else None),
         "median_shift_sigma": dmed.tolist(),
         "max_abs_median_shift_sigma": float(np.max(np.abs(dmed))),
         # a second plain chain with another seed: how far two exact chains
