@@ -1,8 +1,7 @@
-// Kernels of the log-likelihood surrogate (a small MLP): a tiled GEMM with
-// bfloat16 or float operands and float accumulation, GELU forward and
-// backward, the squared-error gradient and an Adam step.  Shapes are small
-// (batch x 256 x 256), so one 64 x 64 output tile per 256-thread workgroup,
-// staged through LDS in K-slices of 64, is ample.
+// Kernels of the log-likelihood surrogate (a small MLP): a tiled GEMM on the
+// matrix cores with bfloat16 or float operands and float arithmetic, GELU
+// forward and backward, the squared-error gradient, an Adam step, and the
+// fused inference of the whole network.
 #include <hip/hip_runtime.h>
 
 #include "mlp_internal.h"
@@ -38,79 +37,98 @@ __device__ __forceinline__ float ld(const void* p, size_t i) {
   return static_cast<const float*>(p)[i];
 }
 
-constexpr int kTile = 64, kKs = 64;
+constexpr int kTile = 64, kKs = 32;
+
+// One K-slice of A and B into registers (kKs x 64 of each per workgroup,
+// 8 values per thread); zeros outside the matrices.
+template <int TA, int TB>
+__device__ __forceinline__ void load_slice(const void* A, int lda, int ta, const void* B, int ldb,
+                                           int tb, int M, int N, int K, int m0, int n0, int k0,
+                                           float* va, float* vb) {
+  const int tid = threadIdx.x;
+  const bool inner = (m0 + kTile <= M) && (n0 + kTile <= N) && (k0 + kKs <= K);
+#pragma unroll
+  for (int q = 0; q < kKs * kTile / 256; ++q) {
+    const int e = tid + 256 * q;
+    const int kk = e / kTile, mm = e % kTile;
+    const int m = m0 + mm, n = n0 + mm, k = k0 + kk;
+    if (inner) {  // uniform over the workgroup: unguarded loads
+      va[q] = ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k);
+      vb[q] = ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n);
+    } else {
+      va[q] = (m < M && k < K)
+                  ? ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k)
+                  : 0.0f;
+      vb[q] = (n < N && k < K)
+                  ? ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n)
+                  : 0.0f;
+    }
+  }
+}
+
+// C = A B (+ bias) on the matrix cores in float: a 64 x 64 tile per
+// workgroup of four waves, each wave 2 x 2 tiles of v_mfma_f32_16x16x4_f32
+// (lane l holds A[l%16][l/16] and B[l/16][l%16] of a 16 x 4 / 4 x 16 step
+// and receives D[4(l/16)..+3][l%16]).  The next K-slice is loaded into
+// registers while the current one is multiplied out of LDS.
+typedef float float4m __attribute__((ext_vector_type(4)));
 
 template <int TA, int TB>
 __global__ __launch_bounds__(256) void gemm_kernel(const void* A, int lda, int ta, const void* B,
                                                    int ldb, int tb, float* C, int ldc,
                                                    const float* bias, int M, int N, int K) {
-  __shared__ float As[kKs][kTile + 1];
-  __shared__ float Bs[kKs][kTile + 1];
-  const int tid = threadIdx.x;
+  __shared__ float As[kKs][kTile + 4];
+  __shared__ float Bs[kKs][kTile + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
-  const int tm = (tid >> 4) * 4, tn = (tid & 15) * 4;  // 4 x 4 outputs per thread
-  float acc[4][4] = {};
-  const bool inner_mn = (m0 + kTile <= M) && (n0 + kTile <= N);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  float4m acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4m{0.0f, 0.0f, 0.0f, 0.0f};
+  constexpr int kPer = kKs * kTile / 256;
+  float va[kPer], vb[kPer];
+  load_slice<TA, TB>(A, lda, ta, B, ldb, tb, M, N, K, m0, n0, 0, va, vb);
   for (int k0 = 0; k0 < K; k0 += kKs) {
-    if (inner_mn && k0 + kKs <= K) {
-      // the whole slice inside both matrices (uniform over the workgroup):
-      // unguarded loads, all in flight before the first use
-      float va[kKs * kTile / 256], vb[kKs * kTile / 256];
 #pragma unroll
-      for (int q = 0; q < kKs * kTile / 256; ++q) {
-        const int e = tid + 256 * q;
-        const int kk = e / kTile, mm = e % kTile;
-        const int m = m0 + mm, n = n0 + mm, k = k0 + kk;
-        va[q] = ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k);
-        vb[q] = ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n);
-      }
-#pragma unroll
-      for (int q = 0; q < kKs * kTile / 256; ++q) {
-        const int e = tid + 256 * q;
-        As[e / kTile][e % kTile] = va[q];
-        Bs[e / kTile][e % kTile] = vb[q];
-      }
-    } else {
-      for (int e = tid; e < kKs * kTile; e += 256) {
-        const int kk = e / kTile, mm = e % kTile;
-        const int m = m0 + mm, k = k0 + kk;
-        float va = 0.0f;
-        if (m < M && k < K)
-          va = ld<TA>(A, ta ? static_cast<size_t>(k) * lda + m : static_cast<size_t>(m) * lda + k);
-        As[kk][mm] = va;
-        const int n = n0 + mm;
-        float vb = 0.0f;
-        if (n < N && k < K)
-          vb = ld<TB>(B, tb ? static_cast<size_t>(n) * ldb + k : static_cast<size_t>(k) * ldb + n);
-        Bs[kk][mm] = vb;
-      }
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + 256 * q;
+      As[e / kTile][e % kTile] = va[q];
+      Bs[e / kTile][e % kTile] = vb[q];
     }
     __syncthreads();
+    if (k0 + kKs < K)
+      load_slice<TA, TB>(A, lda, ta, B, ldb, tb, M, N, K, m0, n0, k0 + kKs, va, vb);
 #pragma unroll
-    for (int kk = 0; kk < kKs; ++kk) {
-      float a[4], b[4];
+    for (int kk = 0; kk < kKs; kk += 4) {
+      float a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a[i] = As[kk][tm + i];
-        b[i] = Bs[kk][tn + i];
-      }
+      for (int i = 0; i < 2; ++i) a[i] = As[kk + lk][wm + 16 * i + li];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 2; ++j) b[j] = Bs[kk + lk][wn + 16 * j + li];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int m = m0 + tm + i;
-    if (m >= M) continue;
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = n0 + tn + j;
-      if (n < N) C[static_cast<size_t>(m) * ldc + n] = acc[i][j] + (bias ? bias[n] : 0.0f);
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn + 16 * j + li;
+      if (n >= N) continue;
+      const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm + 16 * i + 4 * lk + q;
+        if (m < M) C[static_cast<size_t>(m) * ldc + n] = acc[i][j][q] + bn;
+      }
     }
-  }
 }
 
 __global__ void gelu_fwd_kernel(const float* z, bf16* out, size_t n) {
