@@ -4,7 +4,7 @@
 // 0 re-evaluated through calcLike.
 //
 // usage: mcmc_check <initfile> <specfile> <nwalkers> <nsamples> <min_burn>
-//                   <chain out>
+//                   <chain out> [init_steps fixed_burn]
 // Prints "npar", "fit", "bonus", "acc <mean acceptance>", one
 // "stat <i> <mean> <var> <low> <up>" per parameter, "acor ..." and
 // "recheck <chain logL> <direct logL>".
@@ -16,13 +16,17 @@
 #include "../../include/pofd_affine_amd_mcmc.hpp"
 
 int main(int argc, char** argv) {
-  if (argc != 7) {
-    std::cerr << "usage: mcmc_check init spec nwalkers nsamples min_burn chain" << std::endl;
+  if (argc != 7 && argc != 9) {
+    std::cerr << "usage: mcmc_check init spec nwalkers nsamples min_burn chain "
+                 "[init_steps fixed_burn]"
+              << std::endl;
     return 2;
   }
   try {
     const unsigned nw = std::stoul(argv[3]), ns = std::stoul(argv[4]), mb = std::stoul(argv[5]);
-    pofdMCMC mc(argv[1], argv[2], nw, ns, 0, 2.0, mb, false, 5.0f, 2.0f);
+    const unsigned ninit = (argc == 9) ? std::stoul(argv[7]) : 0;
+    const bool fixed = (argc == 9) && std::stoi(argv[8]) != 0;
+    pofdMCMC mc(argv[1], argv[2], nw, ns, ninit, 2.0, mb, fixed, 5.0f, 2.0f);
     mc.setSeed(17);
     mc.sample();
     const unsigned np = mc.getNParams();

@@ -175,7 +175,9 @@ def test_cpp_getlike_from_spec_and_init_files(tmp_path):
 EXE4 = os.path.join(ROOT, "tests", "cpp", "_build", "mcmc_check")
 
 
-def test_cpp_pofdmcmc_from_files(tmp_path):
+@pytest.mark.parametrize("extra", [[], ["30", "1"]],
+                         ids=["acor_burn", "init_steps_fixed_burn"])
+def test_cpp_pofdmcmc_from_files(tmp_path, extra):
     """pofdMCMC (include/pofd_affine_amd_mcmc.hpp) from an init file with
     ranges and a limit and a spec file: burn-in to 5 autocorrelation times,
     200 stored steps of 32 walkers on the GPU; every stored log likelihood
@@ -201,10 +203,12 @@ def test_cpp_pofdmcmc_from_files(tmp_path):
         f.write("fftsize= 32768\nninterp= 512\nbin_data= 1000\n"
                 "minbeamval= 1e-6\nnbeamhist= 120\nfit_sigma= no\n")
     chain = tmp_path / "chain.txt"
+    # extra: 30 initial steps at temperature 2 with the walkers then
+    # regenerated around the best point, and a fixed 40-step burn-in
     out = subprocess.run([EXE4, str(tmp_path / "init.txt"),
                           str(tmp_path / "spec.txt"), "32", "200", "40",
-                          str(chain)], capture_output=True, text=True,
-                         timeout=240)
+                          str(chain)] + extra, capture_output=True,
+                         text=True, timeout=240)
     assert out.returncode == 0, out.stderr
     v = {}
     stats = {}
