@@ -354,6 +354,24 @@ int pofd2d_loglike_batch_device(pofd_ctx *ctx, const float *d_theta,
    precision (the reference's QAG-41 stops at relative 1e-5); NaN for an
    invalid model. */
 int pofd2d_get_bonus(pofd_ctx *ctx, uint32_t nwalk, double *out);
+/* Priors and regularisation of calcLikeDouble::getLogLike
+   (src/calcLikeDouble.cc:1230-1280), added on the device to every 2D batch
+   after the dataset sum: flags POFD_PRIOR2_SIGMA1/2 (widths sigma_width[2],
+   mean 1), POFD_PRIOR2_CFIRB1/2 (cfirb[4] = mean1, sigma1, mean2, sigma2,
+   on the mean flux per area of each band), POFD_PRIOR2_POISSON1/2
+   (poisson[4], on the mean flux^2 per area), and the band-1 knot
+   regularisation when reg_alpha > 0. */
+enum {
+  POFD_PRIOR2_SIGMA1 = 1,
+  POFD_PRIOR2_SIGMA2 = 2,
+  POFD_PRIOR2_CFIRB1 = 4,
+  POFD_PRIOR2_CFIRB2 = 8,
+  POFD_PRIOR2_POISSON1 = 16,
+  POFD_PRIOR2_POISSON2 = 32
+};
+int pofd2d_set_priors(pofd_ctx *ctx, int flags, const double *sigma_width,
+                      const double *cfirb, const double *poisson,
+                      double reg_alpha);
 
 /* ---- parity and debugging (PDFactoryDouble::getPD output) ---- */
 /* info[32]: status, split1, split2, sg1pos, sg2pos, sigma1, sigma2,

@@ -1401,7 +1401,6 @@ class calcLikeDouble {
      sigmult1, sigmult2, ...] */
   void getLogLikeBatch(const float* theta, unsigned int nwalk, unsigned int stride,
                        double* loglike, int* status, double* bonus = nullptr) const {
-    const double half_log_2pi = 0.918938517570495605469;
     if (sets.empty()) {
       for (unsigned w = 0; w < nwalk; ++w) loglike[w] = NAN;
       return;
@@ -1409,48 +1408,44 @@ class calcLikeDouble {
     const unsigned nmp = model.getNParams();
     if (stride < nmp + 2) throw affineExcept("calcLikeDouble", "getLogLike", "Not enough elements in params");
     if (!prepared) throw affineExcept("calcLikeDoubleSingle", "getLogLike", "R range not set");
-    int rc = pofd2d_loglike_batch(ctx.get(), theta, nwalk, stride, loglike,
-                                  reinterpret_cast<int32_t*>(status));
+    // the priors (src/calcLikeDouble.cc:1230-1280) are added on the device
+    // after the dataset sum, for walkers whose datasets did not fail
+    int rc = pushPriors();
+    if (!rc)
+      rc = pofd2d_loglike_batch(ctx.get(), theta, nwalk, stride, loglike,
+                                reinterpret_cast<int32_t*>(status));
     if (rc) pofd_amd_detail::throw_status2(ctx.get(), rc, "calcLikeDouble", "getLogLike");
-    std::vector<double> bo;
-    const bool need_bonus = bonus || has_cfirb[0] || has_cfirb[1] || has_poisson[0] ||
-                            has_poisson[1];
-    if (need_bonus) {
-      bo.resize(5 * static_cast<size_t>(nwalk));
-      rc = pofd2d_get_bonus(ctx.get(), nwalk, bo.data());
+    if (bonus) {
+      rc = pofd2d_get_bonus(ctx.get(), nwalk, bonus);
       if (rc) pofd_amd_detail::throw_status2(ctx.get(), rc, "calcLikeDouble", "getLogLike");
-      if (bonus) std::copy(bo.begin(), bo.end(), bonus);
-    }
-    for (unsigned w = 0; w < nwalk; ++w) {
-      if (status[w] != 0 && status[w] != POFD_E_BADLIKE) continue;
-      const float* t = theta + static_cast<size_t>(w) * stride;
-      if (has_sigma_prior1) {
-        const double val = (static_cast<double>(t[nmp]) - 1.0) / sigma_prior_width1;
-        loglike[w] -= half_log_2pi + log(sigma_prior_width1) + 0.5 * val * val;
-      }
-      if (has_sigma_prior2) {
-        const double val = (static_cast<double>(t[nmp + 1]) - 1.0) / sigma_prior_width2;
-        loglike[w] -= half_log_2pi + log(sigma_prior_width2) + 0.5 * val * val;
-      }
-      for (int band = 0; band < 2; ++band) {  // src/calcLikeDouble.cc:1244-1270
-        if (has_cfirb[band]) {
-          const double val = (cfirb_mean[band] - bo[5 * w + 1 + band]) / cfirb_sigma[band];
-          loglike[w] -= half_log_2pi + log(cfirb_sigma[band]) + 0.5 * val * val;
-        }
-      }
-      for (int band = 0; band < 2; ++band) {
-        if (has_poisson[band]) {
-          const double val = (poisson_mean[band] - bo[5 * w + 3 + band]) / poisson_sigma[band];
-          loglike[w] -= half_log_2pi + log(poisson_sigma[band]) + 0.5 * val * val;
-        }
-      }
-      if (regularization_alpha > 0.0) {
-        numberCountsDoubleLogNormal m = model;
-        m.setParams(paramSet(stride, t));
-        loglike[w] += m.differenceRegularize(regularization_alpha);
-      }
     }
   }
+
+  /* the prior settings onto the context (pofd2d_set_priors) */
+  int pushPriors() const {
+    if (!ctx) return POFD_E_ARG;
+    int flags = 0;
+    if (has_sigma_prior1) flags |= POFD_PRIOR2_SIGMA1;
+    if (has_sigma_prior2) flags |= POFD_PRIOR2_SIGMA2;
+    if (has_cfirb[0]) flags |= POFD_PRIOR2_CFIRB1;
+    if (has_cfirb[1]) flags |= POFD_PRIOR2_CFIRB2;
+    if (has_poisson[0]) flags |= POFD_PRIOR2_POISSON1;
+    if (has_poisson[1]) flags |= POFD_PRIOR2_POISSON2;
+    const double sw[2] = {sigma_prior_width1, sigma_prior_width2};
+    const double cf[4] = {cfirb_mean[0], cfirb_sigma[0], cfirb_mean[1], cfirb_sigma[1]};
+    const double po[4] = {poisson_mean[0], poisson_sigma[0], poisson_mean[1], poisson_sigma[1]};
+    return pofd2d_set_priors(ctx.get(), flags, sw, cf, po, regularization_alpha);
+  }
+
+  /* the device context (after setRRanges), for the sampler: its batches
+     include the priors set here */
+  pofd_ctx* context() const {
+    if (!prepared) throw affineExcept("calcLikeDoubleSingle", "getLogLike", "R range not set");
+    const int rc = pushPriors();
+    if (rc) pofd_amd_detail::throw_status2(ctx.get(), rc, "calcLikeDouble", "context");
+    return ctx.get();
+  }
+
 };
 
 #endif

@@ -780,7 +780,12 @@ class initFileDoubleLogNormal {
   void generateRandomKnotValues(RNG& rng, paramSet& pnew) const {
     paramSet pcen(pnew.getNParams());
     getParams(pcen);
-    if (pnew.getNParams() < b_.size())
+    generateRandomKnotValues(rng, pnew, pcen);
+  }
+  /* around the central values pcen instead of the file's */
+  template <class RNG>
+  void generateRandomKnotValues(RNG& rng, paramSet& pnew, const paramSet& pcen) const {
+    if (pnew.getNParams() < b_.size() || pcen.getNParams() < b_.size())
       throw affineExcept(C, "generateRandomKnotValues", "Not enough space in provided paramSet");
     std::vector<double> cen(b_.size()), out(b_.size());
     for (size_t i = 0; i < b_.size(); ++i) cen[i] = pcen[static_cast<unsigned>(i)];

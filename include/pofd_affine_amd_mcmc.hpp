@@ -282,6 +282,29 @@ class affineEnsemble {
   bool is_init = false;
   unsigned int verbosity = 0;
 
+  /* a standard normal deviate by the ratio-of-uniforms method of Leva
+     (ACM Trans. Math. Softw. 18, 449, 1992), as ran::gauss draws it */
+  double gauss() {
+    double u, v, x, y, q;
+    do {
+      u = rangen.doub();
+      v = 1.7156 * (rangen.doub() - 0.5);
+      x = u - 0.449871;
+      y = std::fabs(v) + 0.386595;
+      q = x * x + y * (0.19600 * y - 0.25472 * x);
+    } while (q > 0.27597 && (q > 0.27846 || v * v > -4.0 * std::log(u) * u * u));
+    return v / u;
+  }
+  /* 1 + s N(0,1), redrawn while not positive (the initial sigma
+     multipliers of pofdMCMC / pofdMCMCDouble) */
+  double sigmaMultDraw(double s, const char* cls) {
+    for (unsigned j = 0; j < 1000; ++j) {
+      const double t = 1.0 + s * gauss();
+      if (t > 0) return t;
+    }
+    throw affineExcept(cls, "generateInitialPosition", "Couldn't generate sigma multiplier value");
+  }
+
   /* the device sampler over a likelihood context (POFD_TARGET_1D/2D), the
      parameter states and the limits (NaN: none) */
   void setTarget(pofd_ctx* ctx, int target, const std::vector<float>& lower,
@@ -527,20 +550,7 @@ class pofdMCMC : public affineEnsemble {
     for (unsigned w = 0; w < getNWalkers(); ++w) {
       paramSet q(npar);
       ifile.generateRandomKnotValues(rangen, q, p);
-      if (spec_info.fit_sigma) {
-        double t = 0.0;
-        unsigned j = 0;
-        for (; j < 1000; ++j) {
-          t = 1.0 + sm * gauss();
-          if (t > 0) break;
-        }
-        if (j == 1000)
-          throw affineExcept("pofdMCMC", "generateInitialPosition",
-                             "Couldn't generate sigma multiplier value");
-        q[nknots_] = static_cast<float>(t);
-      } else {
-        q[nknots_] = 1.0f;
-      }
+      q[nknots_] = spec_info.fit_sigma ? static_cast<float>(sigmaMultDraw(sm, "pofdMCMC")) : 1.0f;
       q[nknots_ + 1] = q[nknots_ + 2] = std::numeric_limits<float>::quiet_NaN();
       pos.push_back(q);
     }
@@ -560,19 +570,131 @@ class pofdMCMC : public affineEnsemble {
   unsigned nknots_ = 0;
   std::unique_ptr<calcLike> likeSet;
 
-  /* a standard normal deviate by the ratio-of-uniforms method of Leva
-     (ACM Trans. Math. Softw. 18, 449, 1992), as ran::gauss draws it */
-  double gauss() {
-    double u, v, x, y, q;
-    do {
-      u = rangen.doub();
-      v = 1.7156 * (rangen.doub() - 0.5);
-      x = u - 0.449871;
-      y = std::fabs(v) + 0.386595;
-      q = x * x + y * (0.19600 * y - 0.25472 * x);
-    } while (q > 0.27597 && (q > 0.27846 || v * v > -4.0 * std::log(u) * u * u));
-    return v / u;
+};
+
+/* ---- the 2D P(D) fit (pofdMCMCDouble, src/pofdMCMCDouble.cc): band-1
+   knots, colour sigma and offset knots from the init file, the likelihood
+   from the spec file, two sigma multipliers, and four bonus parameters
+   (<S1>area, <S2>area, <S1^2>area, <S2^2>area). ---- */
+class pofdMCMCDouble : public affineEnsemble {
+ public:
+  pofdMCMCDouble(const std::string& initfile, const std::string& specfile, unsigned int nwalkers,
+                 unsigned int nsamples, unsigned int init_steps = 0, double init_temp = 2.0,
+                 unsigned int min_burn = 50, bool fixed_burn = false, float burn_multiple = 5,
+                 float scalefac = 2)
+      : affineEnsemble(nwalkers, 1, nsamples, init_steps, init_temp, min_burn, fixed_burn,
+                       burn_multiple, scalefac),
+        initfile_(initfile),
+        specfile_(specfile) {}
+
+  /* pofdMCMCDouble::initChainsMaster (src/pofdMCMCDouble.cc) */
+  void initChains() override {
+    spec_info.readFile(specfile_);
+    if (spec_info.datafiles1.empty())
+      throw affineExcept("pofdMCMCDouble", "initChainsMaster", "No data files read");
+    ifile.readFile(initfile_, true, true);
+    ntot_ = ifile.getNTot();
+    const unsigned npar = ntot_ + 6;
+    setNParams(npar);
+    const unsigned nk = ifile.getNKnots(), ns = ifile.getNSigmas();
+    for (unsigned i = 0; i < ntot_; ++i) {
+      if (i < nk) setParamName(i, "Knot" + std::to_string(i));
+      else if (i < nk + ns) setParamName(i, "Sigma" + std::to_string(i - nk));
+      else setParamName(i, "Offset" + std::to_string(i - nk - ns));
+      if (ifile.isKnotFixed(i)) fixParam(i);
+    }
+    const char* names[6] = {"SigmaMult1", "SigmaMult2", "<S1>area",
+                            "<S2>area",   "<S1^2>area", "<S2^2>area"};
+    for (unsigned i = 0; i < 6; ++i) setParamName(ntot_ + i, names[i]);
+    if (!spec_info.fit_sigma1) fixParam(ntot_);
+    if (!spec_info.fit_sigma2) fixParam(ntot_ + 1);
+    for (unsigned i = ntot_ + 2; i < npar; ++i) setParamBonus(i);
+    likeSet = std::make_unique<calcLikeDouble>(spec_info.fftsize, spec_info.nedge,
+                                               spec_info.edge_set, spec_info.bin_data,
+                                               spec_info.nbins);
+    if (spec_info.has_cfirbprior1)
+      likeSet->setCFIRBPrior1(spec_info.cfirbprior_mean1, spec_info.cfirbprior_stdev1);
+    if (spec_info.has_cfirbprior2)
+      likeSet->setCFIRBPrior2(spec_info.cfirbprior_mean2, spec_info.cfirbprior_stdev2);
+    if (spec_info.has_poissonprior1)
+      likeSet->setPoissonPrior1(spec_info.poissonprior_mean1, spec_info.poissonprior_stdev1);
+    if (spec_info.has_poissonprior2)
+      likeSet->setPoissonPrior2(spec_info.poissonprior_mean2, spec_info.poissonprior_stdev2);
+    if (spec_info.fit_sigma1 && spec_info.has_sigprior1)
+      likeSet->setSigmaPrior1(spec_info.sigprior_stdev1);
+    if (spec_info.fit_sigma2 && spec_info.has_sigprior2)
+      likeSet->setSigmaPrior2(spec_info.sigprior_stdev2);
+    if (spec_info.regularization_alpha > 0.0)
+      likeSet->setRegularizationAlpha(spec_info.regularization_alpha);
+    likeSet->readDataFromFiles(spec_info.datafiles1, spec_info.datafiles2, spec_info.psffiles1,
+                               spec_info.psffiles2, spec_info.sigmas1, spec_info.sigmas2,
+                               spec_info.like_norm, spec_info.ignore_mask, spec_info.mean_sub,
+                               spec_info.minbeamval, spec_info.beam_histogram,
+                               spec_info.nbeamhist, spec_info.exp_conf1, spec_info.exp_conf2);
+    std::vector<double> K, S, O;
+    ifile.getKnotPos(K);
+    ifile.getSigmaPos(S);
+    ifile.getOffsetPos(O);
+    likeSet->setPositions(K, S, O);
+    paramSet p(npar);
+    ifile.getParams(p);
+    p[ntot_] = p[ntot_ + 1] = 1.0f;
+    for (unsigned i = ntot_ + 2; i < npar; ++i) p[i] = std::numeric_limits<float>::quiet_NaN();
+    likeSet->setRRanges(p);
+    setVerbosity(spec_info.verbosity);
+    if (spec_info.has_user_seed) setSeed(spec_info.seed);
+    const float nan = std::numeric_limits<float>::quiet_NaN();
+    std::vector<float> lo(npar, nan), up(npar, nan);
+    for (unsigned i = 0; i < ntot_; ++i) {
+      if (ifile.knotHasLowerLimit(i)) lo[i] = static_cast<float>(ifile.getLowerLimit(i));
+      if (ifile.knotHasUpperLimit(i)) up[i] = static_cast<float>(ifile.getUpperLimit(i));
+    }
+    lo[ntot_] = lo[ntot_ + 1] = std::numeric_limits<float>::min();
+    setTarget(likeSet->context(), POFD_TARGET_2D, lo, up);
+    is_init = true;
+    generateInitialPosition(p);
   }
+
+  bool areParamsValid(const paramSet& p) const override {
+    if (!ifile.isValid(p)) return false;
+    return p[ntot_] > 0.0f && p[ntot_ + 1] > 0.0f;
+  }
+
+  /* pofdMCMCDouble::generateInitialPosition: knots, sigmas and offsets in
+     their init-file ranges around p, each sigma multiplier 1 + s N(0,1)
+     (s its prior width, or 0.1) when fitted */
+  void generateInitialPosition(const paramSet& p) override {
+    const unsigned npar = getNParams();
+    if (p.getNParams() < npar)
+      throw affineExcept("pofdMCMCDouble", "generateInitialPosition",
+                         "Wrong number of params in input");
+    const double s1 = spec_info.has_sigprior1 ? spec_info.sigprior_stdev1 : 0.1;
+    const double s2 = spec_info.has_sigprior2 ? spec_info.sigprior_stdev2 : 0.1;
+    std::vector<paramSet> pos;
+    for (unsigned w = 0; w < getNWalkers(); ++w) {
+      paramSet q(npar);
+      ifile.generateRandomKnotValues(rangen, q, p);
+      q[ntot_] = spec_info.fit_sigma1 ? static_cast<float>(sigmaMultDraw(s1, "pofdMCMCDouble"))
+                                      : 1.0f;
+      q[ntot_ + 1] =
+          spec_info.fit_sigma2 ? static_cast<float>(sigmaMultDraw(s2, "pofdMCMCDouble")) : 1.0f;
+      for (unsigned i = ntot_ + 2; i < npar; ++i) q[i] = std::numeric_limits<float>::quiet_NaN();
+      pos.push_back(q);
+    }
+    setInitialPositions(pos);
+  }
+
+  double getLogLike(const paramSet& p, bool& rejected) const {
+    return likeSet->getLogLike(p, rejected);
+  }
+
+  specFileDouble spec_info;
+  initFileDoubleLogNormal ifile;
+
+ private:
+  std::string initfile_, specfile_;
+  unsigned ntot_ = 0;
+  std::unique_ptr<calcLikeDouble> likeSet;
 };
 
 #endif  // POFD_AFFINE_AMD_MCMC_HPP

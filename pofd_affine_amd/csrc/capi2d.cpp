@@ -499,6 +499,7 @@ struct Ctx2 {
   std::vector<std::unique_ptr<Set2>> sets;
   uint32_t last_nwalk = 0;
   std::unique_ptr<Buf> dtheta, dll, dst;
+  Prior2Dev priors{};  // added on the device after the beam sets
 };
 void ctx2_free(Ctx2* c2) { delete c2; }
 }  // namespace pofd2
@@ -1243,7 +1244,46 @@ int pofd2d_loglike_batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwal
       return ctx_fail(c, rc, std::string("pofd2d_loglike_batch_device: launch failed: ") +
                                  hipGetErrorString(hipGetLastError()));
   }
+  if (C2->priors.flags || C2->priors.reg_alpha > 0.0) {
+    Set2& s0 = *C2->sets[0];
+    rc = pofd2::launch_prior2d(s0.dev, s0.wk, C2->priors, d_theta, stride, nwalk, d_ll, d_st,
+                               stream);
+    if (rc) return ctx_fail(c, rc, "pofd2d_loglike_batch_device: prior launch failed");
+  }
   C2->last_nwalk = nwalk;
+  return POFD_OK;
+}
+
+int pofd2d_set_priors(pofd_ctx* c, int flags, const double* sigma_width, const double* cfirb,
+                      const double* poisson, double reg_alpha) {
+  Ctx2* C2 = get2(c);
+  if (!C2) return POFD_E_ARG;
+  if (flags & ~63) return ctx_fail(c, POFD_E_ARG, "pofd2d_set_priors: unknown prior flag");
+  using namespace pofd2;
+  Prior2Dev p{};
+  p.flags = flags;
+  p.reg_alpha = reg_alpha;
+  for (int b = 0; b < 2; ++b) {
+    if (flags & (kP2Sigma1 << b)) {
+      if (!sigma_width || !(sigma_width[b] > 0.0))
+        return ctx_fail(c, POFD_E_ARG, "pofd2d_set_priors: invalid sigma prior width");
+      p.sigma_width[b] = sigma_width[b];
+    }
+    if (flags & (kP2Cfirb1 << b)) {
+      if (!cfirb || !(cfirb[2 * b + 1] > 0.0))
+        return ctx_fail(c, POFD_E_ARG, "pofd2d_set_priors: invalid CFIRB prior");
+      p.cfirb_mean[b] = cfirb[2 * b];
+      p.cfirb_sigma[b] = cfirb[2 * b + 1];
+    }
+    if (flags & (kP2Poisson1 << b)) {
+      if (!poisson || !(poisson[2 * b + 1] > 0.0))
+        return ctx_fail(c, POFD_E_ARG, "pofd2d_set_priors: invalid Poisson prior");
+      p.poisson_mean[b] = poisson[2 * b];
+      p.poisson_sigma[b] = poisson[2 * b + 1];
+    }
+  }
+  if (reg_alpha < 0.0) return ctx_fail(c, POFD_E_ARG, "pofd2d_set_priors: negative alpha");
+  C2->priors = p;
   return POFD_OK;
 }
 
@@ -1273,6 +1313,15 @@ int pofd2d_loglike_batch(pofd_ctx* c, const float* theta, uint32_t nwalk, uint32
 // info[32]: the per-(walker, dataset) record: status, split1, split2,
 // sg1pos, sg2pos, sigma1, sigma2, integral, 1/integral, minflux1, minflux2,
 // dataset logL (before offset), then mn1, mn2, var1, var2, r0, K1, K2
+// the counts integrals of the last batch (beam set 0's workspace; 8 per
+// batch row: NS, S1, S2, S1^2, S2^2 per area), for the sampler's bonus
+// parameters; null before any batch
+const double* pofd2d_bonus_device(pofd_ctx* c) {
+  Ctx2* C2 = get2(c);
+  if (!C2 || C2->sets.empty() || !C2->sets[0]->wk.bonus) return nullptr;
+  return C2->sets[0]->wk.bonus;
+}
+
 int pofd2d_get_bonus(pofd_ctx* c, uint32_t nwalk, double* out) {
   Ctx2* C2 = get2(c);
   if (!C2 || !out || C2->sets.empty() || !C2->sets[0]->wbuf || nwalk > C2->last_nwalk)

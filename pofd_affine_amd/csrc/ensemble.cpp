@@ -13,8 +13,9 @@
 
 using pofd_ens_dev::EnsDev;
 
-// counts integrals of the last 1D batch on the device (capi.cpp)
+// counts integrals of the last batch on the device (capi.cpp, capi2d.cpp)
 const double* pofd1d_bonus_device(pofd_ctx* c);
+extern "C" const double* pofd2d_bonus_device(pofd_ctx* c);
 
 // surrogate predictions on the device (mlp.cpp)
 int pofd_mlp_predict_device(pofd_mlp* s, const float* src, int stride, const int* idx, int rows,
@@ -82,10 +83,17 @@ int evaluate(pofd_ens* s, const float* x, uint32_t n, double* ll, int32_t* st) {
     case POFD_TARGET_1D: {
       const int rc = pofd1d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
       s->e.bonus_src = pofd1d_bonus_device(s->ctx);  // the workspace may have moved
+      s->e.bonus_stride = 4;
+      s->e.bonus_count = 2;
       return rc;
     }
-    case POFD_TARGET_2D:
-      return pofd2d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
+    case POFD_TARGET_2D: {
+      const int rc = pofd2d_loglike_batch_device(s->ctx, x, n, s->npar, ll, st, s->stream);
+      s->e.bonus_src = pofd2d_bonus_device(s->ctx);
+      s->e.bonus_stride = 8;
+      s->e.bonus_count = 4;
+      return rc;
+    }
     case POFD_TARGET_GAUSS:
       return pofd_ens_dev::launch_gauss(s->e, x, static_cast<int>(n), ll, st, s->stream)
                  ? POFD_E_INTERNAL
@@ -119,6 +127,8 @@ int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int stor
                                                s->npar, s->e.cll, s->e.cst, s->stream);
     if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
     s->e.bonus_src = pofd1d_bonus_device(s->ctx);
+    s->e.bonus_stride = 4;
+    s->e.bonus_count = 2;
     if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, s->stream))
       return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
     if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))

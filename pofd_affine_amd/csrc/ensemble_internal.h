@@ -42,10 +42,13 @@ struct EnsDev {
   double* cll;       // nwalk/2        their exact log likelihoods
   int32_t* cst;      // nwalk/2
   unsigned long long* nexact;  // 1: exact evaluations requested by stage 1
-  // counts integrals of the last likelihood batch (4 per batch row: N, int S n,
-  // int S^2 n): the first two bonus parameters of an accepted step take the
-  // mean flux and flux^2 per area (pofdMCMC::fillBonusParams); null: NaN
+  // counts integrals of the last likelihood batch, bonus_stride per batch
+  // row, [0] the source count: the first bonus_count bonus parameters of
+  // an accepted step take entries 1.. (1D: mean flux and flux^2 per area,
+  // pofdMCMC::fillBonusParams; 2D: both bands' flux and flux^2 per area,
+  // pofdMCMCDouble::fillBonusParams); null: NaN
   const double* bonus_src;
+  int bonus_stride, bonus_count;
 };
 
 int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st);

@@ -90,8 +90,9 @@ __device__ __forceinline__ void fill_bonus(const EnsDev& e, float* x, int r) {
   int nb = 0;
   for (int i = 0; i < e.npar; ++i) {
     if (e.state[i] != kStateBonus) continue;
-    x[i] = (e.bonus_src && nb < 2)
-               ? static_cast<float>(e.bonus_src[static_cast<size_t>(r) * 4 + 1 + nb])
+    x[i] = (e.bonus_src && nb < e.bonus_count)
+               ? static_cast<float>(
+                     e.bonus_src[static_cast<size_t>(r) * e.bonus_stride + 1 + nb])
                : __int_as_float(0x7fc00000);
     ++nb;
   }

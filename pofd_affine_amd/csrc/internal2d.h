@@ -153,4 +153,20 @@ int launch_loglike2d(const Set2Dev& s, const Work2& wk, const RGrid& gmain,
                      int32_t* d_st, int accumulate, void* stream,
                      hipEvent_t* ev);
 
+// priors and regularisation of calcLikeDouble::getLogLike
+// (src/calcLikeDouble.cc:1230-1280): flags kP2Sigma1/2, kP2Cfirb1/2,
+// kP2Poisson1/2; regularisation of the band-1 knots when reg_alpha > 0
+enum { kP2Sigma1 = 1, kP2Sigma2 = 2, kP2Cfirb1 = 4, kP2Cfirb2 = 8, kP2Poisson1 = 16,
+       kP2Poisson2 = 32 };
+struct Prior2Dev {
+  int flags;
+  double sigma_width[2], cfirb_mean[2], cfirb_sigma[2], poisson_mean[2], poisson_sigma[2];
+  double reg_alpha;
+};
+// after every beam set of a batch: one thread per walker; s0/w0 are beam
+// set 0's (the counts integrals)
+int launch_prior2d(const Set2Dev& s0, const Work2& w0, const Prior2Dev& pr, const float* d_theta,
+                   uint32_t stride, uint32_t nwalk, double* d_ll, const int32_t* d_st,
+                   void* stream);
+
 }  // namespace pofd2

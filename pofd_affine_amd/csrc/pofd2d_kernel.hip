@@ -1358,4 +1358,70 @@ int launch_loglike2d(const Set2Dev& s, const Work2& wk, const RGrid& gmain,
   return hipGetLastError() == hipSuccess ? POFD_OK : POFD_E_INTERNAL;
 }
 
+// calcLikeDouble::getLogLike's prior terms (src/calcLikeDouble.cc:
+// 1230-1280): Gaussian priors on the two sigma multipliers (mean 1), on the
+// mean flux per area of each band (the CFIRB) and on the mean flux^2 per
+// area of each band (the Poisson noise), and the difference regularisation
+// of the band-1 knot slopes (numberCountsDoubleLogNormal::
+// differenceRegularize, src/numberCountsDoubleLogNormal.cc:2154-2173)
+__global__ void prior2d_kernel(const Set2Dev s, const Work2 w, const Prior2Dev pr,
+                               const float* __restrict__ theta, const uint32_t stride,
+                               const uint32_t nwalk, double* __restrict__ loglike,
+                               const int32_t* __restrict__ status) {
+  const uint32_t walker = blockIdx.x * blockDim.x + threadIdx.x;
+  if (walker >= nwalk) return;
+  const int st = status[walker];
+  if (st != POFD_OK && st != POFD_E_BADLIKE) return;
+  const double h = 0.918938517570495605469;  // log(2 pi) / 2
+  const float* t = theta + static_cast<size_t>(walker) * stride;
+  const double* bo = w.bonus + static_cast<size_t>(walker) * 8;
+  const int nmp = s.nk + s.ns + s.no;
+  double add = 0.0;
+  for (int b = 0; b < 2; ++b) {
+    if (pr.flags & (kP2Sigma1 << b)) {
+      const double v = (static_cast<double>(t[nmp + b]) - 1.0) / pr.sigma_width[b];
+      add -= h + log(pr.sigma_width[b]) + 0.5 * v * v;
+    }
+  }
+  for (int b = 0; b < 2; ++b) {
+    if (pr.flags & (kP2Cfirb1 << b)) {
+      const double v = (pr.cfirb_mean[b] - bo[1 + b]) / pr.cfirb_sigma[b];
+      add -= h + log(pr.cfirb_sigma[b]) + 0.5 * v * v;
+    }
+  }
+  for (int b = 0; b < 2; ++b) {
+    if (pr.flags & (kP2Poisson1 << b)) {
+      const double v = (pr.poisson_mean[b] - bo[3 + b]) / pr.poisson_sigma[b];
+      add -= h + log(pr.poisson_sigma[b]) + 0.5 * v * v;
+    }
+  }
+  if (pr.reg_alpha > 0.0 && s.nk > 1) {
+    const int n = s.nk;
+    double mean = 0.0;
+    for (int i = 1; i < n; ++i)
+      mean += (s.logfac * static_cast<double>(t[i]) - s.logfac * static_cast<double>(t[i - 1])) /
+              (s.logknots[i] - s.logknots[i - 1]);
+    mean /= static_cast<double>(n - 1);
+    double pen = 0.0;
+    for (int i = 1; i < n; ++i) {
+      const double d =
+          (s.logfac * static_cast<double>(t[i]) - s.logfac * static_cast<double>(t[i - 1])) /
+              (s.logknots[i] - s.logknots[i - 1]) -
+          mean;
+      pen -= d * d;
+    }
+    add += pr.reg_alpha * pen;
+  }
+  loglike[walker] += add;
+}
+
+int launch_prior2d(const Set2Dev& s0, const Work2& w0, const Prior2Dev& pr, const float* d_theta,
+                   uint32_t stride, uint32_t nwalk, double* d_ll, const int32_t* d_st,
+                   void* stream) {
+  hipLaunchKernelGGL(prior2d_kernel, dim3((nwalk + 63) / 64), dim3(64), 0,
+                     static_cast<hipStream_t>(stream), s0, w0, pr, d_theta, stride, nwalk, d_ll,
+                     d_st);
+  return hipGetLastError() == hipSuccess ? POFD_OK : POFD_E_INTERNAL;
+}
+
 }  // namespace pofd2

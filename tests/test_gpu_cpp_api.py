@@ -236,3 +236,73 @@ def test_cpp_pofdmcmc_from_files(tmp_path, extra):
     assert np.array_equal(np.unique(rows[:, 12]), np.arange(32))
     assert np.all(np.isfinite(rows[:, 9:12]))
     assert rows[:, 2].min() >= truth[2] - 1.0                # the limit
+
+
+EXE5 = os.path.join(ROOT, "tests", "cpp", "_build", "mcmc2d_check")
+
+
+def test_cpp_pofdmcmcdouble_from_files(tmp_path):
+    """pofdMCMCDouble from a 2D init file (band-1 knots, colour sigma and
+    offset knots, ranges and a limit) and a 2D spec file with a CFIRB prior
+    on band 1: 60 stored steps of 24 walkers after a fixed burn-in; every
+    stored log likelihood is calcLikeDouble's (prior included), the four
+    bonus parameters are filled, the limit holds, and the text chain has
+    the reference's layout."""
+    from pofd_affine_amd import synth2d
+    from oracle2d_side import make_inputs2d
+    print("[gpu-case] C++ pofdMCMCDouble from init and spec files", flush=True)
+    assert os.path.exists(EXE5), "build tests/cpp first (__graft_entry__.build)"
+    side = 120
+    inp = make_inputs2d(npix_side=side, seed=5, nbeamhist=120, nbins=60)
+    nb = inp["beam1"].shape[0]
+    for k, name in ((1, "beam1"), (2, "beam2")):
+        fitsio.write_image(str(tmp_path / ("b%d.fits" % k)),
+                           inp[name].reshape(nb, nb),
+                           extra={"PIXSIZE": inp["pixsize"]})
+        fitsio.write_image(str(tmp_path / ("m%d.fits" % k)),
+                           inp["d%d" % k].reshape(side, side),
+                           extname="image")
+    kp, sp, op = synth2d.positions()
+    tv = synth2d.true_theta().astype(np.float64)
+    nk, ns, no = len(kp), len(sp), len(op)
+    with open(tmp_path / "init.txt", "w") as f:
+        f.write("%d %d %d\n" % (nk, ns, no))
+        for i, (p, v) in enumerate(zip(list(kp) + list(sp) + list(op),
+                                       tv[:nk + ns + no])):
+            rng = 0.02 if i < nk else 0.01
+            lim = " %.6g" % (v - 1.0) if i == 1 else ""
+            f.write("%.9g %.9g %g%s\n" % (p, v, rng, lim))
+    with open(tmp_path / "spec.txt", "w") as f:
+        f.write("dataset= %s %s %g %g %s %s\n" % (
+            tmp_path / "m1.fits", tmp_path / "m2.fits", synth2d.SIGMA1,
+            synth2d.SIGMA2, tmp_path / "b1.fits", tmp_path / "b2.fits"))
+        f.write("fftsize= 512\nnedge= 64\nbin_data= 60\nminbeamval= 1e-6\n"
+                "nbeamhist= 120\nfit_sigma1= no\nfit_sigma2= no\n"
+                "cfirbprior1= 150 20\n")
+    chain = tmp_path / "chain.txt"
+    out = subprocess.run([EXE5, str(tmp_path / "init.txt"),
+                          str(tmp_path / "spec.txt"), "24", "60", "30",
+                          str(chain), "0", "1"], capture_output=True,
+                         text=True, timeout=240)
+    assert out.returncode == 0, out.stderr
+    v, stats = {}, {}
+    for line in out.stdout.splitlines():
+        w = line.split()
+        if not w:
+            continue
+        if w[0] == "stat":
+            stats[int(w[1])] = [float(x) for x in w[2:6]] + [int(w[6])]
+        elif w[0] in ("npar", "fit", "bonus", "acc", "recheck"):
+            v[w[0]] = w[1:]
+    ntot = nk + ns + no
+    assert v["npar"] == [str(ntot + 6)] and v["fit"] == [str(ntot)]
+    assert v["bonus"] == ["4"]
+    assert 0.02 < float(v["acc"][0]) < 0.95
+    a, b = float(v["recheck"][0]), float(v["recheck"][1])
+    assert a == pytest.approx(b, abs=1e-6)
+    assert stats[ntot][0] == 1.0 and stats[ntot + 1][0] == 1.0
+    for i in range(ntot + 2, ntot + 6):
+        assert np.isfinite(stats[i][0]) and stats[i][0] > 0, i
+    rows = np.loadtxt(chain)
+    assert rows.shape == (60 * 24, ntot + 8)
+    assert rows[:, 1].min() >= tv[1] - 1.0
