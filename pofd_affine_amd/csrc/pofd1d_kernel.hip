@@ -42,10 +42,15 @@ namespace pofd {
 namespace {
 
 constexpr int kWaves = kThreads / 64;
-// padded LDS column: slot(e) = e + e/16, odd column stride (in double2)
-constexpr int kSlots = kM1 + kM1 / 16 + 1;
+// LDS column of 256 double2: element e sits at slot(e) = e ^ ((e >> 2) & 15)
+// (a permutation inside each block of 16), columns kSlots = 257 apart.  With
+// this swizzle the transform's stage reads (e = j + 64m), its stage writes
+// (e = 4j + m, 16(j/4) + j%4 + 4m, ...) and the fill's and accumulation's
+// column accesses (rows 2 at a time over 4 columns) put the 16-byte accesses
+// of each 8 or 16 consecutive lanes in distinct bank groups.
+constexpr int kSlots = kM1 + 1;
 
-__device__ __forceinline__ int slot(int e) { return e + (e >> 4); }
+__device__ __forceinline__ int slot(int e) { return e ^ ((e >> 2) & 15); }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -147,13 +152,19 @@ __device__ void block_argmin(double& v, int& idx, double* red) {
   idx = bi;
 }
 
-// ---- 256-point transforms of 16 padded LDS columns ------------------------
+// ---- 256-point transforms of 16 swizzled LDS columns ----------------------
 // Radix-4 Stockham between two buffers: 64 threads per column, 4 points
 // each, one barrier per stage; after the four stages the result is in `a`.
 // The caller synchronises before the call.  SIGN = -1 is FFTW_FORWARD, +1
-// the unnormalised backward direction.
+// the unnormalised backward direction.  The twiddles of stage Ns come from
+// wst (kStageTw entries per multiplier m = 1..3): wst[(m - 1) * kStageTw +
+// stw_base(Ns) + jm] = exp(-2 pi i m jm (64 / Ns) / 256), so the lanes of a
+// stage read consecutive entries.
+constexpr int kStageTw = 4 + 16 + 64;
+__device__ __forceinline__ int stw_base(int Ns) { return Ns == 4 ? 0 : (Ns == 16 ? 4 : 20); }
+
 template <int SIGN>
-__device__ void fft256_cols(double2* a, double2* b, const double2* w256) {
+__device__ void fft256_cols(double2* a, double2* b, const double2* wst) {
   const int tid = threadIdx.x;
   const int col = tid >> 6, j = tid & 63;
   double2* src = a + col * kSlots;
@@ -164,8 +175,8 @@ __device__ void fft256_cols(double2* a, double2* b, const double2* w256) {
     double2 v2 = src[slot(j + 128)], v3 = src[slot(j + 192)];
     const int jm = j & (Ns - 1);
     if (Ns > 1) {
-      const int e = jm * (64 / Ns);
-      double2 t1 = w256[e], t2 = w256[2 * e], t3 = w256[3 * e];
+      const int e = stw_base(Ns) + jm;
+      double2 t1 = wst[e], t2 = wst[kStageTw + e], t3 = wst[2 * kStageTw + e];
       if (SIGN > 0) {
         t1 = cconj(t1);
         t2 = cconj(t2);
@@ -263,22 +274,46 @@ struct PrepSmem {
 struct XformSmem {
   double2 bufa[kChunk * kSlots];
   double2 bufb[kChunk * kSlots];
-  double2 w256[kM1];
+  double2 wst[3 * kStageTw];  // stage twiddles of the column transforms
   double2 wlo[kM1];
   double2 wm2[512];
   short rowk2[512];
   short rowof[512];
   double red[kWaves * 2];
   double sc[4];
+  double ex2[32];           // 2^(i/32) * dflux (forward only)
   int isc[2 + kMaxFields];  // nrows, kmaxb, then kmax of each dataset
 };
 
 __device__ void load_twiddles(const BeamsetDev& bs, XformSmem& sm) {
-  for (int i = threadIdx.x; i < kM1; i += blockDim.x) {
-    sm.w256[i] = bs.W256[i];
-    sm.wlo[i] = bs.Wlo[i];
+  for (int i = threadIdx.x; i < kM1; i += blockDim.x) sm.wlo[i] = bs.Wlo[i];
+  for (int i = threadIdx.x; i < 3 * kStageTw; i += blockDim.x) {
+    const int m = i / kStageTw + 1, r = i % kStageTw;
+    const int Ns = (r < 4) ? 4 : (r < 20 ? 16 : 64);
+    const int jm = r - stw_base(Ns);
+    sm.wst[i] = bs.W256[(m * jm * (64 / Ns)) & (kM1 - 1)];
   }
   for (int i = threadIdx.x; i < bs.M2; i += blockDim.x) sm.wm2[i] = bs.Wm2[i];
+}
+
+// 2^s * dflux by Tang's table-driven method (ACM TOMS 15:144, 1989):
+// 32 s = 32 n + j + 32 r with |r| <= 1/64, 2^s = 2^n 2^(j/32) 2^r, the table
+// entries tab[j] = 2^(j/32) * dflux and 2^r = exp(r ln 2) as its Taylor
+// polynomial of degree 6 (truncation below 4e-18 relative).  Accurate to
+// about one ulp; 0 below 2^-1075 and +inf above 2^1024 as exp2 gives them.
+__device__ __forceinline__ double exp2_dflux(double s, const double* tab) {
+  const double t = fmin(fmax(s * 32.0, -35000.0), 33000.0);
+  const double n = rint(t);
+  const double r = (t - n) * 0.03125;
+  const int ni = static_cast<int>(n);
+  double p = 1.5403530393381609954e-4;   // (ln 2)^6 / 720
+  p = fma(p, r, 1.3333558146428443423e-3);  // (ln 2)^5 / 120
+  p = fma(p, r, 9.6181291076284771620e-3);  // (ln 2)^4 / 24
+  p = fma(p, r, 5.5504108664821579953e-2);  // (ln 2)^3 / 6
+  p = fma(p, r, 2.4022650695910071233e-1);  // (ln 2)^2 / 2
+  p = fma(p, r, 6.9314718055994530942e-1);  // ln 2
+  p = fma(p, r, 1.0);
+  return ldexp(tab[ni & 31] * p, ni >> 5);
 }
 
 }  // namespace
@@ -540,6 +575,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
                                : wk.zpart + static_cast<size_t>(blockIdx.x) * wk.zpart_stride;
 
   load_twiddles(bs, sm);
+  if (tid < 32) sm.ex2[tid] = exp2(static_cast<double>(tid) * 0.03125) * dflux;
   if (tid == 0) {
     int kmaxb;
     sm.isc[0] = spectral_rows(bs, sigmult, sm.rowk2, sm.rowof, kmaxb);
@@ -558,9 +594,12 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
 
   for (int p0 = 0; p0 < nrows; p0 += kRowsPerPass) {
     const int np = min(kRowsPerPass, nrows - p0);
-    double2 acc[kRowsPerPass];
+    // kept row r of this pass accumulates in lane q = r % 4 of the k1 quad
+    // (slot r / 4), so only two accumulators per thread stay live through
+    // the fill and the column transforms
+    double2 acc[kRowsPerPass / 4];
 #pragma unroll
-    for (int r = 0; r < kRowsPerPass; ++r) acc[r] = make_double2(0.0, 0.0);
+    for (int s = 0; s < kRowsPerPass / 4; ++s) acc[s] = make_double2(0.0, 0.0);
     for (int c = c_lo; c < c_hi; ++c) {
       // fill: thread (n1 = row4, q) makes R at 8 consecutive grid points
       {
@@ -589,7 +628,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
                                        : static_cast<double>(j) * dflux;
             const double delx = cflux - xk;
             const double s = cf.x + delx * (cf.y + delx * (cf.z + delx * cf.w));
-            rv = exp2(s) * dflux;
+            rv = exp2_dflux(s, sm.ex2);
           }
           if (p0 == 0 && j >= 1) {
             // getMeanVarFromR (src/PDFactory.cc:625-679)
@@ -619,36 +658,43 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
         }
       }
       __syncthreads();
-      fft256_cols<-1>(sm.bufa, sm.bufb, sm.w256);
-      // twiddle and accumulate the kept rows: thread (k1 = row4, q)
+      fft256_cols<-1>(sm.bufa, sm.bufb, sm.wst);
+      // twiddle and accumulate the kept rows: thread (k1 = row4, q) holds
+      // columns 4q..4q+3; each row's sum over the chunk's 16 columns is
+      // combined across the quad (the same value in all four lanes)
       {
         const int k1 = row4;
+        double2 a[4];
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) {
-          const int col = 4 * q + cc;
-          const int n2 = kChunk * c + col;
-          double2 a = sm.bufa[col * kSlots + slot(k1)];
+          const int n2 = kChunk * c + 4 * q + cc;
           const int t = k1 * n2;
-          a = cmul(a, cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
-#pragma unroll
-          for (int r = 0; r < kRowsPerPass; ++r)
-            if (r < np) {
-              const double2 w = sm.wm2[(n2 * sm.rowk2[p0 + r]) & (M2 - 1)];
-              acc[r] = cadd(acc[r], cmul(a, w));
-            }
+          a[cc] = cmul(sm.bufa[(4 * q + cc) * kSlots + slot(k1)],
+                       cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
         }
+#pragma unroll
+        for (int r = 0; r < kRowsPerPass; ++r)
+          if (r < np) {
+            const int k2 = sm.rowk2[p0 + r];
+            double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+              const int n2 = kChunk * c + 4 * q + cc;
+              v = cadd(v, cmul(a[cc], sm.wm2[(n2 * k2) & (M2 - 1)]));
+            }
+            v.x += __shfl_xor(v.x, 1);
+            v.y += __shfl_xor(v.y, 1);
+            v.x += __shfl_xor(v.x, 2);
+            v.y += __shfl_xor(v.y, 2);
+            if ((r & 3) == q) acc[r >> 2] = cadd(acc[r >> 2], v);
+          }
       }
       __syncthreads();
     }
-    // combine the 4 partial accumulators of each k1 (lanes q = 0..3)
 #pragma unroll
-    for (int r = 0; r < kRowsPerPass; ++r) {
-      double2 v = acc[r];
-      v.x += __shfl_xor(v.x, 1);
-      v.y += __shfl_xor(v.y, 1);
-      v.x += __shfl_xor(v.x, 2);
-      v.y += __shfl_xor(v.y, 2);
-      if (q == 0 && r < np) zdst[(p0 + r) * kM1 + row4] = v;
+    for (int s = 0; s < kRowsPerPass / 4; ++s) {
+      const int r = 4 * s + q;
+      if (r < np) zdst[(p0 + r) * kM1 + row4] = acc[s];
     }
   }
   const double mn = block_sum(mn_part, sm.red);
@@ -906,7 +952,7 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
       }
     }
     __syncthreads();
-    fft256_cols<+1>(sm.bufa, sm.bufb, sm.w256);
+    fft256_cols<+1>(sm.bufa, sm.bufb, sm.wst);
     {
       const int n1 = row4;
       const int j0 = 2 * M2 * n1 + 2 * kChunk * c + 8 * q;
