@@ -42,15 +42,23 @@ namespace pofd {
 namespace {
 
 constexpr int kWaves = kThreads / 64;
-// LDS column of 256 double2: element e sits at slot(e) = e ^ ((e >> 2) & 15)
-// (a permutation inside each block of 16), columns kSlots = 257 apart.  With
-// this swizzle the transform's stage reads (e = j + 64m), its stage writes
-// (e = 4j + m, 16(j/4) + j%4 + 4m, ...) and the fill's and accumulation's
-// column accesses (rows 2 at a time over 4 columns) put the 16-byte accesses
-// of each 8 or 16 consecutive lanes in distinct bank groups.
+// LDS layout of a chunk: element e of column col (16 columns of 256 double2)
+// sits at cpos(col, e) = col * kSlots + (slot(e) ^ (col >> 2)), with
+// slot(e) = e ^ ((e >> 1) & 15) ^ ((e >> 2) & 15) a permutation inside each
+// block of 16 and kSlots = 257.  The layout was chosen against per-dispatch
+// LDS bank-conflict counts of these access patterns on gfx950, which fit
+// ds_write_b128 serving 8 consecutive lanes a pass over 8 16-byte bank
+// groups and ds_read_b128 the lane sets {0-3, 20-23}, {4-7, 16-19},
+// {8-11, 28-31}, {12-15, 24-27} (and the same +32) over 16 groups; under
+// that model every stage read
+// (e = j + 64m) and stage write (e = 4j + m, 16(j/4) + j%4 + 4m, ...) of the
+// column transforms and the fill's and accumulation's column accesses
+// (2 rows x 4 columns 4q + cc per pass) touch distinct groups.
 constexpr int kSlots = kM1 + 1;
-
-__device__ __forceinline__ int slot(int e) { return e ^ ((e >> 2) & 15); }
+__device__ __forceinline__ int slot(int e) { return e ^ ((e >> 1) & 15) ^ ((e >> 2) & 15); }
+__device__ __forceinline__ int cpos(int col, int e) {
+  return col * kSlots + (slot(e) ^ (col >> 2));
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -167,12 +175,12 @@ template <int SIGN>
 __device__ void fft256_cols(double2* a, double2* b, const double2* wst) {
   const int tid = threadIdx.x;
   const int col = tid >> 6, j = tid & 63;
-  double2* src = a + col * kSlots;
-  double2* dst = b + col * kSlots;
+  double2* src = a;
+  double2* dst = b;
 #pragma unroll
   for (int Ns = 1; Ns < kM1; Ns *= 4) {
-    double2 v0 = src[slot(j)], v1 = src[slot(j + 64)];
-    double2 v2 = src[slot(j + 128)], v3 = src[slot(j + 192)];
+    double2 v0 = src[cpos(col, j)], v1 = src[cpos(col, j + 64)];
+    double2 v2 = src[cpos(col, j + 128)], v3 = src[cpos(col, j + 192)];
     const int jm = j & (Ns - 1);
     if (Ns > 1) {
       const int e = stw_base(Ns) + jm;
@@ -197,10 +205,10 @@ __device__ void fft256_cols(double2* a, double2* b, const double2* wst) {
       y3 = make_double2(a1.x + a3.y, a1.y - a3.x);
     }
     const int d = (j / Ns) * Ns * 4 + jm;
-    dst[slot(d)] = cadd(a0, a2);
-    dst[slot(d + Ns)] = y1;
-    dst[slot(d + 2 * Ns)] = csub(a0, a2);
-    dst[slot(d + 3 * Ns)] = y3;
+    dst[cpos(col, d)] = cadd(a0, a2);
+    dst[cpos(col, d + Ns)] = y1;
+    dst[cpos(col, d + 2 * Ns)] = csub(a0, a2);
+    dst[cpos(col, d + 3 * Ns)] = y3;
     __syncthreads();
     double2* t = src;
     src = dst;
@@ -653,7 +661,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
             }
           }
           if (e & 1)
-            sm.bufa[(4 * q + (e >> 1)) * kSlots + slot(row4)] = make_double2(rprev, rv);
+            sm.bufa[cpos(4 * q + (e >> 1), row4)] = make_double2(rprev, rv);
           rprev = rv;
         }
       }
@@ -669,7 +677,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
         for (int cc = 0; cc < 4; ++cc) {
           const int n2 = kChunk * c + 4 * q + cc;
           const int t = k1 * n2;
-          a[cc] = cmul(sm.bufa[(4 * q + cc) * kSlots + slot(k1)],
+          a[cc] = cmul(sm.bufa[cpos(4 * q + cc, k1)],
                        cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
         }
 #pragma unroll
@@ -948,7 +956,7 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
         }
         const int t = k1 * n2;
         b = cmulc(b, cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
-        sm.bufa[(4 * q + cc) * kSlots + slot(k1)] = b;
+        sm.bufa[cpos(4 * q + cc, k1)] = b;
       }
     }
     __syncthreads();
@@ -959,7 +967,7 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
       double v8[8];
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
-        const double2 z = sm.bufa[(4 * q + cc) * kSlots + slot(n1)];
+        const double2 z = sm.bufa[cpos(4 * q + cc, n1)];
         v8[2 * cc] = z.x;
         v8[2 * cc + 1] = z.y;
       }
