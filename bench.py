@@ -25,9 +25,9 @@ HBM_PEAK_GBS = 8000.0
 # written = 1.0 x WRITE_SIZE KiB); None where not profiled or for another
 # workload.
 TRAFFIC = {
-    "forward": (2.0 * 9545.3 + 16156.6) * 1024,
-    "inverse": (2.0 * 2291.5 + 332998.1) * 1024,
-    "finalize": (2.0 * 9930.3 + 64.0) * 1024,
+    "forward": (2.0 * 5844.8 + 9612.0) * 1024,
+    "inverse": (2.0 * 2297.5 + 333267.3) * 1024,
+    "finalize": (2.0 * 9927.7 + 64.0) * 1024,
 }
 
 
