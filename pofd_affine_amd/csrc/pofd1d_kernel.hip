@@ -472,15 +472,33 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
       double* ynode_g = wk.ynode + walker * wk.ynode_stride + seg * ninterp;
       for (int i = tid; i < ninterp; i += kThreads) {
         double workval = 0.0;
-        for (int j = j0; j < j1; ++j) {
-          const size_t e = static_cast<size_t>(j) * ninterp + i;
-          const int k = bs.rk[e];
-          if (k < 0) continue;
-          const double dx = bs.rdelx[e];
-          const double s =
-              sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
-          const double cts = exp2(s);
-          if (cts > 0) workval += bs.bw[j] * cts * bs.bieta[j];
+        // beam bins in groups of 8: the loads of a group are issued
+        // together, then its terms are added in bin order
+        for (int jg = j0; jg < j1; jg += 8) {
+          int kq[8];
+          double dq[8], wq[8], eq[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int j = jg + u;
+            kq[u] = -1;
+            if (j < j1) {  // uniform across the workgroup
+              const size_t e = static_cast<size_t>(j) * ninterp + i;
+              kq[u] = bs.rk[e];
+              dq[u] = bs.rdelx[e];
+              wq[u] = bs.bw[j];
+              eq[u] = bs.bieta[j];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = kq[u];
+            if (k < 0) continue;
+            const double dx = dq[u];
+            const double s =
+                sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
+            const double cts = exp2(s);
+            if (cts > 0) workval += wq[u] * cts * eq[u];
+          }
         }
         const double R = bs.prefac * workval;
         sm.yn[i] = log2_or_small(R);
