@@ -360,6 +360,12 @@ def bench_da(args, inp):
     ctx = workload.build_context(inp, N, args.ninterp, binned=True)
     th0 = synth.walker_thetas(W, seed=777, scatter=0.01)
     npar, nk = th0.shape[1], 8
+    # workgroups per walker measured for the batch sizes a half-step meets
+    # (the plain chain's 256 proposals, the delayed-acceptance survivors)
+    tune_sizes = [16, 32, 48, 64, 80, 96, 128, 192, 256]
+    t0 = time.perf_counter()
+    tuned = ctx.autotune_parts(th0, tune_sizes, reps=5)
+    t_tune = time.perf_counter() - t0
     state = np.zeros(npar, dtype=np.uint8)
     state[nk] = 1          # sigma multiplier fixed
     state[nk + 1:] = 2     # bonus: the counts integrals
@@ -439,6 +445,10 @@ def bench_da(args, inp):
         "surrogate": {"train_rms": tr, "holdout_rms": ho, "gate": 0.5,
                       "enabled": bool(da["on"]), "train_s": t_train,
                       "epochs": args.da_epochs},
+        "autotune": {"sizes": tune_sizes,
+                     "parts": [int(v) for v in tuned],
+                     "size_rule": [min(4, max(1, 256 // n)) for n in tune_sizes],
+                     "seconds": t_tune},
         "burn_and_collect_s": t_burn,
         "plain": {"wall_s": plain["wall"], "ms_per_step":
                   1e3 * plain["wall"] / S, "tau_mean": tp,

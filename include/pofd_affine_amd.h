@@ -215,6 +215,19 @@ void pofd1d_graph_destroy(pofd_graph *graph);
    are combined in a fixed order, so results depend on the part count only
    in the last bits. */
 int pofd1d_set_parts(pofd_ctx *ctx, int parts);
+/* Autotuning, no reference counterpart: measures the workgroups per walker.
+   For each of the n_sizes batch sizes (1 to 256) every allowed split (1 to
+   min(4, 256 / size)) is timed with HIP events over `reps` batches of the
+   first `size` rows of theta (host memory, at least max(sizes) rows of
+   `theta_stride` floats) after one warm-up batch, and the fastest is
+   written to chosen[i].  The context then uses the measured choice for
+   every batch size (an unmeasured size takes the choice of the nearest
+   measured one) whenever pofd1d_set_parts is 0, in host-count and
+   device-count batches alike.  n_sizes = 0 returns to the size rule.
+   Stage timing must be off. */
+int pofd1d_autotune_parts(pofd_ctx *ctx, const float *theta, uint32_t theta_stride,
+                          const int32_t *sizes, uint32_t n_sizes, uint32_t reps,
+                          int32_t *chosen);
 
 /* ---- parity / debug accessors for the most recent batch ---- */
 /* The P(D) as PDFactory::getPD(sigma, pd, setLog) leaves PD::pd_ (rotated,

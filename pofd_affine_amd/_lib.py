@@ -55,6 +55,8 @@ _SIGS = {
                                    C.POINTER(C.c_int)]),
     "pofd1d_prepare": (C.c_int, [VP]),
     "pofd1d_set_parts": (C.c_int, [VP, C.c_int]),
+    "pofd1d_autotune_parts": (C.c_int, [VP, FP, C.c_uint32, I32P, C.c_uint32,
+                                        C.c_uint32, I32P]),
     "pofd1d_graph_create": (C.c_int, [C.POINTER(VP), C.c_uint32, VP,
                                       C.c_uint32, C.c_uint32, VP, VP, VP,
                                       C.POINTER(VP)]),
@@ -298,6 +300,17 @@ class Context:
         """Workgroups per walker of the 1D transforms (0: by batch size)."""
         self._check(lib().pofd1d_set_parts(self.ptr, int(parts)),
                     "pofd1d_set_parts")
+
+    def autotune_parts(self, thetas, sizes, reps=5):
+        """Measured workgroups per walker for each batch size in `sizes`
+        (pofd1d_autotune_parts); returns the chosen split per size."""
+        th = np.ascontiguousarray(thetas, dtype=np.float32)
+        sz = np.ascontiguousarray(sizes, dtype=np.int32)
+        out = np.zeros(len(sz), dtype=np.int32)
+        self._check(lib().pofd1d_autotune_parts(
+            self.ptr, th.ctypes.data_as(FP), th.shape[1], sz.ctypes.data_as(I32P),
+            len(sz), int(reps), out.ctypes.data_as(I32P)), "pofd1d_autotune_parts")
+        return out
 
     def set_stage_timing(self, on):
         self._check(lib().pofd_set_option(self.ptr, POFD_OPT_STAGE_TIMING,

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -80,6 +81,11 @@ struct pofd_ctx {
   std::vector<std::unique_ptr<Beamset>> sets;
   uint32_t last_nwalk = 0;
   int force_parts = 0;  // workgroups per walker: 0 = by batch size
+  // measured workgroups per walker by batch size (pofd1d_autotune_parts):
+  // host copy and device copy, both empty until the first tuning
+  std::array<uint8_t, pofd::kMaxSlots + 1> parts_tab{};
+  std::unique_ptr<DevBuf> dparts;
+  bool parts_tuned = false;
   pofd::PriorDev priors{};  // added on the device after the beam sets
   std::unique_ptr<DevBuf> dtheta, dll, dst;
   size_t cap_theta = 0, cap_out = 0;
@@ -795,9 +801,10 @@ int batch_device(pofd_ctx* c, const float* d_theta, uint32_t nwalk, const int32_
     hipEvent_t* ev = nullptr;
     if ((rc = pofd2::ctx_stage_events(c, &ev))) return rc;
     s.dev.bonus = (b == 0) ? 1 : 0;  // the counts integrals once per batch
-    rc = pofd::launch_loglike1d(s.dev, s.wk, static_cast<int>(b), d_theta, nwalk, d_count,
-                                stride, d_ll, d_st, (b > 0 || add_to) ? 1 : 0,
-                                c->force_parts, stream, ev);
+    rc = pofd::launch_loglike1d(
+        s.dev, s.wk, static_cast<int>(b), d_theta, nwalk, d_count, stride, d_ll, d_st,
+        (b > 0 || add_to) ? 1 : 0, c->force_parts, c->parts_tuned ? c->parts_tab.data() : nullptr,
+        c->parts_tuned ? static_cast<const uint8_t*>(c->dparts->p) : nullptr, stream, ev);
     if (rc) return fail(c, rc, "pofd1d_loglike_batch_device: launch failed: " +
                                    std::string(hipGetErrorString(hipGetLastError())));
   }
@@ -847,6 +854,109 @@ extern "C" int pofd1d_loglike_batch_device_count(pofd_ctx* c, const float* d_the
                                                  void* stream) {
   if (!d_count) return fail(c, POFD_E_ARG, "pofd1d_loglike_batch_device_count: null count");
   return batch_device(c, d_theta, max_walkers, d_count, stride, d_ll, d_st, stream);
+}
+
+// ---- autotuning of the workgroups per walker: every allowed split of each
+// given batch size is timed with HIP events over `reps` batches after one
+// warm-up batch, the fastest is kept, and sizes not measured take the
+// choice of the nearest measured size (the smaller one on a tie)
+extern "C" int pofd1d_autotune_parts(pofd_ctx* c, const float* theta, uint32_t stride,
+                                     const int32_t* sizes, uint32_t nsizes, uint32_t reps,
+                                     int32_t* chosen) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (nsizes == 0) {  // back to the size rule
+    c->parts_tuned = false;
+    c->parts_tab.fill(0);
+    return POFD_OK;
+  }
+  if (!theta || !sizes || !chosen || reps == 0)
+    return fail(c, POFD_E_ARG, "pofd1d_autotune_parts: null buffer or zero reps");
+  if (stride < c->knots.size() + 1)
+    return fail(c, POFD_E_ARG, "pofd1d_autotune_parts: theta stride too small");
+  if (c->stage_timing)
+    return fail(c, POFD_E_ARG, "pofd1d_autotune_parts: stage timing must be off");
+  uint32_t nmax = 0;
+  for (uint32_t i = 0; i < nsizes; ++i) {
+    if (sizes[i] < 1 || sizes[i] > static_cast<int32_t>(pofd::kMaxSlots))
+      return fail(c, POFD_E_ARG, "pofd1d_autotune_parts: sizes must be in [1, 256]");
+    nmax = std::max<uint32_t>(nmax, static_cast<uint32_t>(sizes[i]));
+  }
+  // workspaces of at least 64 walkers hold the full 256 (walker, part) slots
+  for (auto& s : c->sets)
+    if ((rc = ensure_workspace(c, *s, std::max<uint32_t>(nmax, 64)))) return rc;
+  std::unique_ptr<DevBuf> dth, dll, dst;
+  const size_t tb = static_cast<size_t>(nmax) * stride * sizeof(float);
+  if ((rc = dev_alloc(c, dth, tb)) || (rc = dev_alloc(c, dll, nmax * sizeof(double))) ||
+      (rc = dev_alloc(c, dst, nmax * sizeof(int32_t))))
+    return rc;
+  hipError_t e = hipMemcpy(dth->p, theta, tb, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(theta)");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return hip_fail(c, e, "hipEventCreate");
+  }
+  const int saved_force = c->force_parts;
+  const bool saved_tuned = c->parts_tuned;
+  c->parts_tuned = false;
+  std::array<uint8_t, pofd::kMaxSlots + 1> tab{};
+  const float* d_theta = static_cast<const float*>(dth->p);
+  double* d_ll = static_cast<double*>(dll->p);
+  int32_t* d_st = static_cast<int32_t*>(dst->p);
+  for (uint32_t i = 0; i < nsizes && rc == POFD_OK; ++i) {
+    const uint32_t n = static_cast<uint32_t>(sizes[i]);
+    const int pmax = std::min<int>(4, static_cast<int>(pofd::kMaxSlots / n));
+    int best = 1;
+    float best_ms = 0.0f;
+    for (int p = 1; p <= pmax && rc == POFD_OK; ++p) {
+      c->force_parts = p;
+      if ((rc = batch_device(c, d_theta, n, nullptr, stride, d_ll, d_st, nullptr))) break;
+      (void)hipEventRecord(e0, nullptr);
+      for (uint32_t r = 0; r < reps && rc == POFD_OK; ++r)
+        rc = batch_device(c, d_theta, n, nullptr, stride, d_ll, d_st, nullptr);
+      (void)hipEventRecord(e1, nullptr);
+      if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+        rc = hip_fail(c, e, "hipEventSynchronize");
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (p == 1 || ms < best_ms) {
+        best = p;
+        best_ms = ms;
+      }
+    }
+    tab[n] = static_cast<uint8_t>(best);
+    chosen[i] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->force_parts = saved_force;
+  if (rc) {
+    c->parts_tuned = saved_tuned;
+    return rc;
+  }
+  // unmeasured sizes: the nearest measured one
+  for (int n = 1; n <= static_cast<int>(pofd::kMaxSlots); ++n) {
+    if (tab[n]) continue;
+    int bestd = 1 << 30;
+    uint8_t v = 0;
+    for (uint32_t i = 0; i < nsizes; ++i) {
+      const int d = std::abs(sizes[i] - n);
+      if (d < bestd || (d == bestd && sizes[i] < n)) {
+        bestd = d;
+        v = tab[sizes[i]];
+      }
+    }
+    tab[n] = v;
+  }
+  if ((rc = dev_alloc(c, c->dparts, tab.size()))) return rc;
+  e = hipMemcpy(c->dparts->p, tab.data(), tab.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(parts table)");
+  c->parts_tab = tab;
+  c->parts_tuned = true;
+  return POFD_OK;
 }
 
 // ---- several contexts (fields of different nflux sharing theta) as one

@@ -129,12 +129,19 @@ struct WorkDev {
   const int32_t* dcount;
   int nhost;
   int force_parts;
+  // measured workgroups per walker by batch size (kMaxSlots + 1 entries,
+  // 0 = the size rule), or null before pofd1d_autotune_parts
+  const uint8_t* parts_tab;
 };
 
 // workgroups per walker of the split transforms for a batch of n walkers
-// (the same rule on the host and in the kernels)
-__host__ __device__ inline int batch_parts(int n, int force, uint32_t nslots) {
+// (the same rule on the host and in the kernels): the forced count, else
+// the measured table's entry, else min(4, 256 / n); reduced while
+// n x parts exceeds the workspace's slots
+__host__ __device__ inline int batch_parts(int n, int force, uint32_t nslots,
+                                           const uint8_t* tab) {
   int parts = force;
+  if (parts == 0 && tab && n > 0 && n <= kMaxSlots) parts = tab[n];
   if (parts == 0) {
     parts = (n > 0) ? 256 / n : 4;
     parts = parts < 1 ? 1 : (parts > 4 ? 4 : parts);
@@ -156,7 +163,8 @@ namespace pofd {
 int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk, int beamset_idx,
                      const float* d_theta, uint32_t nwalk, const int32_t* d_count,
                      uint32_t stride, double* d_loglike, int32_t* d_status,
-                     int accumulate, int force_parts, void* stream, hipEvent_t* ev);
+                     int accumulate, int force_parts, const uint8_t* parts_tab_host,
+                     const uint8_t* parts_tab_dev, void* stream, hipEvent_t* ev);
 // priors and regularisation added to the summed log likelihood
 // (calcLike::getLogLike, src/calcLike.cc:923-952); flags: kPriorSigma,
 // kPriorCfirb, kPriorPoisson; regularisation when reg_alpha > 0

@@ -585,7 +585,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
   __shared__ XformSmem sm;
   const int tid = threadIdx.x;
   const int n = batch_n(wk);
-  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots, wk.parts_tab);
   const int walker = blockIdx.x / parts;
   const int part = blockIdx.x - walker * parts;
   if (walker >= n) return;
@@ -745,7 +745,7 @@ __global__ __launch_bounds__(kThreads) void spectrum1d_kernel(const BeamsetDev b
   const int tid = threadIdx.x;
   const int walker = blockIdx.x;
   const int n = batch_n(wk);
-  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots, wk.parts_tab);
   if (walker >= n || parts == 1) return;  // one part: the forward finished
   double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
@@ -887,7 +887,7 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
   __shared__ XformSmem sm;
   const int tid = threadIdx.x;
   const int n = batch_n(wk);
-  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots, wk.parts_tab);
   const int walker = blockIdx.x / parts;
   const int part = blockIdx.x - walker * parts;
   const int f = blockIdx.y;
@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
   const int walker = blockIdx.x;
   const int f = blockIdx.y;
   const int n = batch_n(wk);
-  const int parts = batch_parts(n, wk.force_parts, wk.nslots);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots, wk.parts_tab);
   if (walker >= n) return;
   const double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
@@ -1459,12 +1459,14 @@ __global__ void materialize_pd_kernel(const double* pofd, int N, int split,
 int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk_in, int beamset_idx,
                      const float* d_theta, uint32_t nwalk, const int32_t* d_count,
                      uint32_t stride, double* d_loglike, int32_t* d_status,
-                     int accumulate, int force_parts, void* stream, hipEvent_t* ev) {
+                     int accumulate, int force_parts, const uint8_t* parts_tab_host,
+                     const uint8_t* parts_tab_dev, void* stream, hipEvent_t* ev) {
   if (force_parts < 0 || force_parts > 4) return POFD_E_ARG;
   WorkDev wk = wk_in;
   wk.dcount = d_count;
   wk.nhost = static_cast<int>(nwalk);
   wk.force_parts = force_parts;
+  wk.parts_tab = parts_tab_dev;
   // workgroups of the split transforms: exactly n x parts for a host count;
   // for a device count every batch up to nwalk fits in max(nwalk, nslots)
   uint32_t nxf;
@@ -1473,7 +1475,8 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk_in, int beamset_idx
     nxf = (nwalk > wk.nslots) ? nwalk : wk.nslots;
     split = true;
   } else {
-    const int parts = batch_parts(static_cast<int>(nwalk), force_parts, wk.nslots);
+    const int parts = batch_parts(static_cast<int>(nwalk), force_parts, wk.nslots,
+                                  parts_tab_host);
     nxf = nwalk * static_cast<uint32_t>(parts);
     split = parts > 1;
   }

@@ -123,3 +123,57 @@ def test_device_count_batch(count):
     # the same part count as a host batch of this size
     np.testing.assert_array_equal(ll[:count], ref[:count])
     assert np.all(ll[count:] == 123.0) and np.all(stv[count:] == -7)
+
+
+def test_autotune_parts():
+    """pofd1d_autotune_parts: every measured choice is an allowed split, a
+    batch then runs exactly as with that split forced (unmeasured sizes as
+    the nearest measured one, device-count batches as host ones), and
+    tuning with no sizes restores the size rule."""
+    _say("autotune parts")
+    inp = workload.make_inputs(npix_side=200, seed=7)
+    ctx = workload.build_context(inp, 32768, 512, binned=True)
+    th = synth.walker_thetas(200, seed=23)
+    rule = {}
+    for n in (8, 37, 40, 50, 70, 130, 200):
+        rule[n], st = ctx.loglike_batch(th[:n])
+        assert np.all(st == 0)
+    sizes = [8, 40, 70, 130, 200]
+    chosen = ctx.autotune_parts(th, sizes, reps=3)
+    _say("chosen %s" % list(chosen))
+    choice = {}
+    for n, p in zip(sizes, chosen):
+        assert 1 <= p <= min(4, 256 // n), (n, p)
+        choice[n] = int(p)
+    choice[50] = choice[40]     # nearest measured size
+    choice[37] = choice[40]
+    for n, p in choice.items():
+        tuned, st = ctx.loglike_batch(th[:n])
+        assert np.all(st == 0)
+        ctx.set_parts(p)
+        forced, st = ctx.loglike_batch(th[:n])
+        ctx.set_parts(0)
+        np.testing.assert_array_equal(tuned, forced)
+        np.testing.assert_allclose(tuned, rule[n], rtol=0, atol=1e-8)
+    # a device-count batch takes the same measured split as a host batch
+    nmax, count = 160, 37
+    d_th = hiprt.DeviceBuffer(th[:nmax].nbytes)
+    d_th.upload(np.ascontiguousarray(th[:nmax]))
+    d_n = hiprt.DeviceBuffer(4)
+    d_n.upload(np.array([count], dtype=np.int32))
+    d_ll = hiprt.DeviceBuffer(nmax * 8)
+    d_st = hiprt.DeviceBuffer(nmax * 4)
+    ctx.loglike_batch_device_count(d_th.value, nmax, d_n.value, th.shape[1],
+                                   d_ll.value, d_st.value)
+    hiprt.device_synchronize()
+    ll = np.zeros(nmax)
+    d_ll.download(ll)
+    ref, st = ctx.loglike_batch(th[:count])
+    np.testing.assert_array_equal(ll[:count], ref)
+    # no sizes: the size rule again (40 walkers -> 4 parts)
+    ctx.autotune_parts(th, [], reps=1)
+    again, st = ctx.loglike_batch(th[:40])
+    ctx.set_parts(4)
+    forced, st2 = ctx.loglike_batch(th[:40])
+    ctx.set_parts(0)
+    np.testing.assert_array_equal(again, forced)
