@@ -62,6 +62,11 @@ def main():
     ap.add_argument("--da-epochs", type=int, default=60)
     ap.add_argument("--only-da", action="store_true",
                     help="run the delayed-acceptance leg only")
+    ap.add_argument("--da2d-steps", type=int, default=500,
+                    help="steps of each 2D chain of the two-band delayed-"
+                         "acceptance leg (0 skips that leg)")
+    ap.add_argument("--only-da2d", action="store_true",
+                    help="run the two-band delayed-acceptance leg only")
     ap.add_argument("--multifield-steps", type=int, default=20,
                     help="steps of the uneven multi-field graph leg (0 skips)")
     ap.add_argument("--only-multifield", action="store_true")
@@ -95,6 +100,9 @@ def main():
         print(json.dumps({"twod": bench_2d(args, hiprt)}))
         return
 
+    if args.only_da2d:
+        print(json.dumps({"da2d": bench_da2d(args)}))
+        return
     W, N = args.walkers, args.nflux
     inp = workload.make_inputs(npix_side=args.side, seed=42)
     if args.only_da:
@@ -233,6 +241,8 @@ def main():
         out["twod"] = bench_2d(args, hiprt)
     if args.da_steps > 0:
         out["da"] = bench_da(args, inp)
+    if args.da2d_steps > 0 and args.steps_2d > 0:
+        out["da2d"] = bench_da2d(args)
     if args.multifield_steps > 0:
         out["multifield"] = bench_multifield(args, hiprt)
     if not args.no_cpu and args.cpu_evals > 0:
@@ -343,6 +353,99 @@ def bench_multifield(args, hiprt):
         "graph_vs_eager_max_abs_dlogL": float(np.max(np.abs(ll - want))),
     }
 
+
+
+def bench_da2d(args):
+    """The two-band delayed-acceptance leg at full size: the 2D likelihood
+    of the 2D leg (2048^2 FP64 grid, 1000x1000 unbinned pixel pairs), a
+    64-walker stretch move (32 proposals a half-step, the 2D batch size).
+    A plain burn-in of 150 steps, then the (theta, exact logL) pairs of the
+    next 120 stored steps train a 12 -> 3x256 -> 1 surrogate (band-1 knots,
+    colour sigma and offset knots); from that state a plain chain and a
+    delayed-acceptance chain of the same length run and are compared on
+    effective samples per second and time per step."""
+    from pofd_affine_amd import _lib, synth2d, workload2d
+    n, W, S = args.n_2d, 64, args.da2d_steps
+    b1, b2, pix = synth2d.gaussian_beams()
+    d1, d2, nsrc = synth2d.make_maps(args.side, seed=99, beams=(b1, b2, pix))
+    inp = dict(beam1=b1, beam2=b2, pixsize=pix, d1=d1, d2=d2,
+               minbeamval=1e-6, nbeamhist=120, nsrc=nsrc, nbins=None)
+    ctx = workload2d.build_context2d(inp, n, synth2d.true_theta())
+    th0 = synth2d.walker_thetas(W, seed=4242, scatter=0.01)
+    npar, nfit = th0.shape[1], 12
+    state = np.zeros(npar, dtype=np.uint8)
+    state[nfit:nfit + 2] = 1      # sigma multipliers fixed
+    state[nfit + 2:] = 2          # bonus: the counts integrals
+    fit = list(range(nfit))
+
+    def make(pos, seed):
+        e = _lib.Ensemble(ctx, _lib.TARGET_2D, W, npar, seed=seed)
+        e.set_param_state(state)
+        e.set_exponent_mode(True)
+        e.set_positions(pos)
+        return e
+
+    ens = make(th0, 61)
+    t0 = time.perf_counter()
+    ens.run(150, thin=0)
+    ens.run(120, thin=1)
+    t_burn = time.perf_counter() - t0
+    ch, cll = ens.chain()
+    x = ch[:, :, :nfit].reshape(-1, nfit)
+    y = cll.reshape(-1)
+    start, _, _, _ = ens.state()
+    del ens
+    net = _lib.Mlp(nfit, 256, seed=15)
+    t0 = time.perf_counter()
+    tr, ho = net.train(x, y, epochs=150, batch=256, lr=1e-3,
+                       holdout_frac=0.1, seed=16)
+    t_train = time.perf_counter() - t0
+
+    def chain(use_da):
+        e = make(start, 71)
+        on = e.set_surrogate(net, fit) if use_da else False
+        e.run(3, thin=0)  # warm the launch path
+        n0 = e.counts()
+        e.clear_chain()
+        t0 = time.perf_counter()
+        e.run(S, thin=1)
+        wall = time.perf_counter() - t0
+        n1 = e.counts()
+        taus = []
+        for i in fit:
+            try:
+                taus.append(e.acor(i)[0])
+            except _lib.PofdError:
+                taus.append(float("nan"))
+        tau = float(np.nanmean(taus)) if np.any(np.isfinite(taus)) else float("nan")
+        return dict(on=bool(on), wall=wall, tau=tau,
+                    nprop=int(n1[0] - n0[0]), nexact=int(n1[1] - n0[1]))
+
+    plain = chain(False)
+    da = chain(True)
+
+    def side(r):
+        ess = W * S / r["tau"] / r["wall"] if np.isfinite(r["tau"]) else float("nan")
+        return {"wall_s": r["wall"], "ms_per_step": 1e3 * r["wall"] / S,
+                "tau_mean": r["tau"], "ess_per_s": ess,
+                "exact_evals": r["nexact"], "proposals": r["nprop"]}
+
+    sp, sd = side(plain), side(da)
+    return {
+        "workload": "2D P(D) stretch move, %dx%d grid, %d walkers (32 "
+                    "proposals a half-step), %d steps per chain, %dx%d "
+                    "unbinned pixel pairs; surrogate MLP %d -> 3x256 GELU -> 1 "
+                    "(bf16, fp32 accumulate) trained on %d (theta, logL) "
+                    "pairs from 120 stored steps after 150 burn-in steps"
+                    % (n, n, W, S, args.side, args.side, nfit, x.shape[0]),
+        "surrogate": {"train_rms": tr, "holdout_rms": ho, "gate": 0.5,
+                      "enabled": da["on"], "train_s": t_train},
+        "burn_and_collect_s": t_burn,
+        "plain": sp, "da": sd,
+        "ess_per_s_ratio": sd["ess_per_s"] / sp["ess_per_s"],
+        "step_time_ratio": plain["wall"] / da["wall"],
+        "exact_fraction": da["nexact"] / max(da["nprop"], 1),
+    }
 
 def bench_da(args, inp):
     """The delayed-acceptance leg: the same binned 1D likelihood (nflux
