@@ -360,7 +360,7 @@ def bench_da2d(args):
     of the 2D leg (2048^2 FP64 grid, 1000x1000 unbinned pixel pairs), a
     64-walker stretch move (32 proposals a half-step, the 2D batch size).
     A plain burn-in of 150 steps, then the (theta, exact logL) pairs of the
-    next 120 stored steps train a 12 -> 3x256 -> 1 surrogate (band-1 knots,
+    next 240 stored steps train a 12 -> 3x256 -> 1 surrogate (band-1 knots,
     colour sigma and offset knots); from that state a plain chain and a
     delayed-acceptance chain of the same length run and are compared on
     effective samples per second and time per step."""
@@ -388,7 +388,7 @@ def bench_da2d(args):
     ens = make(th0, 61)
     t0 = time.perf_counter()
     ens.run(150, thin=0)
-    ens.run(120, thin=1)
+    ens.run(240, thin=1)
     t_burn = time.perf_counter() - t0
     ch, cll = ens.chain()
     x = ch[:, :, :nfit].reshape(-1, nfit)
@@ -436,7 +436,7 @@ def bench_da2d(args):
                     "proposals a half-step), %d steps per chain, %dx%d "
                     "unbinned pixel pairs; surrogate MLP %d -> 3x256 GELU -> 1 "
                     "(bf16, fp32 accumulate) trained on %d (theta, logL) "
-                    "pairs from 120 stored steps after 150 burn-in steps"
+                    "pairs from 240 stored steps after 150 burn-in steps"
                     % (n, n, W, S, args.side, args.side, nfit, x.shape[0]),
         "surrogate": {"train_rms": tr, "holdout_rms": ho, "gate": 0.5,
                       "enabled": da["on"], "train_s": t_train},
