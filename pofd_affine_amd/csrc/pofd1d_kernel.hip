@@ -626,66 +626,66 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     double2 acc[kRowsPerPass / 4];
 #pragma unroll
     for (int s = 0; s < kRowsPerPass / 4; ++s) acc[s] = make_double2(0.0, 0.0);
-    for (int c = c_lo; c < c_hi; ++c) {
-      // fill: thread (n1 = row4, q) makes R at 8 consecutive grid points
-      {
-        const int j0 = 2 * M2 * row4 + 2 * kChunk * c + 8 * q;
-        // node intervals of the 8 fluxes (gsl_interp_accel_find, precomputed)
-        const uint4 kq = *reinterpret_cast<const uint4*>(bs.kst + j0);
-        const uint32_t kw[4] = {kq.x, kq.y, kq.z, kq.w};
-        int k = kw[0] & 0xffff;
-        double xk = bs.xnode[k];
-        double4 cf = *reinterpret_cast<const double4*>(coef + 4 * k);
-        double rprev = 0.0;
+    // fill: thread (n1 = row4, q) makes R at 8 consecutive grid points of
+    // chunk c into dst
+    auto fill_chunk = [&](const int c, double2* dst) {
+      const int j0 = 2 * M2 * row4 + 2 * kChunk * c + 8 * q;
+      // node intervals of the 8 fluxes (gsl_interp_accel_find, precomputed)
+      const uint4 kq = *reinterpret_cast<const uint4*>(bs.kst + j0);
+      const uint32_t kw[4] = {kq.x, kq.y, kq.z, kq.w};
+      int k = kw[0] & 0xffff;
+      double xk = bs.xnode[k];
+      double4 cf = *reinterpret_cast<const double4*>(coef + 4 * k);
+      double rprev = 0.0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int j = j0 + e;
-          const int ke = (kw[e >> 1] >> (16 * (e & 1))) & 0xffff;
-          if (ke != k) {
-            k = ke;
-            xk = bs.xnode[k];
-            cf = *reinterpret_cast<const double4*>(coef + 4 * k);
-          }
-          double rv = 0.0;
-          const bool inneg = (j >= bs.minitn && j <= bs.maxitn);
-          if (inneg || (j >= bs.minit && j <= bs.maxit)) {
-            // grid points above wrapRidx hold the wrapped negative fluxes
-            const double cflux = inneg ? static_cast<double>(j) * dflux + bs.mval
-                                       : static_cast<double>(j) * dflux;
-            const double delx = cflux - xk;
-            const double s = cf.x + delx * (cf.y + delx * (cf.z + delx * cf.w));
-            rv = exp2_dflux(s, sm.ex2);
-          }
-          if (p0 == 0 && j >= 1) {
-            // getMeanVarFromR (src/PDFactory.cc:625-679)
-            const int wr = bs.wrapRidx;
-            double prod, cfl;
-            if (j <= wr) {
-              cfl = static_cast<double>(j) * dflux;
-              prod = (j == wr) ? 0.5 * cfl * rv : cfl * rv;
-            } else {
-              cfl = static_cast<double>(j - wr - 1) * dflux + bs.minflux_R;
-              prod = cfl * rv;
-            }
-            mn_part += prod;
-            var_part += cfl * prod;
-            if (j == wr - 1 && wr < bs.N - 1) {
-              // the wrap correction reads R[wrapRidx - 1] (src/PDFactory.cc:
-              // 661-668), kept as the reference has it
-              const double cb = bs.minflux_R;
-              const double pb = 0.5 * cb * rv;
-              mn_part -= pb;
-              var_part -= cb * pb;
-            }
-          }
-          if (e & 1)
-            sm.bufa[cpos(4 * q + (e >> 1), row4)] = make_double2(rprev, rv);
-          rprev = rv;
+      for (int e = 0; e < 8; ++e) {
+        const int j = j0 + e;
+        const int ke = (kw[e >> 1] >> (16 * (e & 1))) & 0xffff;
+        if (ke != k) {
+          k = ke;
+          xk = bs.xnode[k];
+          cf = *reinterpret_cast<const double4*>(coef + 4 * k);
         }
+        double rv = 0.0;
+        const bool inneg = (j >= bs.minitn && j <= bs.maxitn);
+        if (inneg || (j >= bs.minit && j <= bs.maxit)) {
+          // grid points above wrapRidx hold the wrapped negative fluxes
+          const double cflux = inneg ? static_cast<double>(j) * dflux + bs.mval
+                                     : static_cast<double>(j) * dflux;
+          const double delx = cflux - xk;
+          const double s = cf.x + delx * (cf.y + delx * (cf.z + delx * cf.w));
+          rv = exp2_dflux(s, sm.ex2);
+        }
+        if (p0 == 0 && j >= 1) {
+          // getMeanVarFromR (src/PDFactory.cc:625-679)
+          const int wr = bs.wrapRidx;
+          double prod, cfl;
+          if (j <= wr) {
+            cfl = static_cast<double>(j) * dflux;
+            prod = (j == wr) ? 0.5 * cfl * rv : cfl * rv;
+          } else {
+            cfl = static_cast<double>(j - wr - 1) * dflux + bs.minflux_R;
+            prod = cfl * rv;
+          }
+          mn_part += prod;
+          var_part += cfl * prod;
+          if (j == wr - 1 && wr < bs.N - 1) {
+            // the wrap correction reads R[wrapRidx - 1] (src/PDFactory.cc:
+            // 661-668), kept as the reference has it
+            const double cb = bs.minflux_R;
+            const double pb = 0.5 * cb * rv;
+            mn_part -= pb;
+            var_part -= cb * pb;
+          }
+        }
+        if (e & 1)
+          dst[cpos(4 * q + (e >> 1), row4)] = make_double2(rprev, rv);
+        rprev = rv;
       }
-      __syncthreads();
-      fft256_cols<-1>(sm.bufa, sm.bufb, sm.wst);
-      // twiddle and accumulate the kept rows: thread (k1 = row4, q) holds
+    };
+    auto accum_chunk = [&](const int c, const double2* src) {
+      // twiddle and accumulate the kept rows of chunk c (transformed in
+      // src): thread (k1 = row4, q) holds
       // columns 4q..4q+3; each row's sum over the chunk's 16 columns is
       // combined across the quad (the same value in all four lanes)
       {
@@ -695,7 +695,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
         for (int cc = 0; cc < 4; ++cc) {
           const int n2 = kChunk * c + 4 * q + cc;
           const int t = k1 * n2;
-          a[cc] = cmul(sm.bufa[cpos(4 * q + cc, k1)],
+          a[cc] = cmul(src[cpos(4 * q + cc, k1)],
                        cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
         }
 #pragma unroll
@@ -715,6 +715,22 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
             if ((r & 3) == q) acc[r >> 2] = cadd(acc[r >> 2], v);
           }
       }
+    };
+    // the two buffers alternate by chunk: a chunk's column transforms end
+    // in the buffer it was filled into, so its row accumulation reads that
+    // buffer while the next chunk fills the other, in one barrier interval
+    // (the loop starts one chunk early, where it only fills the first chunk)
+    for (int c = c_lo - 1; c < c_hi; ++c) {
+      double2* cur = ((c - c_lo) & 1) ? sm.bufb : sm.bufa;
+      double2* oth = ((c - c_lo) & 1) ? sm.bufa : sm.bufb;
+      if (c >= c_lo) {
+        fft256_cols<-1>(cur, oth, sm.wst);
+        accum_chunk(c, cur);
+      }
+      // each wave keeps the two phases in order (their registers do not
+      // both fit); waves of the workgroup still overlap them
+      __builtin_amdgcn_sched_barrier(0);
+      if (c + 1 < c_hi) fill_chunk(c + 1, oth);
       __syncthreads();
     }
 #pragma unroll
