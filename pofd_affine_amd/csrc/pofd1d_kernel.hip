@@ -161,8 +161,8 @@ __device__ void block_argmin(double& v, int& idx, double* red) {
 }
 
 // ---- 256-point transforms of 16 swizzled LDS columns ----------------------
-// Radix-4 Stockham between two buffers: 64 threads per column, 4 points
-// each, one barrier per stage; after the four stages the result is in `a`.
+// Radix-4 Stockham between two buffers: 64 threads (one wavefront) per
+// column, 4 points each; after the four stages the result is in `a`.
 // The caller synchronises before the call.  SIGN = -1 is FFTW_FORWARD, +1
 // the unnormalised backward direction.  The twiddles of stage Ns come from
 // wst (kStageTw entries per multiplier m = 1..3): wst[(m - 1) * kStageTw +
@@ -209,7 +209,17 @@ __device__ void fft256_cols(double2* a, double2* b, const double2* wst) {
     dst[cpos(col, d + Ns)] = y1;
     dst[cpos(col, d + 2 * Ns)] = csub(a0, a2);
     dst[cpos(col, d + 3 * Ns)] = y3;
-    __syncthreads();
+    // a column belongs to one wavefront (64 threads) in every stage, so
+    // between stages only that wavefront's LDS accesses need ordering; the
+    // last stage ends with a workgroup barrier for the callers, which read
+    // the columns across wavefronts
+    if (Ns < kM1 / 4) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
     double2* t = src;
     src = dst;
     dst = t;
