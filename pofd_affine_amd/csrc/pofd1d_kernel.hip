@@ -713,10 +713,15 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
           if (r < np) {
             const int k2 = sm.rowk2[p0 + r];
             double2 v = make_double2(0.0, 0.0);
+            if (p0 + r == 0) {  // row k2 = 0: the twiddle is 1
 #pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-              const int n2 = kChunk * c + 4 * q + cc;
-              v = cadd(v, cmul(a[cc], sm.wm2[(n2 * k2) & (M2 - 1)]));
+              for (int cc = 0; cc < 4; ++cc) v = cadd(v, a[cc]);
+            } else {
+#pragma unroll
+              for (int cc = 0; cc < 4; ++cc) {
+                const int n2 = kChunk * c + 4 * q + cc;
+                v = cadd(v, cmul(a[cc], sm.wm2[(n2 * k2) & (M2 - 1)]));
+              }
             }
             v.x += __shfl_xor(v.x, 1);
             v.y += __shfl_xor(v.y, 1);
@@ -988,7 +993,9 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
         if (small_rows) {
 #pragma unroll
           for (int r = 0; r < kRowsPerPass; ++r)
-            if (r < nrows) {
+            if (r == 0) {  // row k2 = 0: the twiddle is 1
+              b = cadd(b, zr[0]);
+            } else if (r < nrows) {
               const double2 w = sm.wm2[(n2 * sm.rowk2[r]) & (M2 - 1)];
               b = cadd(b, cmulc(zr[r], w));
             }
