@@ -243,7 +243,7 @@ __device__ int spectral_kmax(double sigma, double iflux, double cut, int M) {
 }
 
 // Spectral rows k2 needed by the widest dataset of the beam set (k <= kmaxb
-// or k >= M - kmaxb); thread 0 only.
+// or k >= M - kmaxb); called by every thread of the workgroup.
 __device__ int spectral_rows(const BeamsetDev& bs, double sigmult, short* rowk2,
                              short* rowof, int& kmaxb_out) {
   const int M = bs.M, M2 = bs.M2;
@@ -251,19 +251,21 @@ __device__ int spectral_rows(const BeamsetDev& bs, double sigmult, short* rowk2,
   for (int f = 0; f < bs.nfields; ++f)
     kmaxb = max(kmaxb, spectral_kmax(sigmult * bs.fields[f].sigma_base,
                                      bs.iflux, bs.spectral_cut, M));
-  for (int r = 0; r < M2; ++r) rowof[r] = -1;
   const int klo = min(kmaxb, M - 1);
   const int lo_rows = klo / kM1 + 1;
   int hi_start = (kmaxb >= M) ? 0 : (M - kmaxb) / kM1;
   if (kmaxb == 0) hi_start = M2;  // only Z[0]
-  int nrows = 0;
-  for (int r = 0; r < M2; ++r)
-    if (r < lo_rows || r >= hi_start) {
-      rowof[r] = static_cast<short>(nrows);
-      rowk2[nrows++] = static_cast<short>(r);
-    }
+  // the kept rows are [0, lo_rows) then [h0, M2), in ascending order; every
+  // thread of the workgroup computes the same bounds and fills its share
+  // of the tables (the caller synchronises before reading them)
+  const int h0 = max(hi_start, lo_rows);
+  for (int r = threadIdx.x; r < M2; r += blockDim.x) {
+    const int pos = (r < lo_rows) ? r : ((r >= h0) ? lo_rows + (r - h0) : -1);
+    rowof[r] = static_cast<short>(pos);
+    if (pos >= 0) rowk2[pos] = static_cast<short>(r);
+  }
   kmaxb_out = kmaxb;
-  return nrows;
+  return lo_rows + (M2 - h0);
 }
 
 __device__ __forceinline__ int batch_n(const WorkDev& wk) {
@@ -612,13 +614,16 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
 
   load_twiddles(bs, sm);
   if (tid < 32) sm.ex2[tid] = exp2(static_cast<double>(tid) * 0.03125) * dflux;
-  if (tid == 0) {
+  {
     int kmaxb;
-    sm.isc[0] = spectral_rows(bs, sigmult, sm.rowk2, sm.rowof, kmaxb);
-    sm.isc[1] = kmaxb;
-    if (part == 0) {
-      st[ST_KMAXB] = kmaxb;
-      st[ST_NROWS] = sm.isc[0];
+    const int nr = spectral_rows(bs, sigmult, sm.rowk2, sm.rowof, kmaxb);
+    if (tid == 0) {
+      sm.isc[0] = nr;
+      sm.isc[1] = kmaxb;
+      if (part == 0) {
+        st[ST_KMAXB] = kmaxb;
+        st[ST_NROWS] = nr;
+      }
     }
   }
   __syncthreads();
@@ -781,10 +786,13 @@ __global__ __launch_bounds__(kThreads) void spectrum1d_kernel(const BeamsetDev b
   double* st = walker_state(wk, walker);
   if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
   const unsigned long long tmark = wall_clock64();
-  if (tid == 0) {
+  {
     int kmaxb;
-    sm.isc[0] = spectral_rows(bs, st[ST_SIGMULT], sm.rowk2, sm.rowof, kmaxb);
-    sm.isc[1] = kmaxb;
+    const int nr = spectral_rows(bs, st[ST_SIGMULT], sm.rowk2, sm.rowof, kmaxb);
+    if (tid == 0) {
+      sm.isc[0] = nr;
+      sm.isc[1] = kmaxb;
+    }
   }
   __syncthreads();
   const int nrows = sm.isc[0];
@@ -944,9 +952,10 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
   const double sgneg = (9 * sigma * sigma > var_noi) ? sigma : sgpos;
 
   load_twiddles(bs, sm);
+  int kmaxb;
+  const int nr_rows = spectral_rows(bs, sigmult, sm.rowk2, sm.rowof, kmaxb);
   if (tid == 0) {
-    int kmaxb;
-    sm.isc[0] = spectral_rows(bs, sigmult, sm.rowk2, sm.rowof, kmaxb);
+    sm.isc[0] = nr_rows;
     // findSplitPoint search range (src/PDFactory.cc:790-801)
     const double n1sigpos = 2.5 * sgpos, n1signeg = 2.5 * sgneg;
     double bd = n1sigpos / dflux;
