@@ -1245,6 +1245,7 @@ typedef struct orc_pdfac {
   int pd_islog;
   unsigned last_split;
   double last_peak, last_integral;
+  int split_shift; /* test hook: added to the split index found */
 } orc_pdfac;
 
 static orc_pdfac *orc_pdfac_new(unsigned n, unsigned ninterp, int precise) {
@@ -1680,6 +1681,25 @@ static int orc_find_split(const orc_pdfac *p, unsigned *out, double *peakout,
                   topidx, botidx, jitter, minval);
 }
 
+/* findSplitPoint on a caller's array (src/PDFactory.cc:691-921), for
+   hand-built cases: the members the reference reads (currsize, dflux,
+   sgpos, sgneg, minflux_R, wrapRidx) are given directly */
+int orc_find_split_array(const double *pofd, unsigned n, double dflux,
+                         double sgpos, double sgneg, double minflux_R,
+                         unsigned wrapRidx, unsigned *out, double *peak,
+                         orc_err *err) {
+  orc_pdfac p;
+  memset(&p, 0, sizeof p);
+  p.pofd = (double *)pofd;
+  p.currsize = n;
+  p.dflux = dflux;
+  p.sgpos = sgpos;
+  p.sgneg = sgneg;
+  p.minflux_R = minflux_R;
+  p.wrapRidx = wrapRidx;
+  return orc_find_split(&p, out, peak, err);
+}
+
 /* PDFactory::unwrapAndNormalizePD, src/PDFactory.cc:934-1009 */
 static int orc_unwrap_normalize(orc_pdfac *p, orc_err *err) {
   unsigned n = p->currsize;
@@ -1690,6 +1710,14 @@ static int orc_unwrap_normalize(orc_pdfac *p, orc_err *err) {
   int rc = orc_find_split(p, &splitidx, &peak, err);
   p->last_peak = peak;
   if (rc) return rc;
+  if (p->split_shift) {
+    /* test hook (logL insensitivity to the split inside the jitter floor) */
+    long sh = (long)splitidx + p->split_shift;
+    if (sh < 1 || sh >= (long)n)
+      return orc_fail(err, ORC_E_ARG, "PDFactory", "unwrapAndNormalizePD",
+                      "split shift leaves the array");
+    splitidx = (unsigned)sh;
+  }
   p->last_split = splitidx;
   memcpy(p->pd, p->pofd + splitidx, (n - splitidx) * sizeof(double));
   memcpy(p->pd + n - splitidx, p->pofd, splitidx * sizeof(double));
@@ -1807,6 +1835,7 @@ struct orc_like {
   orc_pdfac *fac; /* single-threaded path workspace */
   int setup_done;
   int precise;
+  int split_shift; /* test hook, see orc_pdfac */
 };
 
 orc_like *orc_like_new(unsigned fftsize, unsigned ninterp) {
@@ -2020,8 +2049,11 @@ static int orc_loglike_ws(const orc_like *L, orc_knots *model, orc_pdfac *p,
 
 static orc_pdfac *orc_like_fac(orc_like *L) {
   if (!L->fac) L->fac = orc_pdfac_new(L->fftsize, L->ninterp, L->precise);
+  L->fac->split_shift = L->split_shift;
   return L->fac;
 }
+
+void orc_like_set_split_shift(orc_like *L, int shift) { L->split_shift = shift; }
 
 int orc_like_loglike(orc_like *L, const float *theta, unsigned npar,
                      double *loglike) {
@@ -2044,6 +2076,7 @@ typedef struct orc_job {
 static void *orc_worker(void *arg) {
   orc_job *j = (orc_job *)arg;
   orc_pdfac *p = orc_pdfac_new(j->L->fftsize, j->L->ninterp, j->L->precise);
+  p->split_shift = j->L->split_shift;
   orc_knots model = j->L->model;
   orc_err err;
   for (unsigned w = j->tid; w < j->nwalk; w += j->nthreads) {

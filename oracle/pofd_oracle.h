@@ -164,6 +164,15 @@ const orc_err *orc_like_error(const orc_like *L);
    exact-math value of the reference's pipeline); 0: plain double as the
    reference computes it (default). */
 void orc_like_set_precise(orc_like *L, int precise);
+/* test hook: every split index found is moved by `shift` before the
+   rotation (0 = the reference's behaviour) */
+void orc_like_set_split_shift(orc_like *L, int shift);
+/* PDFactory::findSplitPoint (src/PDFactory.cc:691-921) on a given array,
+   with the factory members it reads passed in */
+int orc_find_split_array(const double *pofd, unsigned n, double dflux,
+                         double sgpos, double sgneg, double minflux_R,
+                         unsigned wrapRidx, unsigned *out, double *peak,
+                         orc_err *err);
 int orc_like_set_knot_positions(orc_like *L, unsigned n, const double *pos);
 /* adds a beam set (takes a copy of the beam); returns its index */
 int orc_like_add_beamset(orc_like *L, const orc_beam *bm);

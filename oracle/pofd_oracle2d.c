@@ -1498,6 +1498,13 @@ static int orc2_find_split(const double *data, unsigned cs, double dflux,
   return orc2_fail(err, ORC_E_SPLIT, "PDFactoryDouble", "findSplitPoint", "Unable to find split point");
 }
 
+/* findSplitPoint on a caller's marginal (hand-built cases) */
+int orc2_find_split_array(const double *data, unsigned cs, double dflux,
+                          double sigmapos, double sigmaneg, unsigned *out,
+                          orc_err *err) {
+  return orc2_find_split(data, cs, dflux, sigmapos, sigmaneg, out, err);
+}
+
 /* unwrapAndNormalizePD (:978-1118) into p->pd */
 static int orc2_unwrap_normalize(orc_pdfac2 *p, orc_err *err) {
   const unsigned cs = p->n;

@@ -35,6 +35,12 @@
 extern "C" {
 #endif
 
+/* PDFactoryDouble::findSplitPoint (src/PDFactoryDouble.cc:784-975) on a
+   given marginal of length cs */
+int orc2_find_split_array(const double *data, unsigned cs, double dflux,
+                          double sigmapos, double sigmaneg, unsigned *out,
+                          orc_err *err);
+
 /* ---------------- doublebeam ---------------- */
 typedef struct orc_dbeam {
   double pixsize, minval;

@@ -88,6 +88,10 @@ def lib():
         "orc_like_free": (None, [C.c_void_p]),
         "orc_like_error": (C.POINTER(OrcErr), [C.c_void_p]),
         "orc_like_set_precise": (None, [C.c_void_p, C.c_int]),
+        "orc_like_set_split_shift": (None, [C.c_void_p, C.c_int]),
+        "orc_find_split_array": (C.c_int, [DP, C.c_uint, D, D, D, D, C.c_uint,
+                                           C.POINTER(C.c_uint), DP,
+                                           C.POINTER(OrcErr)]),
         "orc_like_set_knot_positions": (C.c_int, [C.c_void_p, C.c_uint, DP]),
         "orc_like_add_beamset": (C.c_int, [C.c_void_p, C.c_void_p]),
         "orc_like_add_field": (C.c_int, [C.c_void_p, C.c_int, DP, C.c_size_t,
@@ -333,6 +337,21 @@ def pd_loglike_unbinned(pd, minflux, dflux, flux, islog=True):
     return v, comp.value
 
 
+def find_split(pofd, dflux, sgpos, sgneg, minflux_r, wrap_ridx):
+    """PDFactory::findSplitPoint (src/PDFactory.cc:691-921) on `pofd`.
+    Returns (split index, peak); raises OracleError where it throws."""
+    a = _f64(pofd)
+    out = C.c_uint(0)
+    peak = C.c_double(0.0)
+    err = OrcErr()
+    rc = lib().orc_find_split_array(_dp(a), a.size, dflux, sgpos, sgneg,
+                                    minflux_r, wrap_ridx, C.byref(out),
+                                    C.byref(peak), C.byref(err))
+    if rc:
+        raise OracleError(rc, err.text())
+    return int(out.value), float(peak.value)
+
+
 class Like:
     """calcLike restated: beam sets, fields, R ranges, getLogLike."""
 
@@ -360,6 +379,11 @@ class Like:
         arithmetic); 2 and 3: analysis modes (extended forward + double
         inverse; extended transforms with X_k, R^_0 rounded to double)."""
         lib().orc_like_set_precise(self.ptr, int(precise))
+
+    def set_split_shift(self, shift):
+        """Test hook: move every split index found by `shift` before the
+        rotation (0 restores the reference's behaviour)."""
+        lib().orc_like_set_split_shift(self.ptr, int(shift))
 
     def add_beamset(self, beam):
         return lib().orc_like_add_beamset(self.ptr, beam.ptr)

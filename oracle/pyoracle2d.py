@@ -28,6 +28,8 @@ def lib():
         return L
     sig = {
         "orc_rfft2": (C.c_int, [C.c_uint, DP, DP]),
+        "orc2_find_split_array": (C.c_int, [DP, C.c_uint, D, D, D,
+                                            C.POINTER(C.c_uint), EP]),
         "orc_irfft2": (C.c_int, [C.c_uint, DP, DP]),
         "orc_dbeam_new": (VP, []),
         "orc_dbeam_free": (None, [VP]),
@@ -101,6 +103,18 @@ def _f64(a):
 def _check(rc, err):
     if rc:
         raise O.OracleError(rc, err.text())
+
+
+def find_split2(data, dflux, sigmapos, sigmaneg):
+    """PDFactoryDouble::findSplitPoint (src/PDFactoryDouble.cc:784-975) on a
+    marginal; raises OracleError where it throws."""
+    a = _f64(data)
+    out = C.c_uint(0)
+    err = O.OrcErr()
+    _check(lib().orc2_find_split_array(_dp(a), a.size, dflux, sigmapos,
+                                       sigmaneg, C.byref(out), C.byref(err)),
+           err)
+    return int(out.value)
 
 
 def rfft2(x):
