@@ -198,7 +198,14 @@ int pofd1d_set_priors(pofd_ctx *ctx, int flags, double sigma_width,
    the capture; d_theta, d_loglike_out and d_status_out are bound to the
    graph (write new theta rows into d_theta between launches).  Stage
    timing must be off.  Priors set on any of the contexts are added once by
-   that context. */
+   that context.
+   The graph holds the contexts' device pointers and launch parameters as
+   they were at the capture.  Anything that replaces them afterwards -- an
+   eager batch larger than any before (the workspace grows), prepare or
+   set_rrange (tables uploaded again), configure, set_option,
+   set_parts, set_priors, autotune_parts -- makes pofd1d_graph_launch
+   return POFD_E_ARG (message on that context) without launching; create
+   the graph again.  The contexts must outlive the graph. */
 typedef struct pofd_graph pofd_graph;
 int pofd1d_graph_create(pofd_ctx **ctxs, uint32_t n_ctx, const float *d_theta,
                         uint32_t n_walkers, uint32_t theta_stride,

@@ -94,6 +94,10 @@ struct pofd_ctx {
   std::vector<std::array<hipEvent_t, POFD_NSTAGES + 1>> stage_events;
   size_t stage_used = 0;
   pofd2::Ctx2* c2 = nullptr;  // two-band state (capi2d.cpp)
+  // bumped whenever device tables or workspaces are replaced, or a launch
+  // parameter captured by value changes; a captured graph records it and
+  // refuses to launch once it differs
+  uint64_t generation = 0;
   ~pofd_ctx() {
     for (auto& ev : stage_events)
       for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -504,6 +508,7 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   }
   s.wcap = 0;  // workspace layout depends on the tables
   s.prepared = true;
+  ++c->generation;
   return POFD_OK;
 }
 
@@ -579,6 +584,7 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   w.ipart = base;
   w.nslots = nslots;
   s.wcap = nwalk;
+  ++c->generation;
   return POFD_OK;
 }
 
@@ -636,6 +642,7 @@ extern "C" int pofd_set_option(pofd_ctx* c, int option, double value) {
       return fail(c, POFD_E_ARG, "pofd_set_option: spectral cut must be > 0");
     c->spectral_cut = value;
     for (auto& s : c->sets) s->dev.spectral_cut = value;
+    ++c->generation;
     return POFD_OK;
   }
   if (option == POFD_OPT_STAGE_TIMING) {
@@ -675,6 +682,7 @@ extern "C" int pofd1d_configure(pofd_ctx* c, uint32_t nflux, uint32_t ninterp,
   c->exp_conf = exp_conf;
   c->configured = true;
   for (auto& s : c->sets) s->prepared = false;
+  ++c->generation;
   return POFD_OK;
 }
 
@@ -778,6 +786,7 @@ extern "C" int pofd1d_set_parts(pofd_ctx* c, int parts) {
   if (parts < 0 || parts > 4)
     return fail(c, POFD_E_ARG, "pofd1d_set_parts: parts must be 0 to 4");
   c->force_parts = parts;
+  ++c->generation;
   return POFD_OK;
 }
 
@@ -838,6 +847,7 @@ extern "C" int pofd1d_set_priors(pofd_ctx* c, int flags, double sigma_width, dou
     return fail(c, POFD_E_ARG, "pofd1d_set_priors: invalid prior width or alpha");
   c->priors = pofd::PriorDev{flags, sigma_width, cfirb_mean, cfirb_sigma, poisson_mean,
                              poisson_sigma, reg_alpha};
+  ++c->generation;
   return POFD_OK;
 }
 
@@ -868,6 +878,7 @@ extern "C" int pofd1d_autotune_parts(pofd_ctx* c, const float* theta, uint32_t s
   if (nsizes == 0) {  // back to the size rule
     c->parts_tuned = false;
     c->parts_tab.fill(0);
+    ++c->generation;
     return POFD_OK;
   }
   if (!theta || !sizes || !chosen || reps == 0)
@@ -956,6 +967,7 @@ extern "C" int pofd1d_autotune_parts(pofd_ctx* c, const float* theta, uint32_t s
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(parts table)");
   c->parts_tab = tab;
   c->parts_tuned = true;
+  ++c->generation;
   return POFD_OK;
 }
 
@@ -965,6 +977,8 @@ extern "C" int pofd1d_autotune_parts(pofd_ctx* c, const float* theta, uint32_t s
 struct pofd_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the contexts captured and their generations at the capture
+  std::vector<std::pair<pofd_ctx*, uint64_t>> ctxs;
   ~pofd_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1003,12 +1017,20 @@ extern "C" int pofd1d_graph_create(pofd_ctx** ctxs, uint32_t nctx, const float* 
     delete g;
     return rc;
   }
+  // recorded after the capture: the captured launches allocate nothing, so
+  // the generations are those the graph's pointers belong to
+  for (uint32_t i = 0; i < nctx; ++i) g->ctxs.emplace_back(ctxs[i], ctxs[i]->generation);
   *out = g;
   return POFD_OK;
 }
 
 extern "C" int pofd1d_graph_launch(pofd_graph* g, void* stream) {
   if (!g || !g->exec) return POFD_E_ARG;
+  for (auto& cg : g->ctxs)
+    if (cg.first->generation != cg.second)
+      return fail(cg.first, POFD_E_ARG,
+                  "pofd1d_graph_launch: a context's workspace, tables or launch "
+                  "parameters changed after the capture; create the graph again");
   return hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) == hipSuccess
              ? POFD_OK
              : POFD_E_INTERNAL;

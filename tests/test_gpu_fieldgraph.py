@@ -52,3 +52,39 @@ def test_field_graph_replays_match_eager():
         assert np.all(st == 0)
         np.testing.assert_allclose(ll, want, rtol=0, atol=1e-9)
         _say("replay %d matches" % r)
+
+
+def test_field_graph_refuses_after_workspace_growth():
+    """A larger eager batch replaces a context's workspace; the graph that
+    captured the old one must refuse to launch (POFD_E_ARG) instead of
+    touching freed memory, and a graph captured again works."""
+    _say("graph invalidated by a larger eager batch")
+    inp = workload.make_inputs(npix_side=120, sigma=0.003, seed=70)
+    ctx = workload.build_context(inp, 32768, 512, binned=True)
+    W = 8
+    th = synth.walker_thetas(W, seed=9)
+    stride = th.shape[1]
+    d_th = hiprt.DeviceBuffer(th.nbytes)
+    d_ll = hiprt.DeviceBuffer(W * 8)
+    d_st = hiprt.DeviceBuffer(W * 4)
+    stream = hiprt.Stream()
+    d_th.upload(th)
+    g = _lib.FieldGraph([ctx], d_th.value, W, stride, d_ll.value,
+                        d_st.value, stream.ptr.value)
+    g.launch(stream.ptr.value)
+    stream.synchronize()
+    want, s = ctx.loglike_batch(th)
+    ll = np.zeros(W)
+    d_ll.download(ll)
+    np.testing.assert_array_equal(ll, want)
+    big = synth.walker_thetas(4 * W, seed=10)
+    ctx.loglike_batch(big)                 # grows the workspace
+    with pytest.raises(_lib.PofdError) as ei:
+        g.launch(stream.ptr.value)
+    assert ei.value.code == 5  # POFD_E_ARG
+    g2 = _lib.FieldGraph([ctx], d_th.value, W, stride, d_ll.value,
+                         d_st.value, stream.ptr.value)
+    g2.launch(stream.ptr.value)
+    stream.synchronize()
+    d_ll.download(ll)
+    np.testing.assert_array_equal(ll, want)
