@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--ninterp", type=int, default=2048)
     ap.add_argument("--side", type=int, default=1000)
     ap.add_argument("--unbinned-steps", type=int, default=10)
-    ap.add_argument("--cpu-evals", type=int, default=256)
+    ap.add_argument("--cpu-evals", type=int, default=1024)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pcie-steps", type=int, default=10,
@@ -246,26 +246,94 @@ def main():
     if args.multifield_steps > 0:
         out["multifield"] = bench_multifield(args, hiprt)
     if not args.no_cpu and args.cpu_evals > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from oracle_side import build_oracle
-        nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
-        ne = min(args.cpu_evals, W)
-        L = build_oracle(inp, N, args.ninterp, True, th[0])
-        t0 = time.perf_counter()
-        ref, rst = L.loglike_batch(th[:ne], nthr)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {
-            "value": ne / dt, "unit": "evals/s", "cores": nthr,
-            "kind": "port",
-            "sample": "%d walkers of the same workload (binned, nflux 2^17) "
-                      "on %d host threads, one evaluation per thread at a "
-                      "time; own radix-2 FFT (FFTW is absent)" % (ne, nthr),
-        }
-        ok = (rst == 0) & (b["st"][:ne] == 0)
-        out["parity_max_abs_dlogL"] = float(
-            np.max(np.abs(ref[ok] - b["ll"][:ne][ok]))) if ok.any() else None
+        out["cpu_baseline"], par = cpu_baseline_1d(args, inp, th, True,
+                                                   b["ll"], b["st"])
+        out.update(par)
+        if "unbinned" in out:
+            ub, upar = cpu_baseline_1d(args, inp, th, False, u["ll"],
+                                       u["st"], evals=args.cpu_evals // 2)
+            out["unbinned"]["cpu_baseline"] = ub
+            out["unbinned"].update(upar)
     print(json.dumps(out))
+
+
+def host_cpu():
+    """(usable host threads, capped at the 16 a one-GPU box grants; CPU
+    model name)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(16, n), model
+
+
+def cpu_baseline_1d(args, inp, th, binned, gpu_ll, gpu_st, evals=None):
+    """The oracle (a plain-double restatement of the reference CPU path) on
+    the host: one evaluation per worker thread at a time, as the
+    reference's MPI slaves do (the master computes nothing,
+    src/pofdMCMC.cc:200), with nproc - 1 and nproc workers, the per-stage
+    split in the reference's TIMING categories, and the GPU's distance from
+    the double oracle and from the extended-precision (exact-math) one."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_side import build_oracle
+    N, W = args.nflux, len(th)
+    nproc, model = host_cpu()
+    if args.cpu_threads:
+        nproc = args.cpu_threads
+    ne = evals or args.cpu_evals
+    reps = -(-ne // W)
+    sample = np.concatenate([th] * reps)[:ne]
+    L = build_oracle(inp, N, args.ninterp, binned, th[0])
+    L.loglike_batch(th[:nproc], nproc)  # one warm-up evaluation per worker
+    runs = {}
+    for nw in sorted({max(1, nproc - 1), nproc}):
+        L.set_timing(True)
+        t0 = time.perf_counter()
+        L.loglike_batch(sample, nw)
+        dt = time.perf_counter() - t0
+        tm = L.timing()
+        tot = sum(tm.values())
+        runs[nw] = dict(value=ne / dt, wall_s=dt, cpu_s=tot,
+                        stage_frac={k: v / tot for k, v in tm.items()})
+    L.set_timing(False)
+    best = runs[nproc]
+    cb = {
+        "value": best["value"], "unit": "evals/s", "cores": nproc,
+        "kind": "port",
+        "sample": "%d evaluations (the %d walkers of the same workload, "
+                  "%s, nflux %d, cycled) per worker count, one evaluation "
+                  "per host thread at a time; own radix-2 FFT (FFTW is "
+                  "absent)" % (ne, W, "binned" if binned else "unbinned", N),
+        "cpu_model": model,
+        "workers": {str(k): {"value": v["value"], "wall_s": v["wall_s"]}
+                    for k, v in runs.items()},
+        "stage_frac": best["stage_frac"],
+        "cpu_s_per_eval": best["cpu_s"] / ne,
+    }
+    dbl, dst = L.loglike_batch(th, nproc)
+    L.set_precise(1)
+    ext, est = L.loglike_batch(th, nproc)
+    ok = (dst == 0) & (est == 0) & (gpu_st == 0)
+    par = {}
+    if ok.any():
+        par = {
+            "parity_max_abs_dlogL": float(np.max(np.abs(dbl[ok] - gpu_ll[ok]))),
+            "parity_max_abs_dlogL_vs_exact":
+                float(np.max(np.abs(ext[ok] - gpu_ll[ok]))),
+            "oracle_double_vs_exact": float(np.max(np.abs(ext[ok] - dbl[ok]))),
+            "parity_walkers": int(ok.sum()),
+        }
+    return cb, par
 
 
 def bench_multifield(args, hiprt):
@@ -646,7 +714,8 @@ def bench_2d(args, hiprt):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle2d_side import build_oracle2d
         ne = min(args.cpu_evals_2d, W)
-        nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        nthr, model = host_cpu()
+        nthr = args.cpu_threads or nthr
         L = build_oracle2d(inp, n, synth2d.true_theta())
         t0 = time.perf_counter()
         ref, rst = L.loglike_batch(th[:ne], nthr)
@@ -656,7 +725,8 @@ def bench_2d(args, hiprt):
             "kind": "port",
             "sample": "%d walkers of the same 2D workload on %d host "
                       "threads, one evaluation per thread at a time; own "
-                      "radix-2 FFT (FFTW is absent)" % (ne, nthr)}
+                      "radix-2 FFT (FFTW is absent)" % (ne, nthr),
+            "cpu_model": model}
         ok = (rst == 0) & (st[:ne] == 0)
         res["parity_max_abs_dlogL"] = float(
             np.max(np.abs(ref[ok] - ll[:ne][ok]))) if ok.any() else None

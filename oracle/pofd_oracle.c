@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* include/global_settings.h:20-21, 66-74 */
 #define ORC_PI 3.141592653589793238462643383279502884197
@@ -1246,7 +1247,26 @@ typedef struct orc_pdfac {
   unsigned last_split;
   double last_peak, last_integral;
   int split_shift; /* test hook: added to the split index found */
+  /* the reference's TIMING categories (src/PDFactory.cc:93-125), thread
+     CPU seconds, accumulated while `timing` is set */
+  int timing;
+  double tsec[ORC_NTIME];
 } orc_pdfac;
+
+static double orc_now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_THREAD_CPUTIME_ID, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* adds the time since *t0 to category k and restarts the clock */
+static void orc_tick(orc_pdfac *p, double *t0, int k) {
+  if (!p->timing) return;
+  double t1 = orc_now();
+  p->tsec[k] += t1 - *t0;
+  *t0 = t1;
+}
+static double orc_tstart(const orc_pdfac *p) { return p->timing ? orc_now() : 0.0; }
 
 static orc_pdfac *orc_pdfac_new(unsigned n, unsigned ninterp, int precise) {
   orc_pdfac *p = (orc_pdfac *)calloc(1, sizeof(orc_pdfac));
@@ -1502,9 +1522,12 @@ static int orc_initpd(orc_pdfac *p, unsigned n, double minflux,
   if (n != p->currsize || !p->fft)
     return orc_fail(err, ORC_E_ARG, "PDFactory", "setupTransforms",
                     "Transform size %u not supported by this oracle", n);
+  double t0 = orc_tstart(p);
   int rc = orc_init_r(p, n, minflux, maxflux, model, bm, err);
   if (rc) return rc;
+  orc_tick(p, &t0, ORC_T_R);
   orc_mean_var_from_r(p);
+  orc_tick(p, &t0, ORC_T_OTHER);
   p->sgpos = sqrt(p->var_noi);
   p->doshift = (p->sgpos > p->dflux) && (fabs(p->mn) > p->dflux);
   p->shift = p->doshift ? -p->mn : 0.0;
@@ -1535,6 +1558,7 @@ static int orc_initpd(orc_pdfac *p, unsigned n, double minflux,
     return ORC_OK;
   }
   orc_rfft_plan(p->fft, p->rvals, p->prtrans);
+  orc_tick(p, &t0, ORC_T_FFT);
   double *pr = p->prtrans;
   double iflux = orc_two_pi / (n * p->dflux);
   unsigned ncplx = n / 2 + 1;
@@ -1558,6 +1582,7 @@ static int orc_initpd(orc_pdfac *p, unsigned n, double minflux,
       pr[2 * idx + 1] = expfac * sin(ival);
     }
   }
+  orc_tick(p, &t0, ORC_T_P0);
   return ORC_OK;
 }
 
@@ -1703,13 +1728,16 @@ int orc_find_split_array(const double *pofd, unsigned n, double dflux,
 /* PDFactory::unwrapAndNormalizePD, src/PDFactory.cc:934-1009 */
 static int orc_unwrap_normalize(orc_pdfac *p, orc_err *err) {
   unsigned n = p->currsize;
+  double t0 = orc_tstart(p);
   for (unsigned i = 0; i < n; ++i)
     if (p->pofd[i] < 0) p->pofd[i] = 0.0;
+  orc_tick(p, &t0, ORC_T_POS);
   unsigned splitidx = 0;
   double peak = 0.0;
   int rc = orc_find_split(p, &splitidx, &peak, err);
   p->last_peak = peak;
   if (rc) return rc;
+  orc_tick(p, &t0, ORC_T_SPLIT);
   if (p->split_shift) {
     /* test hook (logL insensitivity to the split inside the jitter floor) */
     long sh = (long)splitidx + p->split_shift;
@@ -1725,12 +1753,15 @@ static int orc_unwrap_normalize(orc_pdfac *p, orc_err *err) {
   p->pd_islog = 0;
   p->pd_minflux = 0.0;
   p->pd_dflux = p->dflux;
+  orc_tick(p, &t0, ORC_T_COPY);
   /* PD::normalize src/PD.cc:146-161 */
   double tot = orc_pd_integral(n, p->pd, p->pd_dflux, 0);
   p->last_integral = tot;
   double itot = 1.0 / tot;
   for (unsigned i = 0; i < n; ++i) p->pd[i] *= itot;
+  orc_tick(p, &t0, ORC_T_NORM);
   double tmn = orc_pd_mean(n, p->pd, 0.0, p->pd_dflux, 0, 0);
+  orc_tick(p, &t0, ORC_T_MEAN);
   if (isinf(tmn) || isnan(tmn))
     return orc_fail(err, ORC_E_MEAN, "PDFactory", "unwrapandNormalizePD",
                     "Un-shift amounts not finite: %g At length: %u with noise "
@@ -1771,6 +1802,7 @@ static int orc_getpd(orc_pdfac *p, double sigma, int setlog, orc_err *err) {
     }
     orc_irfft_l(p->fft, lv, p->pofd);
   } else if (sigma > 0) {
+    double t0 = orc_tstart(p);
     double iflux = orc_two_pi / (n * p->dflux);
     double sigfac = -0.5 * sigma * sigma;
     p->pval[0] = 1.0;
@@ -1781,9 +1813,13 @@ static int orc_getpd(orc_pdfac *p, double sigma, int setlog, orc_err *err) {
       p->pval[2 * idx] = expfac * p->prtrans[2 * idx];
       p->pval[2 * idx + 1] = expfac * p->prtrans[2 * idx + 1];
     }
+    orc_tick(p, &t0, ORC_T_P0);
     orc_irfft_plan(p->fft, p->pval, p->pofd);
+    orc_tick(p, &t0, ORC_T_FFT);
   } else {
+    double t0 = orc_tstart(p);
     orc_irfft_plan(p->fft, p->prtrans, p->pofd);
+    orc_tick(p, &t0, ORC_T_FFT);
   }
   p->sgpos = sqrt(p->var_noi + sigma * sigma);
   if (9 * sigma * sigma > p->var_noi)
@@ -1793,12 +1829,14 @@ static int orc_getpd(orc_pdfac *p, double sigma, int setlog, orc_err *err) {
   int rc = orc_unwrap_normalize(p, err);
   if (rc) return rc;
   if (setlog) {
+    double t0 = orc_tstart(p);
     /* PD::applyLog(false) src/PD.cc:169-184 */
     for (unsigned i = 0; i < n; ++i) {
       double val = p->pd[i];
       p->pd[i] = (val > 0.0) ? log2(val) : orc_smalllogval;
     }
     p->pd_islog = 1;
+    orc_tick(p, &t0, ORC_T_LOG);
   }
   return ORC_OK;
 }
@@ -1836,10 +1874,14 @@ struct orc_like {
   int setup_done;
   int precise;
   int split_shift; /* test hook, see orc_pdfac */
+  int timing;      /* per-stage timing of the batch workers */
+  double tsec[ORC_NTIME];
+  pthread_mutex_t tlock;
 };
 
 orc_like *orc_like_new(unsigned fftsize, unsigned ninterp) {
   orc_like *L = (orc_like *)calloc(1, sizeof(orc_like));
+  pthread_mutex_init(&L->tlock, NULL);
   L->fftsize = fftsize;
   L->ninterp = ninterp;
   return L;
@@ -2039,7 +2081,9 @@ static int orc_loglike_ws(const orc_like *L, orc_knots *model, orc_pdfac *p,
     for (unsigned f = 0; f < s->nfields; ++f) {
       rc = orc_getpd(p, sigmult * s->fields[f].sigma_base, 1, err);
       if (rc) return rc;
+      double t0 = orc_tstart(p);
       double curr = orc_field_loglike_pd(p, &s->fields[f], NULL);
+      orc_tick(p, &t0, ORC_T_LIKE);
       LogLike += (curr - s->fields[f].like_offset) / s->fields[f].like_norm;
     }
   }
@@ -2077,6 +2121,7 @@ static void *orc_worker(void *arg) {
   orc_job *j = (orc_job *)arg;
   orc_pdfac *p = orc_pdfac_new(j->L->fftsize, j->L->ninterp, j->L->precise);
   p->split_shift = j->L->split_shift;
+  p->timing = j->L->timing;
   orc_knots model = j->L->model;
   orc_err err;
   for (unsigned w = j->tid; w < j->nwalk; w += j->nthreads) {
@@ -2087,8 +2132,22 @@ static void *orc_worker(void *arg) {
     j->loglike[w] = rc ? NAN : v;
     j->status[w] = rc;
   }
+  if (p->timing) {
+    pthread_mutex_lock(&j->L->tlock);
+    for (int k = 0; k < ORC_NTIME; ++k) j->L->tsec[k] += p->tsec[k];
+    pthread_mutex_unlock(&j->L->tlock);
+  }
   orc_pdfac_free(p);
   return NULL;
+}
+
+void orc_like_set_timing(orc_like *L, int on) {
+  L->timing = on;
+  memset(L->tsec, 0, sizeof L->tsec);
+}
+
+void orc_like_timing(const orc_like *L, double *out) {
+  memcpy(out, L->tsec, sizeof L->tsec);
 }
 
 int orc_like_loglike_batch(orc_like *L, const float *theta, unsigned nwalk,

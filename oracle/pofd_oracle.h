@@ -164,6 +164,16 @@ const orc_err *orc_like_error(const orc_like *L);
    exact-math value of the reference's pipeline); 0: plain double as the
    reference computes it (default). */
 void orc_like_set_precise(orc_like *L, int precise);
+/* the reference's TIMING categories (src/PDFactory.cc:93-125) plus the
+   likelihood sum and what none of them covers; thread CPU seconds summed
+   over the batch workers */
+enum { ORC_T_R = 0, ORC_T_P0, ORC_T_FFT, ORC_T_POS, ORC_T_SPLIT, ORC_T_COPY,
+       ORC_T_NORM, ORC_T_MEAN, ORC_T_LOG, ORC_T_LIKE, ORC_T_OTHER,
+       ORC_NTIME };
+/* on != 0: the batch workers time their stages; resets the totals */
+void orc_like_set_timing(orc_like *L, int on);
+/* out[ORC_NTIME]: totals since the last orc_like_set_timing */
+void orc_like_timing(const orc_like *L, double *out);
 /* test hook: every split index found is moved by `shift` before the
    rotation (0 = the reference's behaviour) */
 void orc_like_set_split_shift(orc_like *L, int shift);

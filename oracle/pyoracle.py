@@ -89,6 +89,8 @@ def lib():
         "orc_like_error": (C.POINTER(OrcErr), [C.c_void_p]),
         "orc_like_set_precise": (None, [C.c_void_p, C.c_int]),
         "orc_like_set_split_shift": (None, [C.c_void_p, C.c_int]),
+        "orc_like_set_timing": (None, [C.c_void_p, C.c_int]),
+        "orc_like_timing": (None, [C.c_void_p, DP]),
         "orc_find_split_array": (C.c_int, [DP, C.c_uint, D, D, D, D, C.c_uint,
                                            C.POINTER(C.c_uint), DP,
                                            C.POINTER(OrcErr)]),
@@ -379,6 +381,21 @@ class Like:
         arithmetic); 2 and 3: analysis modes (extended forward + double
         inverse; extended transforms with X_k, R^_0 rounded to double)."""
         lib().orc_like_set_precise(self.ptr, int(precise))
+
+    TIME_KEYS = ("R", "p0", "fft", "pos", "split", "copy", "norm", "mean",
+                 "log", "like", "other")
+
+    def set_timing(self, on=True):
+        """Per-stage timing of the batch workers in the reference's TIMING
+        categories (src/PDFactory.cc:93-125) plus the likelihood sum and
+        the rest; resets the totals."""
+        lib().orc_like_set_timing(self.ptr, int(on))
+
+    def timing(self):
+        """Thread CPU seconds per stage since the last set_timing."""
+        out = np.zeros(len(self.TIME_KEYS))
+        lib().orc_like_timing(self.ptr, _dp(out))
+        return dict(zip(self.TIME_KEYS, out.tolist()))
 
     def set_split_shift(self, shift):
         """Test hook: move every split index found by `shift` before the
