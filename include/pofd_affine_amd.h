@@ -236,6 +236,32 @@ int pofd1d_autotune_parts(pofd_ctx *ctx, const float *theta, uint32_t theta_stri
                           const int32_t *sizes, uint32_t n_sizes, uint32_t reps,
                           int32_t *chosen);
 
+/* ---- log likelihood of data given a caller's P(D) ---- */
+/* Replaces double PD::getLogLike(const fitsData&) const (include/PD.h:86,
+   src/PD.cc:538-677): the P(D) pd[n] on the grid minflux + i*dflux (log2
+   values when is_log, as PD::logflat), linearly interpolated at every
+   pixel (pix/npix; edges x <= minflux, x >= maxflux - dflux) or at every
+   bin centre bincent0 + i*bindelta weighted by bins[i] (edges < and >),
+   summed and converted to natural log.  Host buffers in, *loglike_out out;
+   runs on ctx's device (no other context state is used).  Errors as the
+   reference throws them: no P(D), no data. */
+int pofd_pd_loglike(pofd_ctx *ctx, const double *pd, uint32_t n, double minflux,
+                    double dflux, int is_log, const double *pix, uint64_t npix,
+                    const uint32_t *bins, uint32_t nbins, double bincent0,
+                    double bindelta, double *loglike_out);
+/* Replaces double PDDouble::getLogLike(const fitsDataDouble&) const
+   (include/PDDouble.h:95, src/PDDouble.cc:976-1222): pd is n1 x n2 with the
+   band-1 index outer (PDDouble::pd_); bilinear interpolation at the pixel
+   pairs (pix1, pix2) or at the nbins1 x nbins2 histogram (band-1 bin
+   index outer) with the reference's edge tests. */
+int pofd_pd2_loglike(pofd_ctx *ctx, const double *pd, uint32_t n1, uint32_t n2,
+                     double minflux1, double dflux1, double minflux2,
+                     double dflux2, int is_log, const double *pix1,
+                     const double *pix2, uint64_t npix, const uint32_t *bins,
+                     uint32_t nbins1, uint32_t nbins2, double bincent01,
+                     double bindelta1, double bincent02, double bindelta2,
+                     double *loglike_out);
+
 /* ---- parity / debug accessors for the most recent batch ---- */
 /* The P(D) as PDFactory::getPD(sigma, pd, setLog) leaves PD::pd_ (rotated,
    normalised, log2 when is_log > 0) plus pd.minflux / pd.dflux

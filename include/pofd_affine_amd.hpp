@@ -23,8 +23,12 @@
  *    relative 1e-5;
  *  - PDFactoryDouble keeps a pointer to the doublebeam given to initPD,
  *    which must outlive it;
- *  - PD::getLogLike is not provided: log likelihoods come from
- *    calcLike::getLogLike, which runs the whole chain on the device.
+ *  - PDFactory / PDFactoryDouble::initPD accept the abstract numberCounts /
+ *    numberCountsDouble, but the device path evaluates only the models this
+ *    header defines (numberCountsKnotsSpline, numberCountsDoubleLogNormal);
+ *    another derived model is rejected with an affineExcept;
+ *  - PD::getLogLike / PDDouble::getLogLike run on the GPU of the calling
+ *    thread's default device, through a context kept per thread.
  */
 #ifndef POFD_AFFINE_AMD_HPP
 #define POFD_AFFINE_AMD_HPP
@@ -178,6 +182,11 @@ class beam {
     return dblpair(p.front(), p.back());
   }
   const pofd_beam* handle() const { return h.get(); }
+  /* the same beam data: copies of one beam share it, and a histogram made
+     since this copy was taken changes the summary it holds */
+  bool sameAs(const beam& o) const {
+    return h == o.h && std::memcmp(info_, o.info_, sizeof info_) == 0;
+  }
 };
 
 /* include/fitsData.h: one map, optionally binned (host, bit-exact) */
@@ -261,8 +270,22 @@ class fitsData {
   std::string getFile() const { return file; }
 };
 
+/* include/numberCounts.h:21-70: the abstract one-band counts model (the part
+   of its interface the device path needs) */
+class numberCounts {
+ public:
+  virtual ~numberCounts() {}
+  virtual void setParams(const paramSet& params) = 0;
+  virtual bool isValid() const noexcept = 0;
+  virtual double getMinFlux() const noexcept = 0;
+  virtual double getMaxFlux() const noexcept = 0;
+  virtual dblpair getRRange(const beam&) const = 0;
+  /* numberCounts::differenceRegularize: no penalty unless a model has one */
+  virtual double differenceRegularize(double) const { return 0.0; }
+};
+
 /* include/numberCountsKnotsSpline.h: knot positions + float knot values */
-class numberCountsKnotsSpline {
+class numberCountsKnotsSpline : public numberCounts {
  private:
   std::vector<double> knots;
   std::vector<float> values;
@@ -293,7 +316,7 @@ class numberCountsKnotsSpline {
     setKnotPositions(d);
   }
   /* numberCountsKnotsSpline::setParams, src/numberCountsKnotsSpline.cc:265-274 */
-  void setParams(const paramSet& F) {
+  void setParams(const paramSet& F) override {
     if (knots.size() > F.getNParams())
       throw affineExcept("numberCountsKnotsSpline", "setKnots",
                          "Number of knot values different than expected");
@@ -301,7 +324,7 @@ class numberCountsKnotsSpline {
     loaded = true;
   }
   /* numberCountsKnots::isValid, src/numberCountsKnots.cc:273-284 */
-  bool isValid() const noexcept {
+  bool isValid() const noexcept override {
     if (knots.empty() || !loaded) return false;
     for (float v : values)
       if (std::isnan(v)) return false;
@@ -309,12 +332,12 @@ class numberCountsKnotsSpline {
   }
   unsigned int getNKnots() const noexcept { return static_cast<unsigned>(knots.size()); }
   double getKnotPos(unsigned int i) const noexcept { return knots[i]; }
-  double getMinFlux() const noexcept { return knots.empty() ? NAN : knots.front(); }
-  double getMaxFlux() const noexcept { return knots.empty() ? NAN : knots.back(); }
+  double getMinFlux() const noexcept override { return knots.empty() ? NAN : knots.front(); }
+  double getMaxFlux() const noexcept override { return knots.empty() ? NAN : knots.back(); }
   const std::vector<double>& positions() const { return knots; }
   const std::vector<float>& knotValues() const { return values; }
   /* numberCountsKnots::getRRangeInternal, src/numberCountsKnots.cc:345-365 */
-  dblpair getRRange(const beam& bm) const {
+  dblpair getRRange(const beam& bm) const override {
     if (!isValid()) throw affineExcept("numberCountsKnots", "getRRange", "Model is not valid");
     if (!bm.hasData()) throw affineExcept("numberCountsKnots", "getRRange", "Beam is empty");
     const double maxknot = knots.back();
@@ -324,7 +347,7 @@ class numberCountsKnotsSpline {
     return dblpair(-maxknot * bm.getMinMaxNeg().second, 0.0);
   }
   /* numberCountsKnotsSpline::differenceRegularize, src/numberCountsKnotsSpline.cc:540-559 */
-  double differenceRegularize(double alpha) const {
+  double differenceRegularize(double alpha) const override {
     if (!loaded) return NAN;
     const size_t n = knots.size();
     if (n < 2) return 0.0;
@@ -385,6 +408,8 @@ class PD {
     logflat = LOG;
     if (N) std::memcpy(pd_, DATA, N * sizeof(double));
   }
+  /* PD::getLogLike, include/PD.h:86, src/PD.cc:538-677 (on the device) */
+  double getLogLike(const fitsData& data) const;
 };
 
 namespace pofd_amd_detail {
@@ -414,10 +439,38 @@ inline CtxPtr make_ctx(int device, const char* cls, const char* meth) {
   if (rc) throw_status(nullptr, rc, cls, meth);
   return CtxPtr(c);
 }
+/* the calling thread's context for stand-alone device work (contexts are
+   bound to one host thread) */
+inline pofd_ctx* thread_ctx(const char* cls, const char* meth) {
+  thread_local CtxPtr c;
+  if (!c) c = make_ctx(default_device(), cls, meth);
+  return c.get();
+}
 }  // namespace pofd_amd_detail
 
+inline double PD::getLogLike(const fitsData& data) const {
+  if (pd_ == nullptr || getDim() == 0)
+    throw affineExcept("PD", "getLogLike", "pd not filled before likelihood calc");
+  if (data.getN() == 0) throw affineExcept("PD", "getLogLike", "No data present");
+  pofd_ctx* c = pofd_amd_detail::thread_ctx("PD", "getLogLike");
+  double out = 0.0;
+  int rc;
+  if (data.isBinned())
+    rc = pofd_pd_loglike(c, pd_, getDim(), minflux, dflux, logflat ? 1 : 0, nullptr, 0,
+                         reinterpret_cast<const uint32_t*>(data.getBinnedData()), data.getNBins(),
+                         data.getBinCent0(), data.getBinDelta(), &out);
+  else
+    rc = pofd_pd_loglike(c, pd_, getDim(), minflux, dflux, logflat ? 1 : 0, data.getData(),
+                         data.getN(), nullptr, 0, 0.0, 0.0, &out);
+  if (rc) pofd_amd_detail::throw_status(c, rc, "PD", "getLogLike");
+  return out;
+}
+
 /* include/PDFactory.h: initPD + getPD for one model, one noise level.
-   Each getPD runs the fused device kernel for a single walker. */
+   The device context is built on the first getPD and kept while the knot
+   positions, R range, beam and sigma stay those it was built for: repeated
+   getPD calls (new knot values through initPD, the same sigma) only run the
+   batch kernels. */
 class PDFactory {
  private:
   unsigned int ninterp;
@@ -427,25 +480,59 @@ class PDFactory {
   beam bm;
   bool initialized = false;
   int device = pofd_amd_detail::default_device();
+  pofd_amd_detail::CtxPtr ctx;
+  bool stale = true;  // the context predates a set-up change
+  double built_sigma = NAN;
+  int bid = 0, fid = 0;
+
+  void build(double sigma) {
+    using namespace pofd_amd_detail;
+    ctx.reset();
+    CtxPtr c = make_ctx(device, "PDFactory", "getPD");
+    const std::vector<double>& k = model.positions();
+    int rc = pofd1d_configure(c.get(), n, ninterp, k.data(), static_cast<uint32_t>(k.size()), 0.0);
+    if (!rc) rc = pofd1d_add_beamset(c.get(), bm.handle(), &bid);
+    const uint32_t one = 1;
+    if (!rc) rc = pofd1d_add_field(c.get(), bid, nullptr, 1, &one, 1, 0.0, 1.0, sigma, 1.0, &fid);
+    if (!rc) rc = pofd1d_prepare(c.get());
+    if (!rc) rc = pofd1d_set_rrange(c.get(), bid, minflux, maxflux);
+    if (rc) throw_status(c.get(), rc, "PDFactory", "getPD");
+    ctx = std::move(c);
+    built_sigma = sigma;
+    stale = false;
+  }
 
  public:
   explicit PDFactory(unsigned int NINTERP = 2048) : ninterp(NINTERP) {}
   unsigned int getNInterp() const { return ninterp; }
-  void setNInterp(unsigned int v) { ninterp = v; }
-  void setDevice(int d) { device = d; }
+  void setNInterp(unsigned int v) {
+    if (v != ninterp) stale = true;
+    ninterp = v;
+  }
+  void setDevice(int d) {
+    if (d != device) stale = true;
+    device = d;
+  }
   /* PDFactory::initPD, src/PDFactory.cc:1031-1141 (checks as written) */
-  bool initPD(unsigned int N, double MINFLUX, double MAXFLUX,
-              const numberCountsKnotsSpline& m, const beam& b) {
+  bool initPD(unsigned int N, double MINFLUX, double MAXFLUX, const numberCounts& m,
+              const beam& b) {
     if (N == 0) throw affineExcept("PDFactory", "initPD", "Invalid (zero) n");
     if (N == 1) throw affineExcept("PDFactory", "initPD", "Invalid (==0) n");
     if (MAXFLUX <= 0.0) throw affineExcept("PDFactory", "initPD", "Invalid (non-positive) maxflux");
     if (MINFLUX == MAXFLUX) throw affineExcept("PDFactory", "initPD", "Invalid (0) range between min/max flux");
     if (!m.isValid()) throw affineExcept("PDFactory", "initPD", "Invalid model");
     if (!b.hasData()) throw affineExcept("PDFactory", "initPD", "Empty beam");
+    const numberCountsKnotsSpline* ks = dynamic_cast<const numberCountsKnotsSpline*>(&m);
+    if (!ks)
+      throw affineExcept("PDFactory", "initPD",
+                         "the device path evaluates numberCountsKnotsSpline models only");
+    // the knot values travel with every batch; anything else rebuilds
+    stale = stale || N != n || MINFLUX != minflux || MAXFLUX != maxflux ||
+            ks->positions() != model.positions() || !bm.sameAs(b);
     n = N;
     minflux = MINFLUX;
     maxflux = MAXFLUX;
-    model = m;
+    model = *ks;
     bm = b;
     initialized = true;
     return true;
@@ -454,27 +541,18 @@ class PDFactory {
   void getPD(double sigma, PD& pd, bool setLog = true) {
     if (!initialized) throw affineExcept("PDFactory", "getPD", "Must call initPD first");
     using namespace pofd_amd_detail;
-    CtxPtr c = make_ctx(device, "PDFactory", "getPD");
-    const std::vector<double>& k = model.positions();
-    int rc = pofd1d_configure(c.get(), n, ninterp, k.data(), static_cast<uint32_t>(k.size()), 0.0);
-    int b = 0, f = 0;
-    if (!rc) rc = pofd1d_add_beamset(c.get(), bm.handle(), &b);
-    const uint32_t one = 1;
-    if (!rc) rc = pofd1d_add_field(c.get(), b, nullptr, 1, &one, 1, 0.0, 1.0, sigma, 1.0, &f);
-    if (!rc) rc = pofd1d_prepare(c.get());
-    if (!rc) rc = pofd1d_set_rrange(c.get(), b, minflux, maxflux);
-    if (rc) throw_status(c.get(), rc, "PDFactory", "getPD");
+    if (stale || !ctx || !(sigma == built_sigma)) build(sigma);
     std::vector<float> th(model.knotValues());
     th.push_back(1.0f);
     double ll = 0.0;
     int32_t st = 0;
-    rc = pofd1d_loglike_batch(c.get(), th.data(), 1, static_cast<uint32_t>(th.size()), &ll, &st);
-    if (rc) throw_status(c.get(), rc, "PDFactory", "getPD");
-    if (st) throw_status(c.get(), st, "PDFactory", "getPD");
+    int rc = pofd1d_loglike_batch(ctx.get(), th.data(), 1, static_cast<uint32_t>(th.size()), &ll, &st);
+    if (rc) throw_status(ctx.get(), rc, "PDFactory", "getPD");
+    if (st) throw_status(ctx.get(), st, "PDFactory", "getPD");
     pd.resize(n);
     double mf = 0.0, df = 0.0;
-    rc = pofd1d_get_pd(c.get(), 0, b, f, setLog ? 1 : 0, pd.pd_, &mf, &df);
-    if (rc) throw_status(c.get(), rc, "PDFactory", "getPD");
+    rc = pofd1d_get_pd(ctx.get(), 0, bid, fid, setLog ? 1 : 0, pd.pd_, &mf, &df);
+    if (rc) throw_status(ctx.get(), rc, "PDFactory", "getPD");
     pd.minflux = mf;
     pd.dflux = df;
     pd.logflat = setLog;
@@ -932,9 +1010,20 @@ class fitsDataDouble {
   std::pair<std::string, std::string> getFiles() const { return std::make_pair(file1, file2); }
 };
 
+/* include/numberCountsDouble.h:22-84: the abstract two-band counts model
+   (the part of its interface the device path needs) */
+class numberCountsDouble {
+ public:
+  virtual ~numberCountsDouble() {}
+  virtual bool isValid() const noexcept = 0;
+  virtual unsigned int getNParams() const noexcept = 0;
+  virtual void setParams(const paramSet& params) = 0;
+  virtual double differenceRegularize(double) const { return 0.0; }
+};
+
 /* include/numberCountsDoubleLogNormal.h: band-1 knot spline plus the
    log-normal colour model (sigma and offset knots) */
-class numberCountsDoubleLogNormal {
+class numberCountsDoubleLogNormal : public numberCountsDouble {
  private:
   std::vector<double> knots, sigmas, offsets;
   std::vector<float> values;  // knot values, sigma values, offset values
@@ -987,7 +1076,7 @@ class numberCountsDoubleLogNormal {
   unsigned int getNKnots() const noexcept { return static_cast<unsigned>(knots.size()); }
   unsigned int getNSigmas() const noexcept { return static_cast<unsigned>(sigmas.size()); }
   unsigned int getNOffsets() const noexcept { return static_cast<unsigned>(offsets.size()); }
-  unsigned int getNParams() const noexcept { return getNKnots() + getNSigmas() + getNOffsets(); }
+  unsigned int getNParams() const noexcept override { return getNKnots() + getNSigmas() + getNOffsets(); }
   double getKnotPosition(unsigned int i) const { return knots[i]; }
   double getSigmaPosition(unsigned int i) const { return sigmas[i]; }
   double getOffsetPosition(unsigned int i) const { return offsets[i]; }
@@ -996,7 +1085,7 @@ class numberCountsDoubleLogNormal {
   const std::vector<double>& offsetPositions() const { return offsets; }
   const std::vector<float>& paramValues() const { return values; }
   /* numberCountsDoubleLogNormal::setParams, :1010-1052 */
-  void setParams(const paramSet& F) {
+  void setParams(const paramSet& F) override {
     if (F.getNParams() < getNParams())
       throw affineExcept("numberCountsDoubleLogNormal", "setParams", "Not enough parameters present");
     values.assign(F.data(), F.data() + getNParams());
@@ -1004,7 +1093,7 @@ class numberCountsDoubleLogNormal {
   }
   /* isValid: knot values not NaN, sigma values finite, offsets not NaN
      (checkKnotsValid/checkSigmasValid/checkOffsetsValid, :1053-1124) */
-  bool isValid() const noexcept {
+  bool isValid() const noexcept override {
     if (!loaded || knots.empty() || sigmas.empty() || offsets.empty()) return false;
     const size_t nk = knots.size(), ns = sigmas.size();
     for (size_t i = 0; i < values.size(); ++i) {
@@ -1015,7 +1104,7 @@ class numberCountsDoubleLogNormal {
     return true;
   }
   /* numberCountsDoubleLogNormal::differenceRegularize, :2154-2173 */
-  double differenceRegularize(double alpha) const {
+  double differenceRegularize(double alpha) const override {
     if (!isValid()) return NAN;
     const size_t n = knots.size();
     if (n < 2) return 0.0;
@@ -1062,6 +1151,30 @@ class PDDouble {
   double getFluxVal1(unsigned int i) const { return minflux1 + static_cast<double>(i) * dflux1; }
   double getFluxVal2(unsigned int j) const { return minflux2 + static_cast<double>(j) * dflux2; }
   double getPDVal(unsigned int i, unsigned int j) const { return pd_[static_cast<size_t>(i) * n2 + j]; }
+  /* PDDouble::getLogLike, include/PDDouble.h:95, src/PDDouble.cc:976-1222
+     (on the device) */
+  double getLogLike(const fitsDataDouble& data) const {
+    if (pd_ == nullptr || n1 == 0 || n2 == 0)
+      throw affineExcept("PDDouble", "getLogLike", "pd not filled before likelihood calc");
+    if (data.getN() == 0) throw affineExcept("PDDouble", "getLogLike", "No data present");
+    pofd_ctx* c = pofd_amd_detail::thread_ctx("PDDouble", "getLogLike");
+    double out = 0.0;
+    int rc;
+    if (data.isBinned()) {
+      const double* bi = data.binInfo();
+      rc = pofd_pd2_loglike(c, pd_, n1, n2, minflux1, dflux1, minflux2, dflux2, logflat ? 1 : 0,
+                            nullptr, nullptr, 0,
+                            reinterpret_cast<const uint32_t*>(data.getBinnedData()),
+                            data.getNBins().first, data.getNBins().second, bi[0], bi[1], bi[2],
+                            bi[3], &out);
+    } else {
+      rc = pofd_pd2_loglike(c, pd_, n1, n2, minflux1, dflux1, minflux2, dflux2, logflat ? 1 : 0,
+                            data.getData1(), data.getData2(), data.getN(), nullptr, 0, 0, 0.0,
+                            0.0, 0.0, 0.0, &out);
+    }
+    if (rc) pofd_amd_detail::throw_status(c, rc, "PDDouble", "getLogLike");
+    return out;
+  }
 };
 
 namespace pofd_amd_detail {
@@ -1090,7 +1203,9 @@ inline std::vector<float> theta2(const numberCountsDoubleLogNormal& m, float sm1
 }  // namespace pofd_amd_detail
 
 /* include/PDFactoryDouble.h: initPD + getPD for one model and one pair of
-   noise levels; each getPD runs the device chain for a single walker */
+   noise levels.  The device context is built on the first getPD and kept
+   while the model's positions, the R ranges, the beam, the edge setting
+   and the two sigmas stay those it was built for. */
 class PDFactoryDouble {
  private:
   unsigned int nedge;
@@ -1101,37 +1216,14 @@ class PDFactoryDouble {
   const doublebeam* bm = nullptr;
   bool initialized = false;
   int device = pofd_amd_detail::default_device();
+  pofd_amd_detail::CtxPtr ctx;
+  bool stale = true;
+  double built_sigma[2] = {NAN, NAN};
+  int bid = 0, fid = 0;
 
- public:
-  explicit PDFactoryDouble(unsigned int NEDGE = 256) : nedge(NEDGE) {}
-  void setNEdge(unsigned int v) { nedge = v; }
-  unsigned int getNEdge() const { return nedge; }
-  void setDevice(int d) { device = d; }
-  /* PDFactoryDouble::initPD, src/PDFactoryDouble.cc:1143-1367 (checks as
-     written); the beam must outlive the factory */
-  bool initPD(unsigned int N, double minflux1, double maxflux1, double minflux2,
-              double maxflux2, const numberCountsDoubleLogNormal& m, const doublebeam& b,
-              bool setedge = true) {
-    if (N == 0) throw affineExcept("PDFactoryDouble", "initPD", "Invalid (non-positive) n");
-    if (maxflux1 <= 0.0 || maxflux2 <= 0.0)
-      throw affineExcept("PDFactoryDouble", "initPD", "Invalid (non-positive) maxflux");
-    if (!m.isValid()) throw affineExcept("PDFactoryDouble", "initPD", "Invalid model");
-    if (!b.hasData()) throw affineExcept("PDFactoryDouble", "initPD", "Empty beam");
-    n = N;
-    range[0] = minflux1;
-    range[1] = maxflux1;
-    range[2] = minflux2;
-    range[3] = maxflux2;
-    setEdge = setedge;
-    model = m;
-    bm = &b;
-    initialized = true;
-    return true;
-  }
-  /* PDFactoryDouble::getPD, src/PDFactoryDouble.cc:1382-1499 */
-  void getPD(double sigma1, double sigma2, PDDouble& pd, bool setLog = true) {
-    if (!initialized) throw affineExcept("PDFactoryDouble", "getPD", "Must call initPD first");
+  void build(double sigma1, double sigma2) {
     using namespace pofd_amd_detail;
+    ctx.reset();
     CtxPtr c = make_ctx(device, "PDFactoryDouble", "getPD");
     const auto& K = model.knotPositions();
     const auto& S = model.sigmaPositions();
@@ -1140,23 +1232,78 @@ class PDFactoryDouble {
                               static_cast<uint32_t>(K.size()), S.data(),
                               static_cast<uint32_t>(S.size()), O.data(),
                               static_cast<uint32_t>(O.size()), model.getMinSigma(), 0.0, 0.0);
-    int b = 0, f = 0;
-    if (!rc) rc = pofd2d_add_beamset(c.get(), bm->handle(), &b);
+    if (!rc) rc = pofd2d_add_beamset(c.get(), bm->handle(), &bid);
     const double z = 0.0;
-    if (!rc) rc = pofd2d_add_field(c.get(), b, &z, &z, 1, nullptr, 0, 0, nullptr, sigma1, sigma2, 1.0, &f);
+    if (!rc)
+      rc = pofd2d_add_field(c.get(), bid, &z, &z, 1, nullptr, 0, 0, nullptr, sigma1, sigma2, 1.0,
+                            &fid);
     std::vector<float> th = theta2(model, 1.0f, 1.0f);
     if (!rc) rc = pofd2d_prepare(c.get(), th.data(), static_cast<uint32_t>(th.size()));
-    if (!rc) rc = pofd2d_set_rrange(c.get(), b, range);
+    if (!rc) rc = pofd2d_set_rrange(c.get(), bid, range);
     if (rc) throw_status2(c.get(), rc, "PDFactoryDouble", "getPD");
+    ctx = std::move(c);
+    built_sigma[0] = sigma1;
+    built_sigma[1] = sigma2;
+    stale = false;
+  }
+
+ public:
+  explicit PDFactoryDouble(unsigned int NEDGE = 256) : nedge(NEDGE) {}
+  void setNEdge(unsigned int v) {
+    if (v != nedge) stale = true;
+    nedge = v;
+  }
+  unsigned int getNEdge() const { return nedge; }
+  void setDevice(int d) {
+    if (d != device) stale = true;
+    device = d;
+  }
+  /* PDFactoryDouble::initPD, src/PDFactoryDouble.cc:1143-1367 (checks as
+     written); the beam must outlive the factory */
+  bool initPD(unsigned int N, double minflux1, double maxflux1, double minflux2,
+              double maxflux2, const numberCountsDouble& m, const doublebeam& b,
+              bool setedge = true) {
+    if (N == 0) throw affineExcept("PDFactoryDouble", "initPD", "Invalid (non-positive) n");
+    if (maxflux1 <= 0.0 || maxflux2 <= 0.0)
+      throw affineExcept("PDFactoryDouble", "initPD", "Invalid (non-positive) maxflux");
+    if (!m.isValid()) throw affineExcept("PDFactoryDouble", "initPD", "Invalid model");
+    if (!b.hasData()) throw affineExcept("PDFactoryDouble", "initPD", "Empty beam");
+    const numberCountsDoubleLogNormal* ln = dynamic_cast<const numberCountsDoubleLogNormal*>(&m);
+    if (!ln)
+      throw affineExcept("PDFactoryDouble", "initPD",
+                         "the device path evaluates numberCountsDoubleLogNormal models only");
+    const double r[4] = {minflux1, maxflux1, minflux2, maxflux2};
+    // the model's values travel with every batch (the colour model set-up
+    // on the device is rebuilt from them); anything else rebuilds
+    stale = stale || N != n || setedge != setEdge || &b != bm ||
+            std::memcmp(r, range, sizeof r) != 0 || ln->knotPositions() != model.knotPositions() ||
+            ln->sigmaPositions() != model.sigmaPositions() ||
+            ln->offsetPositions() != model.offsetPositions() ||
+            ln->getMinSigma() != model.getMinSigma();
+    n = N;
+    std::memcpy(range, r, sizeof r);
+    setEdge = setedge;
+    model = *ln;
+    bm = &b;
+    initialized = true;
+    return true;
+  }
+  /* PDFactoryDouble::getPD, src/PDFactoryDouble.cc:1382-1499 */
+  void getPD(double sigma1, double sigma2, PDDouble& pd, bool setLog = true) {
+    if (!initialized) throw affineExcept("PDFactoryDouble", "getPD", "Must call initPD first");
+    using namespace pofd_amd_detail;
+    if (stale || !ctx || !(sigma1 == built_sigma[0]) || !(sigma2 == built_sigma[1]))
+      build(sigma1, sigma2);
+    std::vector<float> th = theta2(model, 1.0f, 1.0f);
     double ll = 0.0;
     int32_t st = 0;
-    rc = pofd2d_loglike_batch(c.get(), th.data(), 1, static_cast<uint32_t>(th.size()), &ll, &st);
-    if (rc) throw_status2(c.get(), rc, "PDFactoryDouble", "getPD");
-    if (st) throw_status2(c.get(), st, "PDFactoryDouble", "getPD");
+    int rc = pofd2d_loglike_batch(ctx.get(), th.data(), 1, static_cast<uint32_t>(th.size()), &ll, &st);
+    if (rc) throw_status2(ctx.get(), rc, "PDFactoryDouble", "getPD");
+    if (st) throw_status2(ctx.get(), st, "PDFactoryDouble", "getPD");
     pd.resize(n, n);
     double info[8];
-    rc = pofd2d_get_pd(c.get(), 0, b, f, setLog ? 1 : 0, pd.pd_, info);
-    if (rc) throw_status2(c.get(), rc, "PDFactoryDouble", "getPD");
+    rc = pofd2d_get_pd(ctx.get(), 0, bid, fid, setLog ? 1 : 0, pd.pd_, info);
+    if (rc) throw_status2(ctx.get(), rc, "PDFactoryDouble", "getPD");
     pd.minflux1 = info[0];
     pd.minflux2 = info[1];
     pd.dflux1 = info[2];

@@ -1013,13 +1013,19 @@ void orc_pd2_means(unsigned n1, unsigned n2, const double *pd, double minflux1,
   out[1] = mean2 + minflux2;
 }
 
-/* getLogLikeUnbinned, log2-stored branch (:994-1052); the pd must be log2 */
+/* a P(D) value in log2: as stored (logflat) or log2 of it */
+static double orc2_lv(const double *pd, size_t i, int islog) {
+  return islog ? pd[i] : log2(pd[i]);
+}
+
+/* PDDouble::getLogLikeUnbinned (src/PDDouble.cc:994-1082): the log2-stored
+   branch tests the edges with <= / >=, the linear-stored one with < / >
+   (:1047-1066) and takes log2 of each value it reads */
 double orc_pd2_loglike_unbinned(unsigned n1, unsigned n2, const double *pd,
                                 double minflux1, double dflux1,
                                 double minflux2, double dflux2, int islog,
                                 const double *d1, const double *d2,
                                 size_t ndata) {
-  (void)islog;
   double maxflux1 = minflux1 + (double)(n1 - 1) * dflux1;
   double maxflux2 = minflux2 + (double)(n2 - 1) * dflux2;
   unsigned n2n1 = n2 * n1;
@@ -1030,38 +1036,43 @@ double orc_pd2_loglike_unbinned(unsigned n1, unsigned n2, const double *pd,
     double delt2 = (cflux2 - minflux2) * idflux2;
     int idx1 = (int)delt1, idx2 = (int)delt2;
     int n2idx1 = n2 * idx1;
-    if (cflux1 <= minflux1) {
-      if (cflux2 <= minflux2) iv = pd[0];
-      else if (cflux2 >= maxflux2) iv = pd[n2 - 1];
-      else iv = pd[idx2];
-    } else if (cflux1 >= maxflux1) {
-      if (cflux2 <= minflux2) iv = pd[n2n1 - n1];
-      else if (cflux2 >= maxflux2) iv = pd[n2n1 - 1];
-      else iv = pd[n2n1 - n1 + idx2];
-    } else if (cflux2 <= minflux2) {
-      iv = pd[n2idx1];
-    } else if (cflux2 >= maxflux2) {
-      iv = pd[n2idx1 + n2 - 1];
+    int lo1 = islog ? (cflux1 <= minflux1) : (cflux1 < minflux1);
+    int hi1 = islog ? (cflux1 >= maxflux1) : (cflux1 > maxflux1);
+    int lo2 = islog ? (cflux2 <= minflux2) : (cflux2 < minflux2);
+    int hi2 = islog ? (cflux2 >= maxflux2) : (cflux2 > maxflux2);
+    if (lo1) {
+      if (lo2) iv = orc2_lv(pd, 0, islog);
+      else if (hi2) iv = orc2_lv(pd, n2 - 1, islog);
+      else iv = orc2_lv(pd, idx2, islog);
+    } else if (hi1) {
+      if (lo2) iv = orc2_lv(pd, n2n1 - n1, islog);
+      else if (hi2) iv = orc2_lv(pd, n2n1 - 1, islog);
+      else iv = orc2_lv(pd, n2n1 - n1 + idx2, islog);
+    } else if (lo2) {
+      iv = orc2_lv(pd, n2idx1, islog);
+    } else if (hi2) {
+      iv = orc2_lv(pd, n2idx1 + n2 - 1, islog);
     } else {
       double t = delt1 - (double)idx1, u = delt2 - (double)idx2;
       double omu = 1.0 - u, omt = 1.0 - t;
       unsigned base = n2idx1 + idx2;
-      iv = omt * (omu * pd[base] + u * pd[base + 1]) +
-           t * (omu * pd[base + n2] + u * pd[base + n2 + 1]);
+      iv = omt * (omu * orc2_lv(pd, base, islog) + u * orc2_lv(pd, base + 1, islog)) +
+           t * (omu * orc2_lv(pd, base + n2, islog) + u * orc2_lv(pd, base + n2 + 1, islog));
     }
     loglike += iv;
   }
   return log(2.) * loglike;
 }
 
-/* getLogLikeBinned, log2-stored branch (:1103-1170) */
+/* PDDouble::getLogLikeBinned (src/PDDouble.cc:1092-1220): both branches
+   test the edges with < / >; the linear-stored one takes log2 of each
+   value it reads */
 double orc_pd2_loglike_binned(unsigned n1, unsigned n2, const double *pd,
                               double minflux1, double dflux1, double minflux2,
                               double dflux2, int islog, const uint32_t *bins,
                               uint32_t nb1, uint32_t nb2, double bincent01,
                               double bindelta1, double bincent02,
                               double bindelta2) {
-  (void)islog;
   double maxflux1 = minflux1 + (double)(n1 - 1) * dflux1;
   double maxflux2 = minflux2 + (double)(n2 - 1) * dflux2;
   unsigned n2n1 = n2 * n1;
@@ -1077,24 +1088,24 @@ double orc_pd2_loglike_binned(unsigned n1, unsigned n2, const double *pd,
       double cflux2 = bincent02 + (double)j * bindelta2, iv;
       int idx2 = (int)((cflux2 - minflux2) * idflux2);
       if (cflux1 < minflux1) {
-        if (cflux2 < minflux2) iv = pd[0];
-        else if (cflux2 > maxflux2) iv = pd[n2 - 1];
-        else iv = pd[idx2];
+        if (cflux2 < minflux2) iv = orc2_lv(pd, 0, islog);
+        else if (cflux2 > maxflux2) iv = orc2_lv(pd, n2 - 1, islog);
+        else iv = orc2_lv(pd, idx2, islog);
       } else if (cflux1 > maxflux1) {
-        if (cflux2 < minflux2) iv = pd[n2n1 - n1];
-        else if (cflux2 > maxflux2) iv = pd[n2n1 - 1];
-        else iv = pd[n2n1 - n1 + idx2];
+        if (cflux2 < minflux2) iv = orc2_lv(pd, n2n1 - n1, islog);
+        else if (cflux2 > maxflux2) iv = orc2_lv(pd, n2n1 - 1, islog);
+        else iv = orc2_lv(pd, n2n1 - n1 + idx2, islog);
       } else if (cflux2 < minflux2) {
-        iv = pd[n2idx1];
+        iv = orc2_lv(pd, n2idx1, islog);
       } else if (cflux2 > maxflux2) {
-        iv = pd[n2idx1 + n2 - 1];
+        iv = orc2_lv(pd, n2idx1 + n2 - 1, islog);
       } else {
         double t = (cflux1 - minflux1) * idflux1 - (double)idx1;
         double u = (cflux2 - minflux2) * idflux2 - (double)idx2;
         double omu = 1.0 - u, omt = 1.0 - t;
         unsigned base = n2idx1 + idx2;
-        iv = omt * (omu * pd[base] + u * pd[base + 1]) +
-             t * (omu * pd[base + n2] + u * pd[base + n2 + 1]);
+        iv = omt * (omu * orc2_lv(pd, base, islog) + u * orc2_lv(pd, base + 1, islog)) +
+             t * (omu * orc2_lv(pd, base + n2, islog) + u * orc2_lv(pd, base + n2 + 1, islog));
       }
       loglike += (double)ninbin * iv;
     }

@@ -66,6 +66,11 @@ _SIGS = {
     "pofd1d_loglike_batch_device_count": (C.c_int, [VP, VP, C.c_uint32, VP,
                                                     C.c_uint32, VP, VP, VP]),
     "pofd1d_set_rrange": (C.c_int, [VP, C.c_int, D, D]),
+    "pofd_pd_loglike": (C.c_int, [VP, DP, C.c_uint32, D, D, C.c_int, DP,
+                                  C.c_uint64, U32P, C.c_uint32, D, D, DP]),
+    "pofd_pd2_loglike": (C.c_int, [VP, DP, C.c_uint32, C.c_uint32, D, D, D,
+                                   D, C.c_int, DP, DP, C.c_uint64, U32P,
+                                   C.c_uint32, C.c_uint32, D, D, D, D, DP]),
     "pofd1d_beamset_info": (C.c_int, [VP, C.c_int, DP]),
     "pofd1d_loglike_batch": (C.c_int, [VP, FP, C.c_uint32, C.c_uint32, DP,
                                        I32P]),
@@ -400,6 +405,50 @@ class Context:
             self.ptr, VP(d_theta), max_walkers, VP(d_count), stride,
             VP(d_ll), VP(d_st), VP(stream)),
             "pofd1d_loglike_batch_device_count")
+
+    def pd_loglike(self, pd, minflux, dflux, is_log=True, pix=None,
+                   bins=None, bincent0=0.0, bindelta=0.0):
+        """PD::getLogLike of pixels (or a histogram) given a host P(D)."""
+        a = _f64(pd)
+        out = C.c_double(0.0)
+        if bins is not None:
+            bb = np.ascontiguousarray(bins, dtype=np.uint32)
+            rc = lib().pofd_pd_loglike(self.ptr, _dp(a), a.size, minflux,
+                                       dflux, int(is_log), None, 0,
+                                       bb.ctypes.data_as(U32P), bb.size,
+                                       bincent0, bindelta, C.byref(out))
+        else:
+            p = _f64(pix)
+            rc = lib().pofd_pd_loglike(self.ptr, _dp(a), a.size, minflux,
+                                       dflux, int(is_log), _dp(p), p.size,
+                                       None, 0, 0.0, 0.0, C.byref(out))
+        self._check(rc, "pofd_pd_loglike")
+        return out.value
+
+    def pd2_loglike(self, pd, minflux1, dflux1, minflux2, dflux2,
+                    is_log=True, d1=None, d2=None, bins=None, bininfo=None):
+        """PDDouble::getLogLike of pixel pairs (or a 2D histogram, bininfo =
+        (bincent01, bindelta1, bincent02, bindelta2)) given a host P(D)."""
+        a = _f64(pd)
+        n1, n2 = a.shape
+        out = C.c_double(0.0)
+        if bins is not None:
+            bb = np.ascontiguousarray(bins, dtype=np.uint32)
+            nb1, nb2 = bb.shape
+            info = [float(v) for v in bininfo]
+            rc = lib().pofd_pd2_loglike(self.ptr, _dp(a), n1, n2, minflux1,
+                                        dflux1, minflux2, dflux2, int(is_log),
+                                        None, None, 0,
+                                        bb.ctypes.data_as(U32P), nb1, nb2,
+                                        *info, C.byref(out))
+        else:
+            p1, p2 = _f64(d1), _f64(d2)
+            rc = lib().pofd_pd2_loglike(self.ptr, _dp(a), n1, n2, minflux1,
+                                        dflux1, minflux2, dflux2, int(is_log),
+                                        _dp(p1), _dp(p2), p1.size, None, 0, 0,
+                                        0.0, 0.0, 0.0, 0.0, C.byref(out))
+        self._check(rc, "pofd_pd2_loglike")
+        return out.value
 
     def get_pd(self, walker, b=0, f=0, is_log=True):
         """is_log True/False: PD::pd_ after getPD; "raw": the clamped,

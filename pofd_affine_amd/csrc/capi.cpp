@@ -89,6 +89,8 @@ struct pofd_ctx {
   pofd::PriorDev priors{};  // added on the device after the beam sets
   std::unique_ptr<DevBuf> dtheta, dll, dst;
   size_t cap_theta = 0, cap_out = 0;
+  // stand-alone PD / PDDouble likelihoods: the P(D), the data and partials
+  std::unique_ptr<DevBuf> pdl_pd, pdl_d1, pdl_d2, pdl_bins, pdl_part;
   // stage timing: one set of POFD_NSTAGES+1 events per beam-set launch
   bool stage_timing = false;
   std::vector<std::array<hipEvent_t, POFD_NSTAGES + 1>> stage_events;
@@ -1208,4 +1210,93 @@ extern "C" int pofd1d_stage_times(pofd_ctx* c, double* ms_out,
   if (nlaunch) *nlaunch = static_cast<uint32_t>(c->stage_used);
   c->stage_used = 0;
   return POFD_OK;
+}
+
+// ---- PD::getLogLike / PDDouble::getLogLike over a caller's P(D)
+namespace {
+int upload_bytes(pofd_ctx* c, std::unique_ptr<DevBuf>& b, const void* src, size_t bytes) {
+  int rc = dev_alloc(c, b, bytes);
+  if (rc) return rc;
+  hipError_t e = hipMemcpy(b->p, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? POFD_OK : hip_fail(c, e, "hipMemcpy(pd loglike input)");
+}
+
+// the P(D) with one padding row of 1.0 (log2 1 = 0): an interpolation at
+// exactly the top edge reads one cell past the array with weight 0, as the
+// reference's does (src/PDDouble.cc:1066, :1206)
+int upload_pd(pofd_ctx* c, const double* pd, size_t n, size_t pad) {
+  int rc = dev_alloc(c, c->pdl_pd, (n + pad) * sizeof(double));
+  if (rc) return rc;
+  std::vector<double> tail(pad, 1.0);
+  hipError_t e = hipMemcpy(c->pdl_pd->p, pd, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && pad)
+    e = hipMemcpy(static_cast<double*>(c->pdl_pd->p) + n, tail.data(), pad * sizeof(double),
+                  hipMemcpyHostToDevice);
+  return e == hipSuccess ? POFD_OK : hip_fail(c, e, "hipMemcpy(pd)");
+}
+
+int finish_pd_loglike(pofd_ctx* c, int rc, double* out) {
+  if (rc) return fail(c, rc, "PD::getLogLike: launch failed");
+  hipError_t e = hipMemcpy(out, static_cast<double*>(c->pdl_part->p) + pofd::pd_like_scratch_doubles() - 1,
+                           sizeof(double), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? POFD_OK : hip_fail(c, e, "hipMemcpy(pd loglike)");
+}
+}  // namespace
+
+extern "C" int pofd_pd_loglike(pofd_ctx* c, const double* pd, uint32_t n, double minflux,
+                               double dflux, int is_log, const double* pix, uint64_t npix,
+                               const uint32_t* bins, uint32_t nbins, double bincent0,
+                               double bindelta, double* out) {
+  if (!c || !out) return POFD_E_ARG;
+  if (!pd || n < 2) return fail(c, POFD_E_ARG, "PD::getLogLike: pd not filled before likelihood calc");
+  if ((bins && nbins == 0) || (!bins && (!pix || npix == 0)))
+    return fail(c, POFD_E_ARG, "PD::getLogLike: No data present");
+  if (!(dflux > 0.0)) return fail(c, POFD_E_ARG, "PD::getLogLike: dflux must be > 0");
+  if (hipSetDevice(c->device) != hipSuccess)
+    return fail(c, POFD_E_INTERNAL, "PD::getLogLike: hipSetDevice failed");
+  int rc = upload_pd(c, pd, n, 2);
+  if (!rc) rc = dev_alloc(c, c->pdl_part, pofd::pd_like_scratch_doubles() * sizeof(double));
+  if (!rc && bins) rc = upload_bytes(c, c->pdl_bins, bins, nbins * sizeof(uint32_t));
+  if (!rc && !bins) rc = upload_bytes(c, c->pdl_d1, pix, npix * sizeof(double));
+  if (rc) return rc;
+  rc = pofd::launch_pd_loglike1d(
+      static_cast<const double*>(c->pdl_pd->p), static_cast<int>(n), minflux, dflux, is_log ? 1 : 0,
+      bins ? nullptr : static_cast<const double*>(c->pdl_d1->p), bins ? 0 : npix,
+      bins ? static_cast<const uint32_t*>(c->pdl_bins->p) : nullptr, bins ? nbins : 0, bincent0,
+      bindelta, static_cast<double*>(c->pdl_part->p), nullptr);
+  return finish_pd_loglike(c, rc, out);
+}
+
+extern "C" int pofd_pd2_loglike(pofd_ctx* c, const double* pd, uint32_t n1, uint32_t n2,
+                                double minflux1, double dflux1, double minflux2, double dflux2,
+                                int is_log, const double* pix1, const double* pix2,
+                                uint64_t npix, const uint32_t* bins, uint32_t nbins1,
+                                uint32_t nbins2, double bincent01, double bindelta1,
+                                double bincent02, double bindelta2, double* out) {
+  if (!c || !out) return POFD_E_ARG;
+  if (!pd || n1 < 2 || n2 < 2)
+    return fail(c, POFD_E_ARG, "PDDouble::getLogLike: pd not filled before likelihood calc");
+  if ((bins && (nbins1 == 0 || nbins2 == 0)) || (!bins && (!pix1 || !pix2 || npix == 0)))
+    return fail(c, POFD_E_ARG, "PDDouble::getLogLike: No data present");
+  if (!(dflux1 > 0.0) || !(dflux2 > 0.0))
+    return fail(c, POFD_E_ARG, "PDDouble::getLogLike: dflux must be > 0");
+  if (hipSetDevice(c->device) != hipSuccess)
+    return fail(c, POFD_E_INTERNAL, "PDDouble::getLogLike: hipSetDevice failed");
+  const size_t nn = static_cast<size_t>(n1) * n2;
+  int rc = upload_pd(c, pd, nn, static_cast<size_t>(n2) + 2);
+  if (!rc) rc = dev_alloc(c, c->pdl_part, pofd::pd_like_scratch_doubles() * sizeof(double));
+  if (!rc && bins)
+    rc = upload_bytes(c, c->pdl_bins, bins, static_cast<size_t>(nbins1) * nbins2 * sizeof(uint32_t));
+  if (!rc && !bins) rc = upload_bytes(c, c->pdl_d1, pix1, npix * sizeof(double));
+  if (!rc && !bins) rc = upload_bytes(c, c->pdl_d2, pix2, npix * sizeof(double));
+  if (rc) return rc;
+  rc = pofd::launch_pd_loglike2d(
+      static_cast<const double*>(c->pdl_pd->p), static_cast<int>(n1), static_cast<int>(n2), minflux1,
+      dflux1, minflux2, dflux2, is_log ? 1 : 0,
+      bins ? nullptr : static_cast<const double*>(c->pdl_d1->p),
+      bins ? nullptr : static_cast<const double*>(c->pdl_d2->p), bins ? 0 : npix,
+      bins ? static_cast<const uint32_t*>(c->pdl_bins->p) : nullptr, bins ? nbins1 : 0,
+      bins ? nbins2 : 0, bincent01, bindelta1, bincent02, bindelta2,
+      static_cast<double*>(c->pdl_part->p), nullptr);
+  return finish_pd_loglike(c, rc, out);
 }

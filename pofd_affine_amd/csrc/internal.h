@@ -178,6 +178,19 @@ struct PriorDev {
 int launch_prior1d(const BeamsetDev& bs0, const WorkDev& wk0, const PriorDev& pr,
                    const float* d_theta, uint32_t stride, uint32_t nwalk, const int32_t* d_count,
                    double* d_loglike, const int32_t* d_status, void* stream);
+// PD::getLogLike / PDDouble::getLogLike over a P(D) in device memory
+// (pdlike_kernel.hip); d_scratch holds pd_like_scratch_doubles() doubles,
+// the result lands in its last one
+size_t pd_like_scratch_doubles();
+int launch_pd_loglike1d(const double* d_pd, int n, double minflux, double dflux, int islog,
+                        const double* d_pix, uint64_t npix, const uint32_t* d_bins,
+                        uint32_t nbins, double bincent0, double bindelta, double* d_scratch,
+                        void* stream);
+int launch_pd_loglike2d(const double* d_pd, int n1, int n2, double minflux1, double dflux1,
+                        double minflux2, double dflux2, int islog, const double* d_pix1,
+                        const double* d_pix2, uint64_t npix, const uint32_t* d_bins,
+                        uint32_t nb1, uint32_t nb2, double bincent01, double bindelta1,
+                        double bincent02, double bindelta2, double* d_scratch, void* stream);
 int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
                           int is_log, int already_log, double* d_out,
                           void* stream);

@@ -3,11 +3,13 @@
 // and beam files, set knots and R ranges, evaluate log likelihoods.
 //
 // usage: getlike_check fftsize ninterp binned nbins minbeamval nbeamhist
-//                      sigma datafile beamfile knotfile thetafile
+//                      sigma datafile beamfile knotfile thetafile [pdfile]
 // knotfile: knot positions (Jy), one per line; thetafile: one parameter
 // set per line.  Prints "loglike <value>" per parameter set, then one line
 // "pd <minflux> <dflux> <n> <peak log2>" from PDFactory::getPD for the
-// first set.
+// first set, "pdlike <value>" from PD::getLogLike of the data given that
+// P(D) (written as raw doubles to pdfile when given), and "cache <same>"
+// comparing a factory reused for the second set with a fresh one.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -32,7 +34,7 @@ static std::vector<std::vector<double>> read_rows(const char* path) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 12) {
+  if (argc != 12 && argc != 13) {
     std::cerr << "bad usage" << std::endl;
     return 2;
   }
@@ -89,6 +91,31 @@ int main(int argc, char** argv) {
       if (pd.pd_[i] > peak) peak = pd.pd_[i];
     std::printf("pd %.17g %.17g %u %.17g\n", pd.minflux, pd.dflux,
                 pd.getDim(), peak);
+    // PD::getLogLike of the data given that P(D) (binned as the likelihood)
+    fitsData data(argv[8], false, false);
+    if (binned) data.applyBinning(nbins);
+    std::printf("pdlike %.17g\n", pd.getLogLike(data));
+    if (argc == 13) {
+      std::FILE* f = std::fopen(argv[12], "wb");
+      if (!f || std::fwrite(pd.pd_, sizeof(double), pd.getDim(), f) != pd.getDim()) return 5;
+      std::fclose(f);
+    }
+    // the factory's context is kept across initPD calls that change only
+    // the knot values: a second set through it equals a fresh factory's
+    if (thetas.size() > 1) {
+      numberCountsKnotsSpline m2(knots);
+      m2.setParams(paramSet(thetas[1]));
+      pfac.initPD(fftsize, 1.05 * rr.first, 1.05 * rr.second + 7.5 * sigma, m2, bm);
+      PD a, b;
+      pfac.getPD(sigma, a, true);
+      PDFactory fresh(ninterp);
+      fresh.initPD(fftsize, 1.05 * rr.first, 1.05 * rr.second + 7.5 * sigma, m2, bm);
+      fresh.getPD(sigma, b, true);
+      bool same = a.getDim() == b.getDim() && a.minflux == b.minflux && a.dflux == b.dflux &&
+                  std::memcmp(a.pd_, b.pd_, a.getDim() * sizeof(double)) == 0;
+      bool differs = std::memcmp(a.pd_, pd.pd_, a.getDim() * sizeof(double)) != 0;
+      std::printf("cache %d %d\n", same ? 1 : 0, differs ? 1 : 0);
+    }
     // an error path: a NaN knot value must raise affineExcept
     std::vector<double> bad(thetas[0]);
     bad[1] = std::nan("");

@@ -35,7 +35,7 @@ def test_cpp_getlike(tmp_path):
             [EXE, "32768", "512", str(binned), "1000", "1e-6", "120",
              "0.0025", str(tmp_path / "map.fits"),
              str(tmp_path / "beam.fits"), str(tmp_path / "knots.txt"),
-             str(tmp_path / "theta.txt")],
+             str(tmp_path / "theta.txt"), str(tmp_path / "pd.bin")],
             capture_output=True, text=True, timeout=120)
         assert out.returncode == 0, out.stderr
         lines = out.stdout.split("\n")
@@ -60,8 +60,22 @@ def test_cpp_getlike(tmp_path):
         assert float(pl[3]) == pytest.approx(s2, rel=1e-6)
         err = [l for l in lines if l.startswith("error")][0]
         assert err == "error PDFactory::initPD"
-        pd = [l for l in lines if l.startswith("pd")][0].split()
+        pd = [l for l in lines if l.startswith("pd ")][0].split()
         assert int(pd[3]) == 32768 and float(pd[2]) > 0
+        # PD::getLogLike on the device against the oracle's restatement of
+        # src/PD.cc:538-677 on the same P(D) array and data
+        pdv = np.fromfile(tmp_path / "pd.bin", dtype=np.float64)
+        mf, df = float(pd[1]), float(pd[2])
+        if binned:
+            bins, c0, dl = O.data_bin(inp["data"], 1000)
+            want = O.pd_loglike_binned(pdv, mf, df, bins, c0, dl, islog=True)
+        else:
+            want, _ = O.pd_loglike_unbinned(pdv, mf, df, inp["data"],
+                                            islog=True)
+        pl = float([l for l in lines if l.startswith("pdlike")][0].split()[1])
+        assert pl == pytest.approx(want, rel=0, abs=1e-6)
+        # a factory reused with new knot values equals a fresh one
+        assert [l for l in lines if l.startswith("cache")][0] == "cache 1 1"
 
 
 EXE2 = os.path.join(ROOT, "tests", "cpp", "_build", "getlike2d_check")
