@@ -247,11 +247,12 @@ def main():
         out["multifield"] = bench_multifield(args, hiprt)
     if not args.no_cpu and args.cpu_evals > 0:
         out["cpu_baseline"], par = cpu_baseline_1d(args, inp, th, True,
-                                                   b["ll"], b["st"])
+                                                   b["ll"], b["st"], b["ctx"])
         out.update(par)
         if "unbinned" in out:
             ub, upar = cpu_baseline_1d(args, inp, th, False, u["ll"],
-                                       u["st"], evals=args.cpu_evals // 2)
+                                       u["st"], u["ctx"],
+                                       evals=args.cpu_evals // 2)
             out["unbinned"]["cpu_baseline"] = ub
             out["unbinned"].update(upar)
     print(json.dumps(out))
@@ -276,13 +277,16 @@ def host_cpu():
     return min(16, n), model
 
 
-def cpu_baseline_1d(args, inp, th, binned, gpu_ll, gpu_st, evals=None):
+def cpu_baseline_1d(args, inp, th, binned, gpu_ll, gpu_st, ctx, evals=None):
     """The oracle (a plain-double restatement of the reference CPU path) on
     the host: one evaluation per worker thread at a time, as the
     reference's MPI slaves do (the master computes nothing,
     src/pofdMCMC.cc:200), with nproc - 1 and nproc workers, the per-stage
     split in the reference's TIMING categories, and the GPU's distance from
-    the double oracle and from the extended-precision (exact-math) one."""
+    the double oracle and from the extended-precision (exact-math) one --
+    also at the device's own split index for the walkers whose split
+    search landed on the other of two values that agree to within double
+    rounding (the argmin of src/PDFactory.cc:783-795)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_side import build_oracle
@@ -326,14 +330,37 @@ def cpu_baseline_1d(args, inp, th, binned, gpu_ll, gpu_st, evals=None):
     ok = (dst == 0) & (est == 0) & (gpu_st == 0)
     par = {}
     if ok.any():
+        at_dev = ext.copy()
+        amb = []
+        for w in np.nonzero(ok & (np.abs(ext - gpu_ll) > 1e-6))[0]:
+            w = int(w)
+            gsplit = int(ctx.get_debug(w)["split"])
+            L.initpd(0, th[w])
+            esplit = L.getpd(0, 0, float(th[w][args_sigmult_index(L)]),
+                             setlog=False)["split"]
+            if gsplit != esplit:
+                L.set_split_shift(gsplit - esplit)
+                v, st1 = L.loglike_batch(th[w:w + 1], 1)
+                L.set_split_shift(0)
+                if st1[0] == 0:
+                    at_dev[w] = v[0]
+                amb.append([w, gsplit, int(esplit)])
         par = {
             "parity_max_abs_dlogL": float(np.max(np.abs(dbl[ok] - gpu_ll[ok]))),
             "parity_max_abs_dlogL_vs_exact":
                 float(np.max(np.abs(ext[ok] - gpu_ll[ok]))),
+            "parity_max_abs_dlogL_vs_exact_at_device_split":
+                float(np.max(np.abs(at_dev[ok] - gpu_ll[ok]))),
+            "split_ambiguous_walkers": amb,
             "oracle_double_vs_exact": float(np.max(np.abs(ext[ok] - dbl[ok]))),
             "parity_walkers": int(ok.sum()),
         }
     return cb, par
+
+
+def args_sigmult_index(L):
+    """Index of the sigma multiplier in theta: after the knots."""
+    return L.nknots
 
 
 def bench_multifield(args, hiprt):

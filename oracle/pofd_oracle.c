@@ -1755,12 +1755,33 @@ static int orc_unwrap_normalize(orc_pdfac *p, orc_err *err) {
   p->pd_dflux = p->dflux;
   orc_tick(p, &t0, ORC_T_COPY);
   /* PD::normalize src/PD.cc:146-161 */
-  double tot = orc_pd_integral(n, p->pd, p->pd_dflux, 0);
-  p->last_integral = tot;
-  double itot = 1.0 / tot;
-  for (unsigned i = 0; i < n; ++i) p->pd[i] *= itot;
-  orc_tick(p, &t0, ORC_T_NORM);
-  double tmn = orc_pd_mean(n, p->pd, 0.0, p->pd_dflux, 0, 0);
+  double tot, tmn;
+  if (p->precise) {
+    /* the exact-math value: the trapezoid sums and the mean accumulated in
+       extended precision, so no tail term is lost against the running sum
+       (the reference's sequential double sums drop terms below half an ulp
+       of the total) */
+    const double *q = p->pd;
+    long double lt = 0.5L * (long double)q[0];
+    for (unsigned i = 1; i < n - 1; ++i) lt += (long double)q[i];
+    lt += 0.5L * (long double)q[n - 1];
+    lt *= (long double)p->pd_dflux;
+    tot = (double)lt;
+    p->last_integral = tot;
+    long double ilt = 1.0L / lt, lm = 0.0L;
+    for (unsigned i = 1; i < n - 1; ++i) lm += (long double)i * ((long double)q[i] * ilt);
+    lm += 0.5L * (long double)(n - 1) * ((long double)q[n - 1] * ilt);
+    lm *= (long double)p->pd_dflux * (long double)p->pd_dflux;
+    for (unsigned i = 0; i < n; ++i) p->pd[i] = (double)((long double)q[i] * ilt);
+    tmn = (double)lm;
+  } else {
+    tot = orc_pd_integral(n, p->pd, p->pd_dflux, 0);
+    p->last_integral = tot;
+    double itot = 1.0 / tot;
+    for (unsigned i = 0; i < n; ++i) p->pd[i] *= itot;
+    orc_tick(p, &t0, ORC_T_NORM);
+    tmn = orc_pd_mean(n, p->pd, 0.0, p->pd_dflux, 0, 0);
+  }
   orc_tick(p, &t0, ORC_T_MEAN);
   if (isinf(tmn) || isnan(tmn))
     return orc_fail(err, ORC_E_MEAN, "PDFactory", "unwrapandNormalizePD",
@@ -2082,7 +2103,9 @@ static int orc_loglike_ws(const orc_like *L, orc_knots *model, orc_pdfac *p,
       rc = orc_getpd(p, sigmult * s->fields[f].sigma_base, 1, err);
       if (rc) return rc;
       double t0 = orc_tstart(p);
-      double curr = orc_field_loglike_pd(p, &s->fields[f], NULL);
+      double comp = 0.0;
+      double curr = orc_field_loglike_pd(p, &s->fields[f], &comp);
+      if (p->precise) curr = comp;  /* compensated pixel sum */
       orc_tick(p, &t0, ORC_T_LIKE);
       LogLike += (curr - s->fields[f].like_offset) / s->fields[f].like_norm;
     }

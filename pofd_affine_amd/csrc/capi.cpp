@@ -478,6 +478,24 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   if ((rc = upload(c, s, w256, &d.W256))) return rc;
   if ((rc = upload(c, s, wlo, &d.Wlo))) return rc;
   if ((rc = upload(c, s, wm2, &d.Wm2))) return rc;
+  // W_N^t - 1 for t in [0, N/2] (the rest by conjugate symmetry), formed
+  // without cancellation as (-2 sin^2(pi t/N), -sin(2 pi t/N)) in long
+  // double: the direct sums over the head of the grid use it
+  {
+    std::vector<double2> em1(N / 2 + 1);
+    for (size_t t = 0; t <= N / 2; ++t) {
+      const long double a = static_cast<long double>(kPi) * static_cast<long double>(t) /
+                            static_cast<long double>(N);
+      const long double sa = sinl(a);
+      em1[t] = make_double2(static_cast<double>(-2.0L * sa * sa),
+                            static_cast<double>(-sinl(2.0L * a)));
+    }
+    if ((rc = upload(c, s, em1, &d.em1))) return rc;
+  }
+  // the head of the grid: the first half transform row (and the last half
+  // row, which holds the wrapped fluxes just below zero, when there are any)
+  d.headn = static_cast<int>(M2);
+  d.headneg = (minitn <= maxitn && maxitn >= static_cast<long>(N) - d.headn) ? 1 : 0;
   d.nfields = static_cast<int>(s.fields.size());
   for (size_t f = 0; f < s.fields.size(); ++f) {
     FieldHost& fh = s.fields[f];

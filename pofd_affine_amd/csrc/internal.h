@@ -97,6 +97,12 @@ struct BeamsetDev {
   const double2* W256;    // 256: exp(-2 pi i s/256)
   const double2* Wlo;     // 256: exp(-2 pi i s/M)
   const double2* Wm2;     // M2: exp(-2 pi i s/M2)
+  const double2* em1;     // N/2+1: exp(-2 pi i t/N) - 1, without cancellation
+                          // (headn points of the head, headn <= M2)
+  // grid points [0, headn) -- and [N - headn, N) when headneg -- hold R's
+  // largest values (fluxes near zero); they are left out of the transform
+  // and their part of R^(k) - R^(0) is summed directly (forward_tail)
+  int headn, headneg;
   int nfields;
   FieldDev fields[kMaxFields];
 };

@@ -587,6 +587,79 @@ __device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int 
                              XformSmem& sm, const double mn, const double var,
                              unsigned long long tmark);
 
+// R at grid point j, as the fill makes it (the node spline of the interval
+// kst[j], exponentiated by exp2_dflux); 0 outside the fill ranges
+__device__ double r_grid(const BeamsetDev& bs, const double* __restrict__ coef, const int j,
+                         const double* ex2) {
+  const bool inneg = (j >= bs.minitn && j <= bs.maxitn);
+  if (!inneg && !(j >= bs.minit && j <= bs.maxit)) return 0.0;
+  const int k = bs.kst[j];
+  const double xk = bs.xnode[k];
+  const double4 cf = *reinterpret_cast<const double4*>(coef + 4 * k);
+  const double cflux = inneg ? static_cast<double>(j) * bs.dflux + bs.mval
+                             : static_cast<double>(j) * bs.dflux;
+  const double delx = cflux - xk;
+  const double sv = cf.x + delx * (cf.y + delx * (cf.z + delx * cf.w));
+  return exp2_dflux(sv, ex2);
+}
+
+// head sums work in blocks of this many consecutive grid points
+constexpr int kHeadBlk = 8;
+
+// E(t) = exp(-2 pi i t/N) - 1 from two small tables in LDS: t is folded to
+// v in [0, N/2] (E(N - v) = conj E(v)) and split v = kEmHi h + l, and
+// E(v) = E(l) + E(kEmHi h) + E(l) E(kEmHi h) exactly.  On [0, N/2] |E| grows
+// with the angle, so neither part exceeds |E(v)| much and the sum has no
+// cancellation: about two roundings of |E(v)|.
+constexpr int kEmHi = 512;
+__device__ __forceinline__ double2 em1_at(const BeamsetDev& bs, const double2* elo,
+                                          const double2* ehi, uint32_t t) {
+  const int u = static_cast<int>(t & static_cast<uint32_t>(bs.N - 1));
+  const bool lo = u <= bs.M;
+  const int v = lo ? u : bs.N - u;
+  const double2 a = elo[v & (kEmHi - 1)], b = ehi[v / kEmHi];
+  const double2 r = make_double2(a.x + b.x + (a.x * b.x - a.y * b.y),
+                                 a.y + b.y + (a.x * b.y + a.y * b.x));
+  return make_double2(r.x, lo ? r.y : -r.y);
+}
+
+// E(x + y) = E(x) + E(y) + E(x) E(y) for E(t) = exp(-2 pi i t/N) - 1
+__device__ __forceinline__ double2 em1_add(const double2 a, const double2 b) {
+  return make_double2(a.x + b.x + (a.x * b.x - a.y * b.y), a.y + b.y + (a.x * b.y + a.y * b.x));
+}
+
+// sum_i g[i] (exp(-2 pi i k i/N) - 1) over the blocks [a_lo, a_hi) of 8
+// consecutive terms: E(8ak + bk) = W^{8ak} E(bk) + E(8ak) exactly, so each
+// block contributes (1 + E(8ak)) U_a + E(8ak) V_a, U_a = sum_b g E(bk),
+// V_a = sum_b g.  Every term is formed without cancellation, so the error
+// stays relative to the terms (small for small k), not to sum g.  E(8ak)
+// starts from the table at a_lo and steps by E(8k) (em1_add): a run of at
+// most a few steps adds a few roundings of |E(8ak)|.  8 k a < 2^31 for
+// k <= N/2 and a < 2^11.
+__device__ double2 head_sum(const BeamsetDev& bs, const double2* elo, const double2* ehi,
+                            const double* g, const double* vsum, const int a_lo, const int a_hi,
+                            const int k, const double2* eb, const double2 e8) {
+  double2 acc = make_double2(0.0, 0.0);
+  if (a_lo >= a_hi) return acc;
+  double2 ea = em1_at(bs, elo, ehi, static_cast<uint32_t>(kHeadBlk) *
+                                        static_cast<uint32_t>(k) * static_cast<uint32_t>(a_lo));
+  for (int a = a_lo; a < a_hi; ++a) {
+    const double* ga = g + kHeadBlk * a;
+    double2 u = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int b = 1; b < kHeadBlk; ++b) {
+      u.x = fma(ga[b], eb[b].x, u.x);
+      u.y = fma(ga[b], eb[b].y, u.y);
+    }
+    // (1 + ea) u + ea v
+    const double2 t1 = make_double2(u.x + (ea.x * u.x - ea.y * u.y), u.y + (ea.x * u.y + ea.y * u.x));
+    acc.x += t1.x + ea.x * vsum[a];
+    acc.y += t1.y + ea.y * vsum[a];
+    ea = em1_add(ea, e8);
+  }
+  return acc;
+}
+
 // The forward transform of one walker is split over `parts` workgroups
 // (1 to 4) by ranges of the 16-column chunks: with one part the
 // workgroup finishes the spectrum itself; with more, each writes its partial
@@ -643,8 +716,15 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     for (int s = 0; s < kRowsPerPass / 4; ++s) acc[s] = make_double2(0.0, 0.0);
     // fill: thread (n1 = row4, q) makes R at 8 consecutive grid points of
     // chunk c into dst
+    // the head of the grid (the first headn points, and the last headn with
+    // a negative beam) enters the moments but not the transform:
+    // forward_tail adds its part directly
+    const bool head_row = (row4 == 0) || (bs.headneg && row4 == kM1 - 1);
     auto fill_chunk = [&](const int c, double2* dst) {
       const int j0 = 2 * M2 * row4 + 2 * kChunk * c + 8 * q;
+      // headn is a multiple of 8: a thread's 8 points are all in the head or
+      // all outside it
+      const bool hz = head_row && (j0 < bs.headn || j0 >= bs.N - bs.headn);
       // node intervals of the 8 fluxes (gsl_interp_accel_find, precomputed)
       const uint4 kq = *reinterpret_cast<const uint4*>(bs.kst + j0);
       const uint32_t kw[4] = {kq.x, kq.y, kq.z, kq.w};
@@ -694,7 +774,8 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
           }
         }
         if (e & 1)
-          dst[cpos(4 * q + (e >> 1), row4)] = make_double2(rprev, rv);
+          dst[cpos(4 * q + (e >> 1), row4)] = hz ? make_double2(0.0, 0.0)
+                                                 : make_double2(rprev, rv);
         rprev = rv;
       }
     };
@@ -830,13 +911,46 @@ __device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int 
   const bool doshift = (sgpos0 > dflux) && (fabs(mn) > dflux);
   const double shift = doshift ? -mn : 0.0;
   const double2 z0 = zrows[0];  // row k2 = 0 is always row position 0
-  const double r0 = z0.x + z0.y;
+  const double r0 = z0.x + z0.y;  // sum of R outside the head
+  // the head of the grid (left out of the transform): R there, in the
+  // free transform buffer -- hp[i] = R[i], i < H; hn[v] = R[N - v],
+  // 1 <= v <= H (with a negative beam); V sums of kHeadBlk consecutive values
+  const int H = bs.headn;
+  const int nbp = H / kHeadBlk, nbn = bs.headneg ? H / kHeadBlk + 1 : 0;
+  double* hp = reinterpret_cast<double*>(sm.bufa);
+  double* hn = hp + H;                // H + kHeadBlk values, index v (hn[0] = 0)
+  double* vp = hn + H + kHeadBlk;     // nbp
+  double* vn = vp + nbp;              // nbn
+  // the two twiddle tables of em1_at, in the other free transform buffer
+  double2* elo = sm.bufb;             // kEmHi: E(l)
+  double2* ehi = elo + kEmHi;         // M / kEmHi + 1: E(kEmHi h)
+  {
+    const double* coef = wk.coef + walker * wk.coef_stride;
+    for (int i = tid; i < kEmHi; i += kThreads) elo[i] = bs.em1[i];
+    for (int i = tid; i <= bs.M / kEmHi; i += kThreads) ehi[i] = bs.em1[i * kEmHi];
+    if (tid < 32) sm.ex2[tid] = exp2(static_cast<double>(tid) * 0.03125) * dflux;
+    __syncthreads();
+    for (int i = tid; i < H; i += kThreads) hp[i] = r_grid(bs, coef, i, sm.ex2);
+    for (int v = tid; v < H + kHeadBlk; v += kThreads)
+      hn[v] = (bs.headneg && v >= 1 && v <= H) ? r_grid(bs, coef, bs.N - v, sm.ex2) : 0.0;
+    __syncthreads();
+    for (int a = tid; a < nbp + nbn; a += kThreads) {
+      const double* g = (a < nbp) ? hp + kHeadBlk * a : hn + kHeadBlk * (a - nbp);
+      double v = 0.0;
+      for (int b = 0; b < kHeadBlk; ++b) v += g[b];
+      ((a < nbp) ? vp : vn - nbp)[a] = v;
+    }
+    __syncthreads();
+  }
   if (tid == 0) {
+    double vh = 0.0;
+    for (int a = 0; a < nbp; ++a) vh += vp[a];
+    for (int a = 0; a < nbn; ++a) vh += vn[a];
     st[ST_MN] = mn;
     st[ST_VAR] = var;
     st[ST_DOSHIFT] = doshift ? 1.0 : 0.0;
     st[ST_SHIFT] = shift;
-    st[ST_R0] = r0;
+    st[ST_R0] = r0 + vh;
     st[ST_TFWD] = static_cast<double>(wall_clock64() - tmark);
     for (int f = 0; f < bs.nfields; ++f)
       sm.isc[2 + f] = spectral_kmax(sigmult * bs.fields[f].sigma_base, bs.iflux,
@@ -847,6 +961,46 @@ __device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int 
   // ---- r2c post-processing + prtrans = exp(R^ - R^0) e^{i(phase - shift w)}
   // (src/PDFactory.cc:1094-1132)
   double2* prtrans = wk.prtrans + walker * wk.spec_stride;
+  // the head's part of R^(k) - R^(0), summed directly, parked in
+  // prtrans[k]: kHeadSplit consecutive lanes share a k, each summing a
+  // range of the blocks, and their partial sums are combined in lane order
+  {
+    constexpr int kHeadSplit = 4;
+    const int nitems = (kmaxb + 1) * kHeadSplit;
+    for (int it0 = 0; it0 < nitems; it0 += kThreads) {  // uniform trip count
+      const int it = it0 + tid;
+      const bool act = it < nitems;
+      const int k = act ? it / kHeadSplit : 0;
+      const int part = it % kHeadSplit;
+      double2 dh = make_double2(0.0, 0.0);
+      if (act) {
+        // E(bk), b < 8, by steps of E(k) from the table; E(8k) for the blocks
+        double2 eb[kHeadBlk];
+        const double2 e1 = em1_at(bs, elo, ehi, static_cast<uint32_t>(k));
+        eb[0] = make_double2(0.0, 0.0);
+        eb[1] = e1;
+#pragma unroll
+        for (int b = 2; b < kHeadBlk; ++b) eb[b] = em1_add(eb[b - 1], e1);
+        const double2 e8 = em1_add(eb[kHeadBlk - 1], e1);
+        dh = head_sum(bs, elo, ehi, hp, vp, part * nbp / kHeadSplit,
+                      (part + 1) * nbp / kHeadSplit, k, eb, e8);
+        if (nbn) {
+          const double2 dn = head_sum(bs, elo, ehi, hn, vn, part * nbn / kHeadSplit,
+                                      (part + 1) * nbn / kHeadSplit, k, eb, e8);
+          dh.x += dn.x;  // W^{(N - v) k} - 1 = conj(E(vk))
+          dh.y -= dn.y;
+        }
+      }
+#pragma unroll
+      for (int off = 1; off < kHeadSplit; off <<= 1) {
+        dh.x += __shfl_xor(dh.x, off);
+        dh.y += __shfl_xor(dh.y, off);
+      }
+      if (act && part == 0) prtrans[k] = dh;
+    }
+  }
+  __threadfence_block();  // other lanes read these head sums below
+  __syncthreads();
   {
     const double wfac = shift * bs.iflux;
     for (int k = tid; k <= kmaxb; k += kThreads) {
@@ -864,11 +1018,12 @@ __device__ void forward_tail(const BeamsetDev& bs, const WorkDev& wk, const int 
       double sn, cs;
       sincospi(static_cast<double>(k) / static_cast<double>(M), &sn, &cs);
       const double2 X = cadd(A, cmul(make_double2(cs, -sn), B));
+      const double2 dh = prtrans[k];  // the head sum, written above
       double expfac, ival;
       {
 #pragma clang fp contract(off)
-        expfac = exp(X.x - r0);
-        ival = doshift ? X.y - wfac * static_cast<double>(k) : X.y;
+        expfac = exp((X.x - r0) + dh.x);
+        ival = doshift ? (X.y + dh.y) - wfac * static_cast<double>(k) : X.y + dh.y;
       }
       double s2, c2;
       sincos(ival, &s2, &c2);

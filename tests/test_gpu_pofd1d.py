@@ -151,17 +151,17 @@ def test_deterministic(small):
     assert np.array_equal(ll, ll2)
 
 
-def _full_size(binned, smax, nwalk=64):
+def _full_size(binned, smax, nwalk=256):
     inp = workload.make_inputs(npix_side=1000, seed=42, smax=smax)
     th = synth.walker_thetas(nwalk, seed=1234)
     ctx = workload.build_context(inp, 131072, 2048, binned=binned)
     ll, st = ctx.loglike_batch(th)
     L = build_oracle(inp, 131072, 2048, binned, th[0])
     dbl, st_d = L.loglike_batch(th, 16)
-    L.set_precise(1)  # extended-precision transforms: exact-math reference
+    L.set_precise(1)  # extended-precision transforms and sums: exact math
     ext, st_e = L.loglike_batch(th, 16)
     assert np.all(st == 0) and np.all(st_d == 0) and np.all(st_e == 0)
-    return ll, dbl, ext
+    return ctx, L, th, ll, dbl, ext
 
 
 @pytest.mark.parametrize("binned", [True, False])
@@ -170,23 +170,50 @@ def test_config1d_in_support(binned):
     every walker within 1e-6 of the exact-math value of the reference's
     algorithm (extended-precision oracle)."""
     _say("full size, in-support map, binned=%s" % binned)
-    ll, dbl, ext = _full_size(binned, smax=0.45)
+    _, _, _, ll, dbl, ext = _full_size(binned, smax=0.45, nwalk=64)
     np.testing.assert_allclose(ll, ext, rtol=0, atol=1e-6)
     np.testing.assert_allclose(dbl, ext, rtol=0, atol=1e-6)
 
 
 @pytest.mark.parametrize("binned", [True, False])
 def test_config1d_full_size(binned):
-    """The measured configuration: sources to 600 mJy, above the last knot,
-    so the brightest pixels sit where the P(D) is ~1e-9 of its peak.  There
-    any double-precision FFT (the reference's FFTW included) carries
-    rounding noise of a few 1e-6 in logL, measured here as the distance of
-    the double-precision oracle from the extended-precision one.  The GPU
-    must be no noisier than that, to within 1e-6, and within 1e-6 for the
-    median walker."""
+    """The measured configuration, all 256 bench walkers, against the
+    exact-math value (extended-precision oracle) at 1e-6.
+
+    The split search takes the argmin of the P(D) when the values before it
+    decrease (src/PDFactory.cc:783-795).  Near that smooth minimum two
+    neighbouring values can differ by less than any double-precision
+    transform's rounding there (~3e-14 of a 1e-7 value), so which index a
+    double computation -- the reference's FFTW included -- returns is
+    decided by rounding, and one index moves logL by ~3e-6 at this size
+    (test_oracle_split.py).  A walker off by more than 1e-6 must be such a
+    case: its split one index from the exact one, the two candidates'
+    exact values closer than the double transform's own error there, and
+    the exact-math value at the device's split within 1e-6."""
     _say("full size, measured map, binned=%s" % binned)
-    ll, dbl, ext = _full_size(binned, smax=synth.SMAX_SRC)
-    ref_noise = np.abs(dbl - ext)
-    gpu_err = np.abs(ll - ext)
-    assert np.all(gpu_err <= ref_noise + 1e-6), np.max(gpu_err - ref_noise)
-    assert np.median(gpu_err) <= 1e-6
+    ctx, L, th, ll, dbl, ext = _full_size(binned, smax=synth.SMAX_SRC)
+    err = np.abs(ll - ext)
+    off = np.nonzero(err > 1e-6)[0]
+    _say("walkers beyond 1e-6 of the exact value: %s" % off.tolist())
+    assert off.size <= 8, off
+    for w in off:
+        gsplit = int(ctx.get_debug(int(w))["split"])
+        L.set_precise(1)
+        L.initpd(0, th[w])
+        r_ext = L.getpd(0, 0, float(th[w][8]), setlog=False)
+        L.set_precise(0)
+        L.initpd(0, th[w])
+        r_dbl = L.getpd(0, 0, float(th[w][8]), setlog=False)
+        esplit = r_ext["split"]
+        assert abs(gsplit - esplit) == 1, (w, gsplit, esplit)
+        pe, pdb = r_ext["pofd_raw"], r_dbl["pofd_raw"]
+        noise = np.max(np.abs(pdb[esplit - 64:esplit + 64] -
+                              pe[esplit - 64:esplit + 64]))
+        assert abs(pe[gsplit] - pe[esplit]) < noise, (w, pe[gsplit],
+                                                      pe[esplit], noise)
+        L.set_precise(1)
+        L.set_split_shift(gsplit - esplit)
+        at_dev, st_a = L.loglike_batch(th[w:w + 1], 1)
+        L.set_split_shift(0)
+        assert st_a[0] == 0
+        assert abs(ll[w] - at_dev[0]) <= 1e-6, (w, ll[w] - at_dev[0])
