@@ -447,7 +447,8 @@ typedef struct pofd_ens pofd_ens;
 enum {
   POFD_TARGET_1D = 1,     /* pofd1d_loglike_batch_device on the context */
   POFD_TARGET_2D = 2,     /* pofd2d_loglike_batch_device on the context */
-  POFD_TARGET_GAUSS = 3   /* diagonal Gaussian (validation target) */
+  POFD_TARGET_GAUSS = 3,  /* diagonal Gaussian (validation target) */
+  POFD_TARGET_HOST = 4    /* the caller's function (pofd_ens_set_host_target) */
 };
 /* nwalkers even; npar parameters per walker (<= 64, bonus ones included,
    the theta stride of the likelihood); scalefac the stretch scale a > 1
@@ -457,6 +458,23 @@ int pofd_ens_create(pofd_ens **out, pofd_ctx *ctx, int target,
                     uint64_t seed);
 void pofd_ens_destroy(pofd_ens *ens);
 const char *pofd_ens_last_error(const pofd_ens *ens);
+/* A likelihood the caller evaluates (POFD_TARGET_HOST): replaces the
+   virtual affineEnsemble::getLogLike(const paramSet&, bool&) and
+   fillBonusParams(paramSet&, bool) of a subclass (include/affineEnsemble.h:
+   175,179) for likelihoods that are not on the device.  For every
+   half-ensemble step the sampler copies the n proposals (rows of stride
+   floats) to the host and calls fn once; fn writes loglike[n], status[n]
+   (0 accepted as a value, nonzero: the proposal is rejected, as a
+   rejected or invalid parameter set is) and, with nbonus > 0, bonus[n *
+   nbonus]: the values of the bonus parameters (in parameter order) that an
+   accepted step stores.  A nonzero return aborts the run with that code.
+   fn runs on the calling thread inside pofd_ens_run / set_positions. */
+typedef int (*pofd_host_loglike_fn)(void *user, const float *theta,
+                                    uint32_t n, uint32_t stride,
+                                    double *loglike, int32_t *status,
+                                    double *bonus);
+int pofd_ens_set_host_target(pofd_ens *ens, pofd_host_loglike_fn fn,
+                             void *user, uint32_t nbonus);
 /* state[npar]: 0 fit, 1 fixed, 2 bonus (mcmc_affine::FIXED / BONUS) */
 int pofd_ens_set_param_state(pofd_ens *ens, const uint8_t *state);
 /* The acceptance exponent of z: fit_dimension 0 (default) uses the

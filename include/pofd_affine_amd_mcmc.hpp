@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <iostream>
 #include <limits>
 #include <memory>
@@ -270,7 +271,22 @@ class affineEnsemble {
     loglike = chain_ll_;
   }
 
-  /* the subclass sets up its likelihood (setTarget) and parameter states */
+  /* the likelihood of a subclass that evaluates it on the host
+     (include/affineEnsemble.h:175): used when initChains chose
+     setHostTarget; pars_invalid rejects the step.  Subclasses with a
+     device likelihood (pofdMCMC, pofdMCMCDouble) also answer single calls
+     with it. */
+  virtual double getLogLike(const paramSet&, bool& pars_invalid) {
+    pars_invalid = true;
+    throw affineExcept("affineEnsemble", "getLogLike",
+                       "no host likelihood: override getLogLike or use a device target");
+  }
+  /* the bonus parameters of p after its likelihood (include/affineEnsemble.h:
+     179); rej: the step was rejected.  The default leaves them alone. */
+  virtual void fillBonusParams(paramSet&, bool) {}
+
+  /* the subclass sets up its likelihood (setTarget or setHostTarget) and
+     parameter states */
   virtual void initChains() = 0;
   /* the subclass draws the walkers' starting points around p
      (setInitialPositions) */
@@ -321,6 +337,16 @@ class affineEnsemble {
           "setTarget");
     float dexp = 0.0f;
     check(pofd_ens_set_exponent_mode(ens_, fit_dimension_ ? 1 : 0, &dexp), "setTarget");
+  }
+  /* the sampler over the subclass's own getLogLike / fillBonusParams on the
+     host: proposals and acceptance stay on the device, each half-step's
+     proposals come to the host for their likelihoods (a proposal for which
+     areParamsValid is false is rejected without a likelihood call) */
+  void setHostTarget(const std::vector<float>& lower = std::vector<float>(),
+                     const std::vector<float>& upper = std::vector<float>()) {
+    setTarget(nullptr, POFD_TARGET_HOST, lower, upper);
+    check(pofd_ens_set_host_target(ens_, &affineEnsemble::hostLoglike, this, getNBonusParams()),
+          "setHostTarget");
   }
   /* one position per walker; evaluates them (and their bonus parameters) */
   void setInitialPositions(const std::vector<paramSet>& pos) {
@@ -375,7 +401,42 @@ class affineEnsemble {
     for (int s : state_) n += any ? ((s & mask) != 0) : ((s & mask) == 0);
     return n;
   }
+  std::exception_ptr host_error_;
+  static int hostLoglike(void* user, const float* th, uint32_t n, uint32_t stride, double* ll,
+                         int32_t* st, double* bonus) {
+    affineEnsemble* self = static_cast<affineEnsemble*>(user);
+    try {
+      const unsigned P = self->nparams_;
+      std::vector<unsigned> bidx;
+      for (unsigned i = 0; i < P; ++i)
+        if (self->state_[i] & kBonus) bidx.push_back(i);
+      for (uint32_t r = 0; r < n; ++r) {
+        paramSet p(P, th + static_cast<size_t>(r) * stride);
+        bool rej = false;
+        double v = 0.0;
+        if (!self->areParamsValid(p))
+          rej = true;
+        else
+          v = self->getLogLike(p, rej);
+        ll[r] = v;
+        st[r] = rej ? POFD_E_ARG : POFD_OK;
+        if (bonus && !bidx.empty()) {
+          self->fillBonusParams(p, rej);
+          for (size_t b = 0; b < bidx.size(); ++b) bonus[r * bidx.size() + b] = p[bidx[b]];
+        }
+      }
+      return POFD_OK;
+    } catch (...) {
+      self->host_error_ = std::current_exception();
+      return POFD_E_INTERNAL;
+    }
+  }
   void check(int rc, const char* m) const {
+    if (rc && host_error_) {
+      std::exception_ptr e = host_error_;
+      const_cast<affineEnsemble*>(this)->host_error_ = nullptr;
+      std::rethrow_exception(e);
+    }
     if (rc) {
       const char* e = ens_ ? pofd_ens_last_error(ens_) : "";
       throw affineExcept("affineEnsemble", m, std::string(e && *e ? e : "device sampler error"));
@@ -558,7 +619,7 @@ class pofdMCMC : public affineEnsemble {
   }
 
   /* a single log likelihood (pofdMCMC::getLogLike), priors included */
-  double getLogLike(const paramSet& p, bool& rejected) const {
+  double getLogLike(const paramSet& p, bool& rejected) override {
     return likeSet->getLogLike(p, rejected);
   }
 
@@ -684,7 +745,7 @@ class pofdMCMCDouble : public affineEnsemble {
     setInitialPositions(pos);
   }
 
-  double getLogLike(const paramSet& p, bool& rejected) const {
+  double getLogLike(const paramSet& p, bool& rejected) override {
     return likeSet->getLogLike(p, rejected);
   }
 

@@ -37,6 +37,15 @@ struct pofd_ens {
   int* sidx = nullptr;         // parameter index of each surrogate input
   void* dabuf[9] = {};
   uint64_t n_exact = 0, n_proposed = 0;
+  // host likelihood (POFD_TARGET_HOST): the caller's function, host staging
+  // and the device copy of the bonus values it returns
+  pofd_host_loglike_fn host_fn = nullptr;
+  void* host_user = nullptr;
+  uint32_t host_nbonus = 0;
+  std::vector<float> host_theta;
+  std::vector<double> host_ll, host_bonus;
+  std::vector<int32_t> host_st;
+  double* host_bonus_dev = nullptr;
   std::string err;
   ~pofd_ens() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -47,6 +56,7 @@ struct pofd_ens {
     for (void* p : dabuf)
       if (p) (void)hipFree(p);
     if (sidx) (void)hipFree(sidx);
+    if (host_bonus_dev) (void)hipFree(host_bonus_dev);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -77,6 +87,42 @@ void update_dexp(pofd_ens* s) {
   s->e.dexp = static_cast<float>(d - 1);
 }
 
+// the caller's likelihood on the host: the rows come back from the device,
+// the function fills log likelihoods, statuses and bonus values, and they go
+// to the device (the bonus values where the accept kernel reads them)
+int evaluate_host(pofd_ens* s, const float* x, uint32_t n, double* ll, int32_t* st) {
+  if (!s->host_fn) return efail(s, POFD_E_ARG, "pofd_ens_run: no host likelihood set");
+  const size_t P = s->npar, nb = s->host_nbonus;
+  s->host_theta.resize(n * P);
+  s->host_ll.assign(n, 0.0);
+  s->host_st.assign(n, 0);
+  s->host_bonus.assign(n * (nb + 1), 0.0);
+  hipError_t he = hipMemcpyAsync(s->host_theta.data(), x, n * P * sizeof(float),
+                                 hipMemcpyDeviceToHost, s->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(host proposals)");
+  std::vector<double> bon(n * nb, 0.0);
+  const int rc = s->host_fn(s->host_user, s->host_theta.data(), n, static_cast<uint32_t>(P),
+                            s->host_ll.data(), s->host_st.data(), nb ? bon.data() : nullptr);
+  if (rc) return efail(s, rc, "pofd_ens_run: the host likelihood failed");
+  for (uint32_t r = 0; r < n; ++r)
+    for (size_t b = 0; b < nb; ++b) s->host_bonus[r * (nb + 1) + 1 + b] = bon[r * nb + b];
+  he = hipMemcpyAsync(ll, s->host_ll.data(), n * sizeof(double), hipMemcpyHostToDevice, s->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(st, s->host_st.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                        s->stream);
+  if (he == hipSuccess && nb)
+    he = hipMemcpyAsync(s->host_bonus_dev, s->host_bonus.data(), n * (nb + 1) * sizeof(double),
+                        hipMemcpyHostToDevice, s->stream);
+  // the host vectors are reused by the next call: the copies must finish
+  if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(host likelihoods)");
+  s->e.bonus_src = nb ? s->host_bonus_dev : nullptr;
+  s->e.bonus_stride = static_cast<int>(nb + 1);
+  s->e.bonus_count = static_cast<int>(nb);
+  return POFD_OK;
+}
+
 // evaluate n parameter rows x (device) into ll / st (device) on the stream
 int evaluate(pofd_ens* s, const float* x, uint32_t n, double* ll, int32_t* st) {
   switch (s->target) {
@@ -98,6 +144,8 @@ int evaluate(pofd_ens* s, const float* x, uint32_t n, double* ll, int32_t* st) {
       return pofd_ens_dev::launch_gauss(s->e, x, static_cast<int>(n), ll, st, s->stream)
                  ? POFD_E_INTERNAL
                  : POFD_OK;
+    case POFD_TARGET_HOST:
+      return evaluate_host(s, x, n, ll, st);
     default:
       return POFD_E_ARG;
   }
@@ -163,7 +211,8 @@ int pofd_ens_create(pofd_ens** out, pofd_ctx* ctx, int target, uint32_t nwalkers
       npar > static_cast<uint32_t>(pofd_ens_dev::kMaxPar) || !(scalefac > 1.0))
     return POFD_E_ARG;
   if ((target == POFD_TARGET_1D || target == POFD_TARGET_2D) && !ctx) return POFD_E_ARG;
-  if (target != POFD_TARGET_1D && target != POFD_TARGET_2D && target != POFD_TARGET_GAUSS)
+  if (target != POFD_TARGET_1D && target != POFD_TARGET_2D && target != POFD_TARGET_GAUSS &&
+      target != POFD_TARGET_HOST)
     return POFD_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return POFD_E_NOGPU;
@@ -289,10 +338,35 @@ int pofd_ens_set_positions(pofd_ens* s, const float* theta) {
   return refresh_surrogate(s);
 }
 
+int pofd_ens_set_host_target(pofd_ens* s, pofd_host_loglike_fn fn, void* user,
+                             uint32_t nbonus) {
+  if (!s) return POFD_E_ARG;
+  if (s->target != POFD_TARGET_HOST)
+    return efail(s, POFD_E_ARG, "pofd_ens_set_host_target: the sampler's target is not POFD_TARGET_HOST");
+  if (!fn) return efail(s, POFD_E_ARG, "pofd_ens_set_host_target: null function");
+  if (nbonus > s->npar) return efail(s, POFD_E_ARG, "pofd_ens_set_host_target: too many bonus values");
+  if (s->host_bonus_dev) {
+    (void)hipFree(s->host_bonus_dev);
+    s->host_bonus_dev = nullptr;
+  }
+  if (nbonus) {
+    const hipError_t he =
+        hipMalloc(&s->host_bonus_dev, static_cast<size_t>(s->nwalk) * (nbonus + 1) * sizeof(double));
+    if (he != hipSuccess) return ehip(s, he, "hipMalloc(host bonus)");
+  }
+  s->host_fn = fn;
+  s->host_user = user;
+  s->host_nbonus = nbonus;
+  return POFD_OK;
+}
+
 int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_index,
                            uint32_t nidx, double gate, double holdout_rms, int* enabled) {
   if (!s) return POFD_E_ARG;
   if (enabled) *enabled = 0;
+  if (s->target == POFD_TARGET_HOST && mlp)
+    return efail(s, POFD_E_ARG,
+                 "pofd_ens_set_surrogate: delayed acceptance needs a device likelihood");
   s->use_da = false;
   s->mlp = mlp;
   if (!mlp) return POFD_OK;

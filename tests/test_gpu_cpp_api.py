@@ -320,3 +320,58 @@ def test_cpp_pofdmcmcdouble_from_files(tmp_path):
     rows = np.loadtxt(chain)
     assert rows.shape == (60 * 24, ntot + 8)
     assert rows[:, 1].min() >= tv[1] - 1.0
+
+
+EXE_HOST = os.path.join(ROOT, "tests", "cpp", "_build", "hostlike_check")
+
+
+def _multigauss_cov():
+    sig = np.array([1.0, 0.9, 0.8, 0.7, 0.6])
+    band = {0: 1.0, 1: 0.2, 2: 0.1}
+    corr = np.array([[band.get(abs(i - j), 0.0) for j in range(5)]
+                     for i in range(5)])
+    return corr * sig[:, None] * sig[None, :]
+
+
+def test_cpp_host_likelihood_subclass():
+    """affineEnsemble's virtual getLogLike / fillBonusParams /
+    areParamsValid (include/affineEnsemble.h:175,179) in a subclass written
+    as the reference's examples are (a correlated 5-d Gaussian centred at
+    0.5 with a bonus parameter, the sum of the five), sampled by the device
+    stretch move with the likelihood on the host: the chain reproduces the
+    target's means and variances -- parameter 0 as the normal truncated at
+    3 by areParamsValid -- every stored bonus value is the sum of its
+    row, and an exception thrown by getLogLike reaches the caller."""
+    print("[gpu-case] C++ affineEnsemble subclass with a host likelihood",
+          flush=True)
+    assert os.path.exists(EXE_HOST), "build tests/cpp first"
+    cov = _multigauss_cov()
+    icov = np.linalg.inv(cov)
+    stdin = "\n".join("%.17g" % v for v in icov.ravel())
+    out = subprocess.run([EXE_HOST, "64", "2000", "7"], input=stdin,
+                         capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr + out.stdout
+    vals = {}
+    for line in out.stdout.split("\n"):
+        f = line.split()
+        if len(f) == 3:
+            vals[(f[0], int(f[1]))] = float(f[2])
+        elif len(f) == 2:
+            vals[f[0]] = float(f[1])
+    # truncated normal of parameter 0 at (3 - 0.5) / 1 = 2.5 sigma
+    from scipy.stats import truncnorm
+    tn = truncnorm(-np.inf, 2.5, loc=0.5, scale=1.0)
+    sd = np.sqrt(np.diag(cov))
+    for i in range(5):
+        want_m = tn.mean() if i == 0 else 0.5
+        want_v = tn.var() if i == 0 else cov[i, i]
+        assert abs(vals[("mean", i)] - want_m) < 0.12 * sd[i], (i, vals)
+        assert abs(vals[("var", i)] / want_v - 1.0) < 0.15, (i, vals)
+    assert vals["bonus_maxdiff"] == 0.0
+    assert vals["max0"] < 3.0
+    assert vals["rows"] == 64 * 2000
+    assert 0.2 < vals["accept"] < 0.8
+    out = subprocess.run([EXE_HOST, "8", "50", "7", "300"], input=stdin,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr + out.stdout
+    assert "caught multiGauss::getLogLike" in out.stdout
