@@ -176,6 +176,24 @@ int pofd1d_loglike_batch_device_count(pofd_ctx *ctx, const float *d_theta,
                                       uint32_t theta_stride,
                                       double *d_loglike_out,
                                       int32_t *d_status_out, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory -- code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc: pofd1d_loglike_batch_device with the per-walker
+   workspace of every beam set laid out in a fixed device scratch of 512 MiB
+   per GPU (reserved when the library's code is loaded, held until the
+   process ends) instead of memory the context allocates; it calls no
+   hipMalloc or hipFree and records no stage events.  A batch whose
+   workspace exceeds the scratch returns POFD_E_ARG and says how much it
+   needs (at nflux 2^17 with one dataset: up to about 64 walkers).  All
+   calls on one GPU share its scratch: calls that overlap on one GPU (other
+   streams, other host threads, other contexts) race.  Results equal
+   pofd1d_loglike_batch_device's bit for bit; the debug accessors describe
+   the allocating entry points' batches only. */
+int pofd1d_loglike_batch_device_noalloc(pofd_ctx *ctx, const float *d_theta,
+                                        uint32_t n_walkers,
+                                        uint32_t theta_stride,
+                                        double *d_loglike_out,
+                                        int32_t *d_status_out, void *stream);
 /* Priors and regularisation of calcLike::getLogLike (src/calcLike.cc:
    923-952), added on the device to every batch's log likelihood after the
    dataset sum (walkers whose datasets failed are left as they are):

@@ -76,6 +76,9 @@ _SIGS = {
                                        I32P]),
     "pofd1d_loglike_batch_device": (C.c_int, [VP, VP, C.c_uint32,
                                               C.c_uint32, VP, VP, VP]),
+    "pofd1d_loglike_batch_device_noalloc": (C.c_int, [VP, VP, C.c_uint32,
+                                                      C.c_uint32, VP, VP,
+                                                      VP]),
     "pofd1d_get_pd": (C.c_int, [VP, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                 DP, DP, DP]),
     "pofd1d_get_debug": (C.c_int, [VP, C.c_uint32, C.c_int, C.c_int, DP]),
@@ -396,6 +399,14 @@ class Context:
         self._check(lib().pofd1d_loglike_batch_device(
             self.ptr, VP(d_theta), nwalk, stride, VP(d_ll), VP(d_st),
             VP(stream)), "pofd1d_loglike_batch_device")
+
+    def loglike_batch_device_noalloc(self, d_theta, nwalk, stride, d_ll,
+                                     d_st, stream=0):
+        """loglike_batch_device with the workspace in the library's fixed
+        device scratch: allocates nothing."""
+        self._check(lib().pofd1d_loglike_batch_device_noalloc(
+            self.ptr, VP(d_theta), nwalk, stride, VP(d_ll), VP(d_st),
+            VP(stream)), "pofd1d_loglike_batch_device_noalloc")
 
     def loglike_batch_device_count(self, d_theta, max_walkers, d_count,
                                    stride, d_ll, d_st, stream=0):

@@ -100,6 +100,7 @@ struct pofd_ctx {
   // parameter captured by value changes; a captured graph records it and
   // refuses to launch once it differs
   uint64_t generation = 0;
+  int noalloc_rc = POFD_E_INTERNAL;  // pofd::noalloc_init's result
   ~pofd_ctx() {
     for (auto& ev : stage_events)
       for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -532,59 +533,50 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   return POFD_OK;
 }
 
-int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
-  if (s.wcap >= nwalk && s.wbuf) return POFD_OK;
-  const BeamsetDev& d = s.dev;
-  WorkDev& w = s.wk;
+// Per-walker workspace of a beam set for batches of up to nwalk walkers:
+// the strides and pointers into `base` (nullptr: sizes only).  Returns the
+// number of doubles it spans.
+size_t workspace_layout(const BeamsetDev& d, uint32_t nwalk, double* base, WorkDev& w) {
   const size_t nf = static_cast<size_t>(d.nfields);
-  w.coef_stride = static_cast<size_t>(d.nseg) * d.ninterp * 4;
-  w.ynode_stride = static_cast<size_t>(d.nseg) * d.ninterp;
-  // forward rows, then the noisy rows of each dataset
-  w.zrows_stride = (1 + nf) * static_cast<size_t>(d.M);
-  w.spec_stride = static_cast<size_t>(d.M) + 1;
-  w.pofd_stride = static_cast<size_t>(d.N);
-  w.blk_stride = static_cast<size_t>(d.N / pofd::kBlk);
   auto r8 = [](size_t n) { return (n + 7) & ~size_t(7); };  // 64-byte units
-  const size_t n_coef = r8(w.coef_stride) * nwalk;
-  const size_t n_ynode = r8(w.ynode_stride) * nwalk;
-  const size_t n_z = 2 * r8(w.zrows_stride) * nwalk;
-  const size_t n_spec = r8(2 * w.spec_stride) * nwalk;  // prtrans
-  const size_t n_pofd = r8(w.pofd_stride) * nf * nwalk;
-  const size_t n_blk = 2 * r8(w.blk_stride) * nf * nwalk;
-  const size_t n_dbg = r8(pofd::kDbg) * nf * nwalk;
-  const size_t n_state = r8(pofd::kState) * nwalk;
-  const size_t n_upart = r8(pofd::kPixBlocks) * nf * nwalk;
-  const size_t n_bonus = r8(4 * static_cast<size_t>(nwalk));
+  w.coef_stride = r8(static_cast<size_t>(d.nseg) * d.ninterp * 4);
+  w.ynode_stride = r8(static_cast<size_t>(d.nseg) * d.ninterp);
+  // forward rows, then the noisy rows of each dataset
+  w.zrows_stride = r8((1 + nf) * static_cast<size_t>(d.M) * 2) / 2;
+  w.spec_stride = r8(2 * (static_cast<size_t>(d.M) + 1)) / 2;
+  w.pofd_stride = r8(static_cast<size_t>(d.N));
+  w.blk_stride = r8(static_cast<size_t>(d.N / pofd::kBlk));
   // (walker, part) slots for batches split over several workgroups per
   // walker: at most 4 parts, and walkers x parts <= 256
   const uint32_t nslots = std::min<uint32_t>(pofd::kMaxSlots, 4 * nwalk);
   w.zpart_stride = r8(2 * static_cast<size_t>(d.M)) / 2;
+  const size_t n_coef = w.coef_stride * nwalk;
+  const size_t n_ynode = w.ynode_stride * nwalk;
+  const size_t n_z = 2 * w.zrows_stride * nwalk;
+  const size_t n_spec = 2 * w.spec_stride * nwalk;  // prtrans
+  const size_t n_pofd = w.pofd_stride * nf * nwalk;
+  const size_t n_blk = 2 * w.blk_stride * nf * nwalk;
+  const size_t n_dbg = r8(pofd::kDbg) * nf * nwalk;
+  const size_t n_state = r8(pofd::kState) * nwalk;
+  const size_t n_upart = r8(pofd::kPixBlocks) * nf * nwalk;
+  const size_t n_bonus = r8(4 * static_cast<size_t>(nwalk));
   const size_t n_zpart = 2 * w.zpart_stride * nslots;
   const size_t n_mpart = r8(2 * static_cast<size_t>(nslots));
   const size_t n_ipart = r8(static_cast<size_t>(nslots) * nf * pofd::kIPart);
-  const size_t total = n_coef + n_ynode + n_z + n_spec + n_pofd + n_blk +
-                       n_dbg + n_state + n_upart + n_bonus + n_zpart + n_mpart +
-                       n_ipart;
-  s.wbuf.reset();
-  int rc = dev_alloc(c, s.wbuf, total * sizeof(double));
-  if (rc) return rc;
-  double* base = static_cast<double*>(s.wbuf->p);
-  w.coef_stride = r8(w.coef_stride);
-  w.ynode_stride = r8(w.ynode_stride);
+  const size_t total = n_coef + n_ynode + n_z + n_spec + n_pofd + n_blk + n_dbg + n_state +
+                       n_upart + n_bonus + n_zpart + n_mpart + n_ipart;
+  w.nslots = nslots;
+  if (!base) return total;
   w.coef = base;
   base += n_coef;
   w.ynode = base;
   base += n_ynode;
-  w.zrows_stride = r8(w.zrows_stride * 2) / 2;
   w.zrows = reinterpret_cast<double2*>(base);
   base += n_z;
-  w.spec_stride = r8(2 * w.spec_stride) / 2;
   w.prtrans = reinterpret_cast<double2*>(base);
   base += n_spec;
-  w.pofd_stride = r8(w.pofd_stride);
   w.pofd = base;
   base += n_pofd;
-  w.blk_stride = r8(w.blk_stride);
   w.bmin = base;
   base += n_blk / 2;
   w.bsum = base;
@@ -602,7 +594,17 @@ int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
   w.mpart = base;
   base += n_mpart;
   w.ipart = base;
-  w.nslots = nslots;
+  return total;
+}
+
+int ensure_workspace(pofd_ctx* c, Beamset& s, uint32_t nwalk) {
+  if (s.wcap >= nwalk && s.wbuf) return POFD_OK;
+  WorkDev w{};
+  const size_t total = workspace_layout(s.dev, nwalk, nullptr, w);
+  s.wbuf.reset();
+  int rc = dev_alloc(c, s.wbuf, total * sizeof(double));
+  if (rc) return rc;
+  workspace_layout(s.dev, nwalk, static_cast<double*>(s.wbuf->p), s.wk);
   s.wcap = nwalk;
   ++c->generation;
   return POFD_OK;
@@ -640,6 +642,9 @@ extern "C" int pofd_create(pofd_ctx** out, int hip_device) {
   auto* c = new (std::nothrow) pofd_ctx();
   if (!c) return POFD_E_INTERNAL;
   c->device = hip_device;
+  // the allocation-free entry point's scratch on this device (its address
+  // is resolved once per device); a failure is reported by that entry point
+  c->noalloc_rc = pofd::noalloc_init(hip_device);
   *out = c;
   return POFD_OK;
 }
@@ -884,6 +889,60 @@ extern "C" int pofd1d_loglike_batch_device_count(pofd_ctx* c, const float* d_the
                                                  void* stream) {
   if (!d_count) return fail(c, POFD_E_ARG, "pofd1d_loglike_batch_device_count: null count");
   return batch_device(c, d_theta, max_walkers, d_count, stride, d_ll, d_st, stream);
+}
+
+// ---- the allocation-free batch: the same launches as
+// pofd1d_loglike_batch_device, with every beam set's workspace laid out in
+// the device scratch of noalloc.hip instead of memory the context
+// allocates.  Every kernel writes each workspace value before it reads it,
+// so nothing in the scratch has to start at zero.
+extern "C" int pofd1d_loglike_batch_device_noalloc(pofd_ctx* c, const float* d_theta,
+                                                   uint32_t nwalk, uint32_t stride,
+                                                   double* d_ll, int32_t* d_st, void* stream) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (nwalk == 0) return POFD_OK;
+  if (!d_theta || !d_ll || !d_st)
+    return fail(c, POFD_E_ARG, "pofd1d_loglike_batch_device_noalloc: null buffer");
+  if (stride < c->knots.size() + 1)
+    return fail(c, POFD_E_ARG, "calcLike::getLogLike: Not enough elements in paramSet");
+  if (c->noalloc_rc != POFD_OK)
+    return fail(c, POFD_E_INTERNAL,
+                "pofd1d_loglike_batch_device_noalloc: the device scratch could not be resolved");
+  char* scratch = static_cast<char*>(pofd::noalloc_scratch(c->device));
+  if (!scratch)
+    return fail(c, POFD_E_INTERNAL, "pofd1d_loglike_batch_device_noalloc: no device scratch");
+  const size_t nb = c->sets.size();
+  std::vector<WorkDev> wks(nb);
+  std::vector<size_t> offs(nb);
+  size_t total = 0;
+  for (size_t b = 0; b < nb; ++b) {
+    offs[b] = total;
+    const size_t bytes = workspace_layout(c->sets[b]->dev, nwalk, nullptr, wks[b]) * sizeof(double);
+    total += (bytes + 255) & ~size_t(255);
+  }
+  if (total > pofd::kNoallocScratchBytes)
+    return fail(c, POFD_E_ARG,
+                "pofd1d_loglike_batch_device_noalloc: " + std::to_string(nwalk) +
+                    " walkers need " + std::to_string(total >> 20) +
+                    " MiB of workspace, above the " +
+                    std::to_string(pofd::kNoallocScratchBytes >> 20) + " MiB device scratch");
+  for (size_t b = 0; b < nb; ++b) {
+    Beamset& s = *c->sets[b];
+    workspace_layout(s.dev, nwalk, reinterpret_cast<double*>(scratch + offs[b]), wks[b]);
+    s.dev.bonus = (b == 0) ? 1 : 0;  // the counts integrals once per batch
+    rc = pofd::launch_loglike1d(
+        s.dev, wks[b], static_cast<int>(b), d_theta, nwalk, nullptr, stride, d_ll, d_st,
+        b > 0 ? 1 : 0, c->force_parts, c->parts_tuned ? c->parts_tab.data() : nullptr,
+        c->parts_tuned ? static_cast<const uint8_t*>(c->dparts->p) : nullptr, stream, nullptr);
+    if (rc) return fail(c, rc, "pofd1d_loglike_batch_device_noalloc: launch failed");
+  }
+  if (c->priors.flags || c->priors.reg_alpha > 0.0) {
+    rc = pofd::launch_prior1d(c->sets[0]->dev, wks[0], c->priors, d_theta, stride, nwalk,
+                              nullptr, d_ll, d_st, stream);
+    if (rc) return fail(c, rc, "pofd1d_loglike_batch_device_noalloc: prior launch failed");
+  }
+  return POFD_OK;
 }
 
 // ---- autotuning of the workgroups per walker: every allowed split of each

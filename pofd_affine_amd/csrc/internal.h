@@ -43,6 +43,8 @@ constexpr int kPixBlocks = 64;   // workgroups per (walker, field), unbinned
 constexpr int kGL = 32;          // Gauss-Legendre points per knot interval
 constexpr int kIPart = 8;        // partial statistics per inverse part
 constexpr int kMaxSlots = 256;   // (walker, part) slots of the split transforms
+// device scratch of the allocation-free batch entry point, per GPU
+constexpr size_t kNoallocScratchBytes = size_t(512) << 20;
 
 // One field (dataset) of a beam set, device view.
 struct FieldDev {
@@ -197,6 +199,10 @@ int launch_pd_loglike2d(const double* d_pd, int n1, int n2, double minflux1, dou
                         const double* d_pix2, uint64_t npix, const uint32_t* d_bins,
                         uint32_t nb1, uint32_t nb2, double bincent01, double bindelta1,
                         double bincent02, double bindelta2, double* d_scratch, void* stream);
+// the allocation-free entry point's scratch (noalloc.hip): resolve its
+// address on `device` (current) once, then look it up (nullptr: none)
+int noalloc_init(int device);
+void* noalloc_scratch(int device);
 int launch_materialize_pd(const double* d_pofd, int N, int split, double itot,
                           int is_log, int already_log, double* d_out,
                           void* stream);
