@@ -371,18 +371,15 @@ def bench_multifield(args, hiprt):
     Per-step latency percentiles beside evals/s, and the same work launched
     eagerly field by field for comparison."""
     from pofd_affine_amd import _lib, synth, workload
-    rng = np.random.default_rng(2024)
-    nfield = 8
-    npix = np.exp(rng.uniform(np.log(2e5), np.log(4e6), nfield))
-    sides = np.maximum(64, np.round(np.sqrt(npix)).astype(int))
-    sigmas = rng.uniform(0.0015, 0.004, nfield)
-    nfluxes = (2 ** rng.integers(15, 19, nfield)).astype(int)
+    fields = workload.uneven_fields()
+    nfield = len(fields)
+    sides = np.array([f[0] for f in fields])
+    sigmas = np.array([f[1] for f in fields])
+    nfluxes = np.array([f[2] for f in fields])
     ctxs = []
-    for i in range(nfield):
-        inp = workload.make_inputs(npix_side=int(sides[i]), sigma=float(sigmas[i]),
-                                   seed=300 + i)
-        ctxs.append(workload.build_context(inp, int(nfluxes[i]), args.ninterp,
-                                           binned=False))
+    for side, sigma, nflux, seed in fields:
+        inp = workload.make_inputs(npix_side=side, sigma=sigma, seed=seed)
+        ctxs.append(workload.build_context(inp, nflux, args.ninterp, binned=False))
     W = args.walkers
     th = synth.walker_thetas(W, seed=77)
     stride = th.shape[1]

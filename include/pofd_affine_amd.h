@@ -233,16 +233,17 @@ int pofd1d_graph_launch(pofd_graph *graph, void *stream);
 void pofd1d_graph_destroy(pofd_graph *graph);
 /* Tuning, no reference counterpart: workgroups per walker of the forward
    and inverse transforms.  0 (the default) chooses by batch size,
-   min(4, 256 / n_walkers) and at least 1, so that small batches (such as
-   the survivors of a delayed-acceptance first stage) still fill the GPU's
-   256 compute units; 1 to 4 forces it (reduced while walkers x parts
-   exceeds 256).  With more than one part the partial sums
-   are combined in a fixed order, so results depend on the part count only
-   in the last bits. */
+   min(4, 256 / n_walkers) and at least 1, with 3 taken as 2, so that small
+   batches (such as the survivors of a delayed-acceptance first stage)
+   still fill the GPU's 256 compute units; 1, 2 or 4 forces it (halved
+   while walkers x parts exceeds 256); other values return POFD_E_ARG.
+   Every sum the split touches runs over four fixed column groups combined
+   as (G0 + G1) + (G2 + G3), so the results are the same bits for every
+   part count and batch size. */
 int pofd1d_set_parts(pofd_ctx *ctx, int parts);
 /* Autotuning, no reference counterpart: measures the workgroups per walker.
-   For each of the n_sizes batch sizes (1 to 256) every allowed split (1 to
-   min(4, 256 / size)) is timed with HIP events over `reps` batches of the
+   For each of the n_sizes batch sizes (1 to 256) every allowed split (1, 2
+   or 4, at most 256 / size) is timed with HIP events over `reps` batches of the
    first `size` rows of theta (host memory, at least max(sizes) rows of
    `theta_stride` floats) after one warm-up batch, and the fastest is
    written to chosen[i].  The context then uses the measured choice for

@@ -808,8 +808,8 @@ extern "C" int pofd1d_beamset_info(const pofd_ctx* c, int b, double* out) {
 
 extern "C" int pofd1d_set_parts(pofd_ctx* c, int parts) {
   if (!c) return POFD_E_ARG;
-  if (parts < 0 || parts > 4)
-    return fail(c, POFD_E_ARG, "pofd1d_set_parts: parts must be 0 to 4");
+  if (parts < 0 || parts > 4 || parts == 3)
+    return fail(c, POFD_E_ARG, "pofd1d_set_parts: parts must be 0, 1, 2 or 4");
   c->force_parts = parts;
   ++c->generation;
   return POFD_OK;
@@ -999,7 +999,7 @@ extern "C" int pofd1d_autotune_parts(pofd_ctx* c, const float* theta, uint32_t s
     const int pmax = std::min<int>(4, static_cast<int>(pofd::kMaxSlots / n));
     int best = 1;
     float best_ms = 0.0f;
-    for (int p = 1; p <= pmax && rc == POFD_OK; ++p) {
+    for (int p = 1; p <= pmax && rc == POFD_OK; p <<= 1) {
       c->force_parts = p;
       if ((rc = batch_device(c, d_theta, n, nullptr, stride, d_ll, d_st, nullptr))) break;
       (void)hipEventRecord(e0, nullptr);

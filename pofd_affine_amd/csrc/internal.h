@@ -142,10 +142,17 @@ struct WorkDev {
   const uint8_t* parts_tab;
 };
 
+// The transforms' column chunks fall in kGroups fixed groups, each summed
+// in chunk order, and the group sums combine as (G0 + G1) + (G2 + G3).  A
+// split transform's part covers whole groups (1, 2 or 4 parts), so a
+// walker's sums, and with them its log likelihood, are the same bits for
+// every batch size.
+constexpr int kGroups = 4;
+
 // workgroups per walker of the split transforms for a batch of n walkers
 // (the same rule on the host and in the kernels): the forced count, else
-// the measured table's entry, else min(4, 256 / n); reduced while
-// n x parts exceeds the workspace's slots
+// the measured table's entry, else min(4, 256 / n); 3 becomes 2, and the
+// count halves while n x parts exceeds the workspace's slots
 __host__ __device__ inline int batch_parts(int n, int force, uint32_t nslots,
                                            const uint8_t* tab) {
   int parts = force;
@@ -154,7 +161,8 @@ __host__ __device__ inline int batch_parts(int n, int force, uint32_t nslots,
     parts = (n > 0) ? 256 / n : 4;
     parts = parts < 1 ? 1 : (parts > 4 ? 4 : parts);
   }
-  while (parts > 1 && static_cast<uint32_t>(n * parts) > nslots) --parts;
+  if (parts == 3) parts = 2;
+  while (parts > 1 && static_cast<uint32_t>(n * parts) > nslots) parts >>= 1;
   return parts;
 }
 

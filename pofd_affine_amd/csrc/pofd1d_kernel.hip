@@ -110,6 +110,54 @@ __device__ double block_sum(double v, double* red) {
   return s;
 }
 
+// two block_sum reductions of a kThreads workgroup in one pass (red holds
+// 2 kWaves values), each in block_sum's order
+__device__ void block_sum2(double& a, double& b, double* red) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    red[tid >> 6] = a;
+    red[kWaves + (tid >> 6)] = b;
+  }
+  __syncthreads();
+  a = red[0];
+  b = red[kWaves];
+#pragma unroll
+  for (int i = 1; i < kWaves; ++i) {
+    a += red[i];
+    b += red[kWaves + i];
+  }
+}
+
+// first chunk of chunk group g (of kGroups) over nchunks chunks
+__device__ __forceinline__ int group_chunk(const int g, const int nchunks) {
+  return g * nchunks / kGroups;
+}
+
+// Thread 0's step of the group tree (G0 + G1) + (G2 + G3) of two block
+// sums a and b, taken as the groups complete in order: the open pair
+// becomes G_g (g even) or G_(g-1) + G_g (g odd), and a workgroup holding
+// all four groups keeps G0 + G1 when g = 1.  The open pair starts at zero
+// (tree_start), so a part holding only G1 or G3 gives that group's sum.
+__device__ __forceinline__ void tree_start(double* gt) {
+  if (threadIdx.x == 0) gt[0] = gt[1] = gt[2] = gt[3] = 0.0;
+}
+__device__ __forceinline__ void tree_add(double* gt, const double a, const double b,
+                                         const int g, const int parts) {
+  if (threadIdx.x != 0) return;
+  gt[0] = (g & 1) ? gt[0] + a : a;
+  gt[1] = (g & 1) ? gt[1] + b : b;
+  if (parts == 1 && g == 1) {
+    gt[2] = gt[0];
+    gt[3] = gt[1];
+  }
+}
+
 __device__ double block_max(double v, double* red) {
   const int tid = threadIdx.x;
 #pragma unroll
@@ -301,6 +349,9 @@ struct XformSmem {
   short rowof[512];
   double red[kWaves * 2];
   double sc[4];
+  // the group tree of two block sums (kept by thread 0): the open pair of
+  // each, then the closed G0 + G1 of each
+  double gtree[4];
   double ex2[32];           // 2^(i/32) * dflux (forward only)
   int isc[2 + kMaxFields];  // nrows, kmaxb, then kmax of each dataset
 };
@@ -661,10 +712,10 @@ __device__ double2 head_sum(const BeamsetDev& bs, const double2* elo, const doub
 }
 
 // The forward transform of one walker is split over `parts` workgroups
-// (1 to 4) by ranges of the 16-column chunks: with one part the
-// workgroup finishes the spectrum itself; with more, each writes its partial
-// kept rows and partial moments and spectrum1d_kernel sums them in part
-// order and finishes.
+// (1, 2 or 4) by chunk groups: with one part the workgroup finishes the
+// spectrum itself; with more, each writes its partial kept rows and partial
+// moments and spectrum1d_kernel combines them by the group tree and
+// finishes.  Either way the rows and moments are the same bits.
 __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs,
                                                               const WorkDev wk) {
   __shared__ XformSmem sm;
@@ -702,9 +753,18 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
   __syncthreads();
   const int nrows = sm.isc[0];
   const int nchunks = M2 / kChunk;
-  const int c_lo = part * nchunks / parts, c_hi = (part + 1) * nchunks / parts;
+  // this part's chunk groups (kGroups / parts of them) and their chunks
+  const int g_lo = part * kGroups / parts, g_hi = (part + 1) * kGroups / parts;
+  const int c_lo = group_chunk(g_lo, nchunks), c_hi = group_chunk(g_hi, nchunks);
   const int row4 = tid >> 2, q = tid & 3;
+  // moments: this thread's share of the open group; the group tree in
+  // sm.gtree
   double mn_part = 0.0, var_part = 0.0;
+  tree_start(sm.gtree);
+  // the rows' group tree lives in HBM: the open pair of G0, G1 in zdst and
+  // of G2, G3 in zhi -- with one part the first noisy-row area, which
+  // forward_tail fills only later
+  double2* zhi = (parts == 1) ? zrows + bs.M : zdst;
 
   for (int p0 = 0; p0 < nrows; p0 += kRowsPerPass) {
     const int np = min(kRowsPerPass, nrows - p0);
@@ -714,6 +774,34 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     double2 acc[kRowsPerPass / 4];
 #pragma unroll
     for (int s = 0; s < kRowsPerPass / 4; ++s) acc[s] = make_double2(0.0, 0.0);
+    // closes, in order, every group whose chunks end at or before chunk
+    // cend: its row sums (this thread's rows) and, in the first pass, its
+    // moments enter the group tree
+    int gnext = g_lo;
+    auto close_groups = [&](const int cend) {
+      while (gnext < g_hi && group_chunk(gnext + 1, nchunks) <= cend) {
+        const int g = gnext++;
+        double2* zg = (g >= 2) ? zhi : zdst;
+        const bool add = (g & 1) && g > g_lo;  // G_(g-1) is parked there
+#pragma unroll
+        for (int s = 0; s < kRowsPerPass / 4; ++s) {
+          const int r = 4 * s + q;
+          if (r < np) {
+            double2* z = zg + (p0 + r) * kM1 + row4;
+            *z = add ? cadd(*z, acc[s]) : acc[s];
+          }
+          acc[s] = make_double2(0.0, 0.0);
+        }
+        if (p0 == 0) {
+          double a = mn_part, b = var_part;
+          block_sum2(a, b, sm.red);
+          tree_add(sm.gtree, a, b, g, parts);
+          mn_part = 0.0;
+          var_part = 0.0;
+        }
+      }
+    };
+    close_groups(c_lo);  // groups of no chunks ahead of this part's first
     // fill: thread (n1 = row4, q) makes R at 8 consecutive grid points of
     // chunk c into dst
     // the head of the grid (the first headn points, and the last headn with
@@ -827,6 +915,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       if (c >= c_lo) {
         fft256_cols<-1>(cur, oth, sm.wst);
         accum_chunk(c, cur);
+        close_groups(c + 1);
       }
       // each wave keeps the two phases in order (their registers do not
       // both fit); waves of the workgroup still overlap them
@@ -834,27 +923,34 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       if (c + 1 < c_hi) fill_chunk(c + 1, oth);
       __syncthreads();
     }
+    // one part: (G0 + G1) + (G2 + G3); more parts leave their pair or
+    // group in their slot
+    if (parts == 1) {
 #pragma unroll
-    for (int s = 0; s < kRowsPerPass / 4; ++s) {
-      const int r = 4 * s + q;
-      if (r < np) zdst[(p0 + r) * kM1 + row4] = acc[s];
+      for (int s = 0; s < kRowsPerPass / 4; ++s) {
+        const int r = 4 * s + q;
+        if (r < np) {
+          const int e = (p0 + r) * kM1 + row4;
+          zdst[e] = cadd(zdst[e], zhi[e]);
+        }
+      }
     }
   }
-  const double mn = block_sum(mn_part, sm.red);
-  const double var = block_sum(var_part, sm.red);
   if (parts > 1) {
     if (tid == 0) {
-      wk.mpart[2 * blockIdx.x] = mn;
-      wk.mpart[2 * blockIdx.x + 1] = var;
+      wk.mpart[2 * blockIdx.x] = sm.gtree[0];
+      wk.mpart[2 * blockIdx.x + 1] = sm.gtree[1];
     }
     return;
   }
+  __syncthreads();  // thread 0's group tree
+  const double mn = sm.gtree[2] + sm.gtree[0], var = sm.gtree[3] + sm.gtree[1];
   __threadfence_block();  // the row stores, visible to this workgroup
   __syncthreads();
   forward_tail(bs, wk, walker, sm, mn, var, tmark);
 }
 
-// the sum of the parts' kept rows and moments, in part order, then the
+// the parts' kept rows and moments combined by the group tree, then the
 // spectrum and the noisy rows (one workgroup per walker)
 __global__ __launch_bounds__(kThreads) void spectrum1d_kernel(const BeamsetDev bs,
                                                                const WorkDev wk) {
@@ -877,17 +973,24 @@ __global__ __launch_bounds__(kThreads) void spectrum1d_kernel(const BeamsetDev b
   }
   __syncthreads();
   const int nrows = sm.isc[0];
+  // the group tree over the parts: 2 parts hold G0 + G1 and G2 + G3, 4
+  // parts one group each
   const size_t p0 = static_cast<size_t>(walker) * parts;
-  double mn = wk.mpart[2 * p0], var = wk.mpart[2 * p0 + 1];
-  for (int p = 1; p < parts; ++p) {
-    mn += wk.mpart[2 * (p0 + p)];
-    var += wk.mpart[2 * (p0 + p) + 1];
+  const double* mp = wk.mpart + 2 * p0;
+  double mn, var;
+  if (parts == 2) {
+    mn = mp[0] + mp[2];
+    var = mp[1] + mp[3];
+  } else {
+    mn = (mp[0] + mp[2]) + (mp[4] + mp[6]);
+    var = (mp[1] + mp[3]) + (mp[5] + mp[7]);
   }
   double2* zrows = wk.zrows + walker * wk.zrows_stride;
+  const double2* zp = wk.zpart + p0 * wk.zpart_stride;
+  const size_t zs = wk.zpart_stride;
   for (int e = tid; e < nrows * kM1; e += kThreads) {
-    double2 v = wk.zpart[p0 * wk.zpart_stride + e];
-    for (int p = 1; p < parts; ++p) v = cadd(v, wk.zpart[(p0 + p) * wk.zpart_stride + e]);
-    zrows[e] = v;
+    zrows[e] = (parts == 2) ? cadd(zp[e], zp[zs + e])
+                            : cadd(cadd(zp[e], zp[zs + e]), cadd(zp[2 * zs + e], zp[3 * zs + e]));
   }
   __threadfence_block();
   __syncthreads();
@@ -1137,11 +1240,27 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
   __syncthreads();
   const int nrows = sm.isc[0], botidx = sm.isc[1], topidx = sm.isc[2];
   const int nchunks = M2 / kChunk;
-  const int c_lo = part * nchunks / parts, c_hi = (part + 1) * nchunks / parts;
+  const int g_lo = part * kGroups / parts, g_hi = (part + 1) * kGroups / parts;
+  const int c_lo = group_chunk(g_lo, nchunks), c_hi = group_chunk(g_hi, nchunks);
   const int row4 = tid >> 2, q = tid & 3;
 
+  // the sums of the P(D) and of j P(D): this thread's share of the open
+  // chunk group; the group tree in sm.gtree
   double pk = -1.0, mnv = 0.0, s_part = 0.0, sj_part = 0.0;
   int mni = -1;
+  tree_start(sm.gtree);
+  int gnext = g_lo;
+  auto close_groups = [&](const int cend) {
+    while (gnext < g_hi && group_chunk(gnext + 1, nchunks) <= cend) {
+      const int g = gnext++;
+      double a = s_part, b = sj_part;
+      block_sum2(a, b, sm.red);
+      tree_add(sm.gtree, a, b, g, parts);
+      s_part = 0.0;
+      sj_part = 0.0;
+    }
+  };
+  close_groups(c_lo);
   const bool small_rows = nrows <= kRowsPerPass;
   double2 zr[kRowsPerPass];
 #pragma unroll
@@ -1219,12 +1338,14 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
       }
     }
     __syncthreads();
+    close_groups(c + 1);
   }
   const double peak = block_max(pk, sm.red);
   block_argmin(mnv, mni, sm.red);
-  const double S = block_sum(s_part, sm.red);
-  const double SJ = block_sum(sj_part, sm.red);
   if (tid == 0) {
+    // one part: (G0 + G1) + (G2 + G3); more: this part's pair or group
+    const double S = (parts == 1) ? sm.gtree[2] + sm.gtree[0] : sm.gtree[0];
+    const double SJ = (parts == 1) ? sm.gtree[3] + sm.gtree[1] : sm.gtree[1];
     if (parts == 1) {
       dbg[1] = peak;
       dbg[2] = mni;  // -1: empty search range
@@ -1271,22 +1392,31 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
   const double* bsum = wk.bsum + wf * wk.blk_stride;
   double* dbg = wk.dbg + wf * kDbg;
   if (parts > 1) {
-    // the inverse transform's parts, combined in part (= index) order
+    // the inverse transform's parts: peak and argmin in part (= index)
+    // order, the sums by the group tree (2 parts hold G0 + G1 and G2 + G3,
+    // 4 parts one group each)
     if (tid == 0) {
       const double* ip0 =
           wk.ipart + (static_cast<size_t>(walker) * parts * bs.nfields + f) * kIPart;
-      double pk = ip0[0], bv = ip0[2], s = ip0[3], sj = ip0[4];
+      const size_t ps = static_cast<size_t>(bs.nfields) * kIPart;
+      double pk = ip0[0], bv = ip0[2];
       int bi = static_cast<int>(ip0[1]);
       for (int p = 1; p < parts; ++p) {
-        const double* ip = ip0 + static_cast<size_t>(p) * bs.nfields * kIPart;
+        const double* ip = ip0 + p * ps;
         pk = (ip[0] > pk) ? ip[0] : pk;
         const int ui = static_cast<int>(ip[1]);
         const double u = ip[2];
         const bool take = (ui >= 0) && (bi < 0 || u < bv || (u == bv && ui < bi));
         bv = take ? u : bv;
         bi = take ? ui : bi;
-        s += ip[3];
-        sj += ip[4];
+      }
+      double s, sj;
+      if (parts == 2) {
+        s = ip0[3] + ip0[ps + 3];
+        sj = ip0[4] + ip0[ps + 4];
+      } else {
+        s = (ip0[3] + ip0[ps + 3]) + (ip0[2 * ps + 3] + ip0[3 * ps + 3]);
+        sj = (ip0[4] + ip0[ps + 4]) + (ip0[2 * ps + 4] + ip0[3 * ps + 4]);
       }
       dbg[1] = pk;
       dbg[2] = bi;

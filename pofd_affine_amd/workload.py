@@ -40,3 +40,16 @@ def build_context(inp, nflux=131072, ninterp=2048, binned=True, device=0,
     ctx.prepare()
     ctx._beam = beam
     return ctx
+
+
+def uneven_fields(seed=2024, nfield=8):
+    """The multi-field workload: nfield maps of 2e5-4e6 pixels (log-uniform
+    sizes), noise 1.5-4 mJy and nflux 2^15-2^18, as (side, sigma, nflux,
+    map seed) tuples."""
+    rng = np.random.default_rng(seed)
+    npix = np.exp(rng.uniform(np.log(2e5), np.log(4e6), nfield))
+    sides = np.maximum(64, np.round(np.sqrt(npix)).astype(int))
+    sigmas = rng.uniform(0.0015, 0.004, nfield)
+    nfluxes = (2 ** rng.integers(15, 19, nfield)).astype(int)
+    return [(int(sides[i]), float(sigmas[i]), int(nfluxes[i]), 300 + i)
+            for i in range(nfield)]

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from pofd_affine_amd import synth, workload
-from oracle_side import build_oracle
+from oracle_side import build_oracle, check_exact_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -192,28 +192,5 @@ def test_config1d_full_size(binned):
     the exact-math value at the device's split within 1e-6."""
     _say("full size, measured map, binned=%s" % binned)
     ctx, L, th, ll, dbl, ext = _full_size(binned, smax=synth.SMAX_SRC)
-    err = np.abs(ll - ext)
-    off = np.nonzero(err > 1e-6)[0]
-    _say("walkers beyond 1e-6 of the exact value: %s" % off.tolist())
-    assert off.size <= 8, off
-    for w in off:
-        gsplit = int(ctx.get_debug(int(w))["split"])
-        L.set_precise(1)
-        L.initpd(0, th[w])
-        r_ext = L.getpd(0, 0, float(th[w][8]), setlog=False)
-        L.set_precise(0)
-        L.initpd(0, th[w])
-        r_dbl = L.getpd(0, 0, float(th[w][8]), setlog=False)
-        esplit = r_ext["split"]
-        assert abs(gsplit - esplit) == 1, (w, gsplit, esplit)
-        pe, pdb = r_ext["pofd_raw"], r_dbl["pofd_raw"]
-        noise = np.max(np.abs(pdb[esplit - 64:esplit + 64] -
-                              pe[esplit - 64:esplit + 64]))
-        assert abs(pe[gsplit] - pe[esplit]) < noise, (w, pe[gsplit],
-                                                      pe[esplit], noise)
-        L.set_precise(1)
-        L.set_split_shift(gsplit - esplit)
-        at_dev, st_a = L.loglike_batch(th[w:w + 1], 1)
-        L.set_split_shift(0)
-        assert st_a[0] == 0
-        assert abs(ll[w] - at_dev[0]) <= 1e-6, (w, ll[w] - at_dev[0])
+    off = check_exact_parity(ctx, L, th, ll)
+    _say("walkers at an ambiguous split: %s" % off.tolist())

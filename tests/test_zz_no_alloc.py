@@ -75,7 +75,7 @@ def test_noalloc_two_beamsets_unbinned():
         b, sb = _run(ctx, th, noalloc=False)
         assert np.array_equal(sa, sb) and np.all(sa == 0)
         assert np.array_equal(a, b)
-    ctx.set_parts(3)
+    ctx.set_parts(2)
     th = synth.walker_thetas(16, seed=8)
     a, _ = _run(ctx, th, noalloc=True)
     b, _ = _run(ctx, th, noalloc=False)
