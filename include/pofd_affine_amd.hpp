@@ -73,6 +73,15 @@ inline std::ostream& operator<<(std::ostream& os, const affineExcept& ex) {
   return os << ex.what();
 }
 
+/* HDF5 output (writeToHDF5 / writeToHDF5Handle) where the HDF5 header is on
+   the include path, unless POFD_AFFINE_NO_HDF5 is defined */
+#if !defined(POFD_AFFINE_NO_HDF5) && defined(__has_include)
+#if __has_include(<hdf5.h>)
+#define POFD_AFFINE_HAVE_HDF5 1
+#include "pofd_affine_amd_hdf5.hpp"
+#endif
+#endif
+
 namespace pofd_amd_detail {
 inline int default_device() { return 0; }
 inline double logfac() { return log2(10.0); }  /* global_settings.h:69 */
@@ -410,6 +419,25 @@ class PD {
   }
   /* PD::getLogLike, include/PD.h:86, src/PD.cc:538-677 (on the device) */
   double getLogLike(const fitsData& data) const;
+#ifdef POFD_AFFINE_HAVE_HDF5
+  /* PD::writeToHDF5Handle / writeToHDF5, src/PD.cc:489-526: IsLog, dFlux
+     and MinFlux attributes, Flux and PD datasets */
+  void writeToHDF5Handle(hid_t objid) const {
+    pofd_hdf5::need(objid >= 0 && H5Iget_ref(objid) >= 0, "PD", "writeToHDF5Handle",
+                    "Output handle not open");
+    pofd_hdf5::writeAttBool(objid, "IsLog", logflat);
+    pofd_hdf5::writeAttDouble(objid, "dFlux", dflux);
+    pofd_hdf5::writeAttDouble(objid, "MinFlux", minflux);
+    std::vector<double> flux(getDim());
+    for (unsigned i = 0; i < getDim(); ++i) flux[i] = static_cast<double>(i) * dflux + minflux;
+    pofd_hdf5::writeDataDoubles(objid, "Flux", flux);
+    pofd_hdf5::writeDataDoubles(objid, "PD", store);
+  }
+  void writeToHDF5(const std::string& outputfile) const {
+    pofd_hdf5::withNewFile(outputfile, "PD", "writeToHDF5",
+                           [&](hid_t f) { writeToHDF5Handle(f); });
+  }
+#endif
 };
 
 namespace pofd_amd_detail {
@@ -568,6 +596,7 @@ class calcLike {
   mutable numberCountsKnotsSpline model;
   std::vector<fitsData> datasets;
   std::vector<std::pair<beam, std::vector<size_t>>> sets;  // beam, dataset idx
+  std::vector<std::string> set_beamfiles;                   // per beam set
   std::vector<double> sigmas, like_norms;
   double exp_conf = 0.0;
   bool has_sigma_prior = false;
@@ -660,6 +689,7 @@ class calcLike {
     for (size_t i = 0; i < ndat; ++i) groups[beamfiles[i]].push_back(i);
     datasets.clear();
     sets.clear();
+    set_beamfiles.clear();
     sigmas.clear();
     like_norms.clear();
     for (auto& g : groups) {
@@ -677,6 +707,7 @@ class calcLike {
         idx.push_back(datasets.size() - 1);
       }
       sets.emplace_back(bm, idx);
+      set_beamfiles.push_back(g.first);
     }
     exp_conf = EXPCONF;
     prepared = false;
@@ -798,6 +829,72 @@ class calcLike {
                              cfirb_prior_sigma, poisson_prior_mean, poisson_prior_sigma,
                              regularization_alpha);
   }
+
+#ifdef POFD_AFFINE_HAVE_HDF5
+  /* calcLike::writeToHDF5Handle, src/calcLike.cc:986-1096: the transform
+     and prior settings as attributes, the knot positions (and the last
+     parameters' log10 knot values) in a Model group, and per beam set a
+     BeamSets/BeamSet<i> group (calcLikeSingle::writeToHDF5Handle,
+     :449-509) with NFiles, Filenames, BeamFilename and the data and R flux
+     ranges (NaN before setRRanges).  The data and mask FITS extensions the
+     reference also records are not kept here. */
+  void writeToHDF5Handle(hid_t objid) const {
+    using namespace pofd_hdf5;
+    need(objid >= 0 && H5Iget_ref(objid) >= 0, "calcLike", "writeToHDF5",
+         "Input handle is not valid");
+    writeAttUnsignedInt(objid, "FFTSize", fftsize);
+    writeAttUnsignedInt(objid, "NRInterp", ninterp);
+    writeAttUnsignedInt(objid, "NBeamSets", getNBeamSets());
+    writeAttBool(objid, "WasDataBinned", bin_data);
+    if (bin_data) writeAttUnsignedInt(objid, "NDataBins", nbins);
+    writeAttBool(objid, "HasCfirbPrior", has_cfirb_prior);
+    if (has_cfirb_prior) {
+      writeAttDouble(objid, "CfirbPriorMean", cfirb_prior_mean);
+      writeAttDouble(objid, "CfirbPriorSigma", cfirb_prior_sigma);
+    }
+    writeAttBool(objid, "HasPoissonPrior", has_poisson_prior);
+    if (has_poisson_prior) {
+      writeAttDouble(objid, "PoissonPriorMean", poisson_prior_mean);
+      writeAttDouble(objid, "PoissonPriorSigma", poisson_prior_sigma);
+    }
+    writeAttBool(objid, "HasSigmaPrior", has_sigma_prior);
+    if (has_sigma_prior) writeAttDouble(objid, "SigmaPriorWidth", sigma_prior_width);
+    writeAttBool(objid, "DidRegularize", regularization_alpha > 0.0);
+    if (regularization_alpha > 0.0)
+      writeAttDouble(objid, "RegularizationAlpha", regularization_alpha);
+    {
+      Id g(createGroup(objid, "Model", "calcLike", "writeToHDF5Handle"), H5Gclose);
+      writeAttUnsignedInt(g.get(), "NKnots", model.getNKnots());
+      if (model.getNKnots() > 0) {
+        writeDataDoubles(g.get(), "KnotPositions", model.positions());
+        const std::vector<float>& kv = model.knotValues();
+        if (kv.size() >= model.getNKnots())
+          writeDataDoubles(g.get(), "Log10KnotValues",
+                           std::vector<double>(kv.begin(), kv.begin() + model.getNKnots()));
+      }
+    }
+    Id bsg(createGroup(objid, "BeamSets", "calcLike", "writeToHDF5Handle"), H5Gclose);
+    for (size_t s = 0; s < sets.size(); ++s) {
+      Id g(createGroup(bsg.get(), "BeamSet" + std::to_string(s), "calcLikeSingle",
+                       "writeToNewHDF5Group"),
+           H5Gclose);
+      const std::vector<size_t>& idx = sets[s].second;
+      writeAttUnsignedInt(g.get(), "NFiles", static_cast<unsigned>(idx.size()));
+      std::vector<std::string> names;
+      for (size_t i : idx) names.push_back(datasets[i].getFile());
+      if (!names.empty() && !names[0].empty()) writeDataStrings(g.get(), "Filenames", names);
+      if (s < set_beamfiles.size() && !set_beamfiles[s].empty())
+        writeDataStrings(g.get(), "BeamFilename", {set_beamfiles[s]});
+      double info[8];
+      for (double& v : info) v = NAN;
+      if (prepared) (void)pofd1d_beamset_info(ctx.get(), static_cast<int>(s), info);
+      writeDataDouble(g.get(), "MinDataFlux", info[3]);
+      writeDataDouble(g.get(), "MaxDataFlux", info[4]);
+      writeDataDouble(g.get(), "MinRFlux", info[0]);
+      writeDataDouble(g.get(), "MaxRFlux", info[1]);
+    }
+  }
+#endif
 
   /* the device context (after setRRanges), for the sampler: its batches
      include the priors set here */
@@ -1175,6 +1272,31 @@ class PDDouble {
     if (rc) pofd_amd_detail::throw_status(c, rc, "PDDouble", "getLogLike");
     return out;
   }
+#ifdef POFD_AFFINE_HAVE_HDF5
+  /* PDDouble::writeToHDF5Handle / writeToHDF5, src/PDDouble.cc:912-966:
+     IsLog, dFlux1, dFlux2, MinFlux1 and MinFlux2 attributes, Flux1, Flux2
+     and the n1 x n2 PD datasets */
+  void writeToHDF5Handle(hid_t objid) const {
+    pofd_hdf5::need(objid >= 0 && H5Iget_ref(objid) >= 0, "PDDouble", "writeToHDF5Handle",
+                    "Failed to open HDF5 file to write");
+    pofd_hdf5::writeAttBool(objid, "IsLog", logflat);
+    pofd_hdf5::writeAttDouble(objid, "dFlux1", dflux1);
+    pofd_hdf5::writeAttDouble(objid, "dFlux2", dflux2);
+    pofd_hdf5::writeAttDouble(objid, "MinFlux1", minflux1);
+    pofd_hdf5::writeAttDouble(objid, "MinFlux2", minflux2);
+    std::vector<double> f1(n1), f2(n2);
+    for (unsigned i = 0; i < n1; ++i) f1[i] = static_cast<double>(i) * dflux1 + minflux1;
+    for (unsigned j = 0; j < n2; ++j) f2[j] = static_cast<double>(j) * dflux2 + minflux2;
+    pofd_hdf5::writeDataDoubles(objid, "Flux1", f1);
+    pofd_hdf5::writeDataDoubles(objid, "Flux2", f2);
+    pofd_hdf5::writeData(objid, "PD", H5T_NATIVE_DOUBLE, H5T_NATIVE_DOUBLE, {n1, n2},
+                         store.data(), "writeData2DDoubles");
+  }
+  void writeToHDF5(const std::string& outputfile) const {
+    pofd_hdf5::withNewFile(outputfile, "PDDouble", "writeToHDF5",
+                           [&](hid_t f) { writeToHDF5Handle(f); });
+  }
+#endif
 };
 
 namespace pofd_amd_detail {

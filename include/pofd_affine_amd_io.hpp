@@ -94,6 +94,17 @@ struct specFile {
   std::vector<double> sigmas;
   std::vector<std::string> psffiles;
   std::vector<double> like_norm;
+#ifdef POFD_AFFINE_HAVE_HDF5
+  /* specFile::writeToHDF5Handle, src/specFile.cc:406-416 */
+  void writeToHDF5Handle(hid_t objid) const {
+    pofd_hdf5::need(objid >= 0 && H5Iget_ref(objid) >= 0, "specFile", "writeToHDF5Handle",
+                    "Input handle is not valid");
+    pofd_hdf5::writeDataStrings(objid, "DataFiles", datafiles);
+    pofd_hdf5::writeDataStrings(objid, "BeamFiles", psffiles);
+    pofd_hdf5::writeDataDoubles(objid, "InstrumentSigma", sigmas);
+    pofd_hdf5::writeDataDoubles(objid, "LikelihoodNormalization", like_norm);
+  }
+#endif
   bool bin_data;
   unsigned int nbins;
   bool mean_sub;
@@ -666,6 +677,42 @@ class initFileKnots {
     idx_ok(i, "getUpperLimit");
     return b_.uplim(i) ? b_.up[i] : std::numeric_limits<double>::quiet_NaN();
   }
+  bool hasRange() const { return b_.has_range; }
+  bool hasLowerLimits() const { return b_.has_lower; }
+  bool hasUpperLimits() const { return b_.has_upper; }
+#ifdef POFD_AFFINE_HAVE_HDF5
+  /* initFileKnots::writeToHDF5Handle, src/initFileKnots.cc:627-655 (limits
+     a knot does not have are written as NaN) */
+  void writeToHDF5Handle(hid_t objid) const {
+    using namespace pofd_hdf5;
+    need(objid >= 0 && H5Iget_ref(objid) >= 0, C, "writeToHDF5Handle",
+         "Input handle is not valid");
+    writeAttBool(objid, "HasInitParamRange", b_.has_range);
+    writeAttBool(objid, "HasParamLowerLimits", b_.has_lower);
+    writeAttBool(objid, "HasParamUpperLimits", b_.has_upper);
+    const unsigned n = getNKnots();
+    if (n == 0) return;
+    if (b_.has_range) writeDataDoubles(objid, "ParamInitRange", b_.range);
+    std::vector<bool> has(n);
+    std::vector<double> lim(n);
+    if (b_.has_lower) {
+      for (unsigned i = 0; i < n; ++i) {
+        has[i] = knotHasLowerLimit(i);
+        lim[i] = getLowerLimit(i);
+      }
+      writeDataBools(objid, "ParamHasLowerlim", has);
+      writeDataDoubles(objid, "ParamLowerlim", lim);
+    }
+    if (b_.has_upper) {
+      for (unsigned i = 0; i < n; ++i) {
+        has[i] = knotHasUpperLimit(i);
+        lim[i] = getUpperLimit(i);
+      }
+      writeDataBools(objid, "ParamHasUpperlim", has);
+      writeDataDoubles(objid, "ParamUpperlim", lim);
+    }
+  }
+#endif
   /* central values into p[0..nknots) */
   void getParams(paramSet& p) const {
     if (p.getNParams() < b_.size()) throw affineExcept(C, "getParams", "Not enough space in provided paramSet");

@@ -264,6 +264,59 @@ class affineEnsemble {
     if (n) check(pofd_ens_get_chain(ens_, 0, n, th.data(), ll.data()), "writeToFile");
     writeChainText(path, th.data(), ll.data(), n, nwalkers_, nparams_);
   }
+#ifdef POFD_AFFINE_HAVE_HDF5
+  /* affineEnsemble::writeToHDF5Handle / writeToHDF5,
+     src/affineEnsemble.cc:1583-1767: an AffineSettings group (ScaleFactor,
+     NInitSteps, MinBurn, FixedBurn, BurnMultiple, NSteps), a ParamInfo
+     group (NFixed, NIgnore, NBonus, the IsParamFixed /
+     IsParamAutocorrIgnore / IsParamBonus flags, InitialPosition and
+     RegeneratedFirstStep when known, ParamNames when any is set) and a
+     Chains group (NAccept per walker, then the chain as
+     affineChainSet::writeToHDF5Handle lays it out) */
+  virtual void writeToHDF5Handle(hid_t objid) const {
+    using namespace pofd_hdf5;
+    need(objid >= 0 && H5Iget_ref(objid) >= 0, "affineEnsemble", "writeToHDF5Handle",
+         "Input handle is not valid");
+    {
+      Id g(createGroup(objid, "AffineSettings", "affineEnsemble", "writeToHDF5Handle"),
+           H5Gclose);
+      writeAttFloat(g.get(), "ScaleFactor", scalefac_);
+      writeAttUnsignedInt(g.get(), "NInitSteps", init_steps_);
+      writeAttUnsignedInt(g.get(), "MinBurn", min_burn_);
+      writeAttBool(g.get(), "FixedBurn", fixed_burn_);
+      writeAttFloat(g.get(), "BurnMultiple", burn_multiple_);
+      writeAttUnsignedInt(g.get(), "NSteps", nsteps_);
+    }
+    {
+      Id g(createGroup(objid, "ParamInfo", "affineEnsemble", "writeToHDF5Handle"), H5Gclose);
+      writeAttUnsignedInt(g.get(), "NFixed", count(kFixed, true));
+      writeAttUnsignedInt(g.get(), "NIgnore", count(kIgnore, true));
+      writeAttUnsignedInt(g.get(), "NBonus", count(kBonus, true));
+      std::vector<bool> fl(nparams_);
+      for (unsigned i = 0; i < nparams_; ++i) fl[i] = (state_[i] & kFixed) != 0;
+      writeAttBools(g.get(), "IsParamFixed", fl);
+      for (unsigned i = 0; i < nparams_; ++i) fl[i] = (state_[i] & kIgnore) != 0;
+      writeAttBools(g.get(), "IsParamAutocorrIgnore", fl);
+      for (unsigned i = 0; i < nparams_; ++i) fl[i] = (state_[i] & kBonus) != 0;
+      writeAttBools(g.get(), "IsParamBonus", fl);
+      if (has_init_step_) writeDataFloats(g.get(), "InitialPosition", init_step_);
+      if (has_regen_step_) writeDataFloats(g.get(), "RegeneratedFirstStep", regen_step_);
+      if (std::find(has_name_.begin(), has_name_.end(), true) != has_name_.end())
+        writeAttStrings(g.get(), "ParamNames", names_);
+    }
+    Id g(createGroup(objid, "Chains", "affineEnsemble", "writeToHDF5Handle"), H5Gclose);
+    std::vector<uint32_t> na(nwalkers_, 0u);
+    if (ens_) stateCopy(nullptr, nullptr, na.data());
+    writeDataUnsignedInts(g.get(), "NAccept", std::vector<unsigned int>(na.begin(), na.end()));
+    fetchChain();
+    const unsigned n = static_cast<unsigned>(chain_ll_.size() / nwalkers_);
+    writeChainSet(g.get(), chain_th_.data(), chain_ll_.data(), n, nwalkers_, nparams_);
+  }
+  void writeToHDF5(const std::string& filename) const {
+    pofd_hdf5::withNewFile(filename, "affineEnsemble", "writeToHDF5",
+                           [&](hid_t f) { writeToHDF5Handle(f); });
+  }
+#endif
   /* the stored chain, [steps][walkers][params] and [steps][walkers] */
   void getChain(std::vector<float>& theta, std::vector<double>& loglike) const {
     fetchChain();
@@ -348,6 +401,12 @@ class affineEnsemble {
     check(pofd_ens_set_host_target(ens_, &affineEnsemble::hostLoglike, this, getNBonusParams()),
           "setHostTarget");
   }
+  /* the starting point the walkers were drawn around (recorded as
+     ParamInfo/InitialPosition, src/pofdMCMC.cc:205-206) */
+  void setInitStep(const paramSet& p) {
+    has_init_step_ = true;
+    init_step_.assign(p.data(), p.data() + p.getNParams());
+  }
   /* one position per walker; evaluates them (and their bonus parameters) */
   void setInitialPositions(const std::vector<paramSet>& pos) {
     if (pos.size() != nwalkers_)
@@ -379,6 +438,8 @@ class affineEnsemble {
   mutable std::vector<double> chain_ll_;
   mutable bool chain_valid_ = false;
   unsigned int steps_since_reset_ = 0;
+  bool has_init_step_ = false, has_regen_step_ = false;
+  std::vector<float> init_step_, regen_step_;
 
   void resizeParams() {
     state_.assign(nparams_, 0);
@@ -479,6 +540,8 @@ class affineEnsemble {
     paramSet best(nparams_);
     getMaxLogLikeParam(ll, best);
     generateInitialPosition(best);
+    has_regen_step_ = true;
+    regen_step_.assign(best.data(), best.data() + best.getNParams());
   }
   /* min_burn steps, extended until the stored length exceeds
      burn_multiple x the largest autocorrelation time (doBurnIn,
@@ -589,6 +652,7 @@ class pofdMCMC : public affineEnsemble {
     lo[nknots_] = std::numeric_limits<float>::min();
     setTarget(likeSet->context(), POFD_TARGET_1D, lo, up);
     is_init = true;
+    setInitStep(p);
     generateInitialPosition(p);
   }
 
@@ -622,6 +686,25 @@ class pofdMCMC : public affineEnsemble {
   double getLogLike(const paramSet& p, bool& rejected) override {
     return likeSet->getLogLike(p, rejected);
   }
+
+#ifdef POFD_AFFINE_HAVE_HDF5
+  /* pofdMCMC::writeToHDF5Handle, src/pofdMCMC.cc:365-393: the ensemble's
+     groups, the init file's ranges and limits added to ParamInfo, and a
+     LikelihoodParams group with the likelihood's and the spec file's
+     settings */
+  void writeToHDF5Handle(hid_t objid) const override {
+    using namespace pofd_hdf5;
+    affineEnsemble::writeToHDF5Handle(objid);
+    {
+      Id g(H5Gopen2(objid, "ParamInfo", H5P_DEFAULT), H5Gclose);
+      need(g.ok(), "pofdMCMC", "writeToHDF5Handle", "Can't open ParamInfo group");
+      ifile.writeToHDF5Handle(g.get());
+    }
+    Id g(createGroup(objid, "LikelihoodParams", "pofdMCMC", "writeToHDF5Handle"), H5Gclose);
+    if (likeSet) likeSet->writeToHDF5Handle(g.get());
+    spec_info.writeToHDF5Handle(g.get());
+  }
+#endif
 
   specFile spec_info;
   initFileKnots ifile;
@@ -713,6 +796,7 @@ class pofdMCMCDouble : public affineEnsemble {
     lo[ntot_] = lo[ntot_ + 1] = std::numeric_limits<float>::min();
     setTarget(likeSet->context(), POFD_TARGET_2D, lo, up);
     is_init = true;
+    setInitStep(p);
     generateInitialPosition(p);
   }
 

@@ -4,10 +4,11 @@
 // 0 re-evaluated through calcLike.
 //
 // usage: mcmc_check <initfile> <specfile> <nwalkers> <nsamples> <min_burn>
-//                   <chain out> [init_steps fixed_burn]
+//                   <chain out> [init_steps fixed_burn] [h5=<file>]
 // Prints "npar", "fit", "bonus", "acc <mean acceptance>", one
 // "stat <i> <mean> <var> <low> <up>" per parameter, "acor ..." and
-// "recheck <chain logL> <direct logL>".
+// "recheck <chain logL> <direct logL>"; with h5= also writes the run as
+// pofdMCMC::writeToHDF5 does.
 #include <cstdio>
 #include <iostream>
 #include <string>
@@ -16,6 +17,11 @@
 #include "../../include/pofd_affine_amd_mcmc.hpp"
 
 int main(int argc, char** argv) {
+  std::string h5;
+  if (argc > 1 && std::string(argv[argc - 1]).rfind("h5=", 0) == 0) {
+    h5 = std::string(argv[argc - 1]).substr(3);
+    --argc;
+  }
   if (argc != 7 && argc != 9) {
     std::cerr << "usage: mcmc_check init spec nwalkers nsamples min_burn chain "
                  "[init_steps fixed_burn]"
@@ -57,6 +63,14 @@ int main(int argc, char** argv) {
     std::printf("recheck %.17g %.17g\n", ll[last * nw], direct);
     std::printf("maxlike %.17g\n", mc.getMaxLogLike());
     mc.writeToFile(argv[6]);
+    if (!h5.empty()) {
+#ifdef POFD_AFFINE_HAVE_HDF5
+      mc.writeToHDF5(h5);
+#else
+      std::cerr << "built without HDF5" << std::endl;
+      return 1;
+#endif
+    }
     mc.printStatistics(0.683, std::cout);
   } catch (const affineExcept& e) {
     std::cerr << "affineExcept: " << e.what() << std::endl;

@@ -217,12 +217,13 @@ def test_cpp_pofdmcmc_from_files(tmp_path, extra):
         f.write("fftsize= 32768\nninterp= 512\nbin_data= 1000\n"
                 "minbeamval= 1e-6\nnbeamhist= 120\nfit_sigma= no\n")
     chain = tmp_path / "chain.txt"
+    h5 = tmp_path / "run.h5"
     # extra: 30 initial steps at temperature 2 with the walkers then
     # regenerated around the best point, and a fixed 40-step burn-in
     out = subprocess.run([EXE4, str(tmp_path / "init.txt"),
                           str(tmp_path / "spec.txt"), "32", "200", "40",
-                          str(chain)] + extra, capture_output=True,
-                         text=True, timeout=240)
+                          str(chain)] + extra + ["h5=%s" % h5],
+                         capture_output=True, text=True, timeout=240)
     assert out.returncode == 0, out.stderr
     v = {}
     stats = {}
@@ -250,6 +251,72 @@ def test_cpp_pofdmcmc_from_files(tmp_path, extra):
     assert np.array_equal(np.unique(rows[:, 12]), np.arange(32))
     assert np.all(np.isfinite(rows[:, 9:12]))
     assert rows[:, 2].min() >= truth[2] - 1.0                # the limit
+    _check_pofdmcmc_hdf5(h5, rows, extra, tmp_path)
+
+
+def _check_pofdmcmc_hdf5(h5, rows, extra, tmp_path):
+    """pofdMCMC::writeToHDF5 (src/pofdMCMC.cc:365-393 over
+    src/affineEnsemble.cc:1583-1767), read back through the HDF5 C API:
+    the settings, the parameter states, names and initial position, the
+    chain and likelihoods equal to the text chain's, the init file's
+    ranges and limits, and the likelihood's settings."""
+    from test_hdf5_output import h5dump
+    h = h5dump(h5)
+    assert {"/AffineSettings", "/ParamInfo", "/Chains", "/LikelihoodParams",
+            "/LikelihoodParams/Model", "/LikelihoodParams/BeamSets",
+            "/LikelihoodParams/BeamSets/BeamSet0"} <= h["groups"]
+    a = h["attrs"]
+    assert a["/AffineSettings@ScaleFactor"] == ("f4", [2.0])
+    assert a["/AffineSettings@NSteps"] == ("u4", [200])
+    assert a["/AffineSettings@MinBurn"] == ("u4", [40])
+    assert a["/AffineSettings@NInitSteps"][1][0] == (30 if extra else 0)
+    assert a["/AffineSettings@FixedBurn"] == ("u1", [1 if extra else 0])
+    assert a["/AffineSettings@BurnMultiple"][1][0] == 5.0
+    assert a["/ParamInfo@NFixed"][1][0] == 1                 # sigma
+    assert a["/ParamInfo@NBonus"][1][0] == 2
+    assert list(a["/ParamInfo@IsParamFixed"][1]) == [0] * 8 + [1, 0, 0]
+    assert list(a["/ParamInfo@IsParamBonus"][1]) == [0] * 9 + [1, 1]
+    assert a["/ParamInfo@ParamNames"][1] == (
+        ["Knot%d" % i for i in range(8)] + ["SigmaMult", "<S>area",
+                                            "<S^2>area"])
+    assert a["/ParamInfo@HasInitParamRange"][1][0] == 1
+    assert a["/ParamInfo@HasParamLowerLimits"][1][0] == 1
+    init = h["data"]["/ParamInfo/InitialPosition"][2]
+    truth = synth.true_theta()[:8]
+    np.testing.assert_allclose(init[:8], truth, rtol=1e-6)
+    assert init[8] == 1.0
+    assert ("/ParamInfo/RegeneratedFirstStep" in h["data"]) == bool(extra)
+    lo = h["data"]["/ParamInfo/ParamHasLowerlim"][2]
+    assert list(lo) == [0, 0, 1, 0, 0, 0, 0, 0]
+    kind, dims, ch = h["data"]["/Chains/Chain"]
+    assert kind == "f4" and dims == (32, 200, 11)
+    assert a["/Chains/Chain@NWalkers"][1][0] == 32
+    ch = ch.reshape(32 * 200, 11)
+    np.testing.assert_allclose(ch, rows[:, :11], rtol=1e-5, atol=1e-30)
+    kind, dims, ll = h["data"]["/Chains/Likelihood"]
+    assert dims == (32, 200)
+    np.testing.assert_allclose(ll, rows[:, 11], rtol=1e-7)
+    nacc = h["data"]["/Chains/NAccept"][2]
+    assert nacc.shape == (32,) and np.all(nacc > 0) and np.all(nacc <= 200)
+    lp = "/LikelihoodParams"
+    assert a[lp + "@FFTSize"][1][0] == 32768
+    assert a[lp + "@NRInterp"][1][0] == 512
+    assert a[lp + "@NBeamSets"][1][0] == 1
+    assert a[lp + "@WasDataBinned"][1][0] == 1
+    assert a[lp + "@NDataBins"][1][0] == 1000
+    assert a[lp + "@HasSigmaPrior"][1][0] == 0
+    kp = h["data"][lp + "/Model/KnotPositions"][2]
+    np.testing.assert_allclose(kp, synth.knot_positions(), rtol=1e-9)
+    bs = lp + "/BeamSets/BeamSet0"
+    assert a[bs + "@NFiles"][1][0] == 1
+    assert h["data"][bs + "/Filenames"][2] == [str(tmp_path / "map.fits")]
+    assert h["data"][bs + "/BeamFilename"][2] == [str(tmp_path / "beam.fits")]
+    rmin, rmax = (h["data"][bs + "/MinRFlux"][2][0],
+                  h["data"][bs + "/MaxRFlux"][2][0])
+    assert np.isfinite(rmin) and rmax > rmin
+    assert h["data"][lp + "/DataFiles"][2] == [str(tmp_path / "map.fits")]
+    np.testing.assert_array_equal(h["data"][lp + "/InstrumentSigma"][2],
+                                  [0.0025])
 
 
 EXE5 = os.path.join(ROOT, "tests", "cpp", "_build", "mcmc2d_check")
