@@ -60,6 +60,9 @@ __device__ __forceinline__ int cpos(int col, int e) {
   return col * kSlots + (slot(e) ^ (col >> 2));
 }
 
+// accumulator of v_mfma_f64_16x16x4_f64 (four doubles per lane)
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -367,11 +370,12 @@ __device__ void load_twiddles(const BeamsetDev& bs, XformSmem& sm) {
   for (int i = threadIdx.x; i < bs.M2; i += blockDim.x) sm.wm2[i] = bs.Wm2[i];
 }
 
-// 2^s * dflux by Tang's table-driven method (ACM TOMS 15:144, 1989):
+// 2^s * scale by Tang's table-driven method (ACM TOMS 15:144, 1989):
 // 32 s = 32 n + j + 32 r with |r| <= 1/64, 2^s = 2^n 2^(j/32) 2^r, the table
-// entries tab[j] = 2^(j/32) * dflux and 2^r = exp(r ln 2) as its Taylor
-// polynomial of degree 6 (truncation below 4e-18 relative).  Accurate to
-// about one ulp; 0 below 2^-1075 and +inf above 2^1024 as exp2 gives them.
+// entries tab[j] = 2^(j/32) * scale (the fill's scale is dflux) and
+// 2^r = exp(r ln 2) as its Taylor polynomial of degree 6 (truncation below
+// 4e-18 relative).  Accurate to about one ulp; 0 below 2^-1075 and +inf
+// above 2^1024 as exp2 gives them.
 __device__ __forceinline__ double exp2_dflux(double s, const double* tab) {
   const double t = fmin(fmax(s * 32.0, -35000.0), 33000.0);
   const double n = rint(t);
@@ -768,15 +772,22 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
 
   for (int p0 = 0; p0 < nrows; p0 += kRowsPerPass) {
     const int np = min(kRowsPerPass, nrows - p0);
-    // kept row r of this pass accumulates in lane q = r % 4 of the k1 quad
-    // (slot r / 4), so only two accumulators per thread stay live through
-    // the fill and the column transforms
-    double2 acc[kRowsPerPass / 4];
-#pragma unroll
-    for (int s = 0; s < kRowsPerPass / 4; ++s) acc[s] = make_double2(0.0, 0.0);
+    // The pass's kept rows accumulate on the FP64 matrix cores:
+    // [Re Z | Im Z] (256 x 16) += [Re Yt | Im Yt] (256 x 32) x
+    // [[Re V, Im V], [-Im V, Re V]] (32 x 16) per chunk, Yt the twiddled
+    // column transforms and V[n2][r] = W_M2^(n2 k2_r).  Wave wv takes rows
+    // k1 in [16 wv, 16 wv + 16) in v_mfma_f64_16x16x4_f64 steps (lane l
+    // holds A[l & 15][l >> 4], B[l >> 4][l & 15], D[i][l & 15] for
+    // i = (l >> 4) + 4 rr in register rr): step s takes the real (s < 4) or
+    // imaginary parts of Yt at columns 4 (s & 3) + (l >> 4); output column
+    // j is Re (j < 8) or Im (j >= 8) of kept row p0 + (j & 7).
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int lj = tid & 15, lk = (tid >> 4) & 3, k1w = 16 * (tid >> 6);
+    const int lr = lj & 7;  // this lane's output row (of the pass)
     // closes, in order, every group whose chunks end at or before chunk
-    // cend: its row sums (this thread's rows) and, in the first pass, its
-    // moments enter the group tree
+    // cend: its row sums and, in the first pass, its moments enter the
+    // group tree; lanes j < 8 store (Re, Im) of row lr at rows k1 of
+    // their registers, the imaginary parts from lane j + 8
     int gnext = g_lo;
     auto close_groups = [&](const int cend) {
       while (gnext < g_hi && group_chunk(gnext + 1, nchunks) <= cend) {
@@ -784,13 +795,14 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
         double2* zg = (g >= 2) ? zhi : zdst;
         const bool add = (g & 1) && g > g_lo;  // G_(g-1) is parked there
 #pragma unroll
-        for (int s = 0; s < kRowsPerPass / 4; ++s) {
-          const int r = 4 * s + q;
-          if (r < np) {
-            double2* z = zg + (p0 + r) * kM1 + row4;
-            *z = add ? cadd(*z, acc[s]) : acc[s];
+        for (int rr = 0; rr < 4; ++rr) {
+          const double im = __shfl_xor(acc[rr], 8);
+          if (lj < 8 && lr < np) {
+            double2* z = zg + (p0 + lr) * kM1 + k1w + lk + 4 * rr;
+            const double2 v = make_double2(acc[rr], im);
+            *z = add ? cadd(*z, v) : v;
           }
-          acc[s] = make_double2(0.0, 0.0);
+          acc[rr] = 0.0;
         }
         if (p0 == 0) {
           double a = mn_part, b = var_part;
@@ -868,42 +880,27 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       }
     };
     auto accum_chunk = [&](const int c, const double2* src) {
-      // twiddle and accumulate the kept rows of chunk c (transformed in
-      // src): thread (k1 = row4, q) holds
-      // columns 4q..4q+3; each row's sum over the chunk's 16 columns is
-      // combined across the quad (the same value in all four lanes)
-      {
-        const int k1 = row4;
-        double2 a[4];
+      // Yt at rows k1w + lj, columns 4 m + lk of the chunk, and V of those
+      // columns at this lane's output row lr (zero past the pass's rows)
+      const int k2 = (lr < np) ? sm.rowk2[p0 + lr] : 0;
+      double2 y[4], v[4];
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-          const int n2 = kChunk * c + 4 * q + cc;
-          const int t = k1 * n2;
-          a[cc] = cmul(src[cpos(4 * q + cc, k1)],
-                       cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
-        }
-#pragma unroll
-        for (int r = 0; r < kRowsPerPass; ++r)
-          if (r < np) {
-            const int k2 = sm.rowk2[p0 + r];
-            double2 v = make_double2(0.0, 0.0);
-            if (p0 + r == 0) {  // row k2 = 0: the twiddle is 1
-#pragma unroll
-              for (int cc = 0; cc < 4; ++cc) v = cadd(v, a[cc]);
-            } else {
-#pragma unroll
-              for (int cc = 0; cc < 4; ++cc) {
-                const int n2 = kChunk * c + 4 * q + cc;
-                v = cadd(v, cmul(a[cc], sm.wm2[(n2 * k2) & (M2 - 1)]));
-              }
-            }
-            v.x += __shfl_xor(v.x, 1);
-            v.y += __shfl_xor(v.y, 1);
-            v.x += __shfl_xor(v.x, 2);
-            v.y += __shfl_xor(v.y, 2);
-            if ((r & 3) == q) acc[r >> 2] = cadd(acc[r >> 2], v);
-          }
+      for (int m = 0; m < 4; ++m) {
+        const int n2l = 4 * m + lk;
+        const int n2 = kChunk * c + n2l;
+        const int t = (k1w + lj) * n2;
+        y[m] = cmul(src[cpos(n2l, k1w + lj)], cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
+        v[m] = (lr < np) ? sm.wm2[(n2 * k2) & (M2 - 1)] : make_double2(0.0, 0.0);
       }
+      const bool im_col = lj >= 8;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)  // real parts of Yt
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(y[m].x, im_col ? v[m].y : v[m].x, acc, 0,
+                                                   0, 0);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)  // imaginary parts of Yt
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(y[m].y, im_col ? v[m].x : -v[m].y, acc, 0,
+                                                   0, 0);
     };
     // the two buffers alternate by chunk: a chunk's column transforms end
     // in the buffer it was filled into, so its row accumulation reads that
@@ -925,14 +922,11 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     }
     // one part: (G0 + G1) + (G2 + G3); more parts leave their pair or
     // group in their slot
-    if (parts == 1) {
+    if (parts == 1 && lj < 8 && lr < np) {
 #pragma unroll
-      for (int s = 0; s < kRowsPerPass / 4; ++s) {
-        const int r = 4 * s + q;
-        if (r < np) {
-          const int e = (p0 + r) * kM1 + row4;
-          zdst[e] = cadd(zdst[e], zhi[e]);
-        }
+      for (int rr = 0; rr < 4; ++rr) {
+        const int e = (p0 + lr) * kM1 + k1w + lk + 4 * rr;
+        zdst[e] = cadd(zdst[e], zhi[e]);
       }
     }
   }
@@ -1261,36 +1255,54 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
     }
   };
   close_groups(c_lo);
-  const bool small_rows = nrows <= kRowsPerPass;
-  double2 zr[kRowsPerPass];
-#pragma unroll
-  for (int r = 0; r < kRowsPerPass; ++r)
-    zr[r] = (small_rows && r < nrows) ? zp[r * kM1 + row4] : make_double2(0.0, 0.0);
+  // The pruned transform across columns, B[k1][n2] = sum_r Z[k1][r]
+  // conj(W_M2^(n2 k2_r)) for the chunk's 16 columns, as a real matrix
+  // product on the FP64 matrix cores: [Re B | Im B] (256 x 32) =
+  // [Re Z | Im Z] (256 x 2R) x [[Re V, -Im V], [Im V, Re V]] (2R x 32),
+  // V[n2][r] = W_M2^(n2 k2_r), kept rows in blocks of R = 8.  Wave wv takes
+  // rows k1 in [16 wv, 16 wv + 16) in v_mfma_f64_16x16x4_f64 steps (lane l
+  // holds A[l & 15][l >> 4], B[l >> 4][l & 15], and D[i][l & 15] for
+  // i = (l >> 4) + 4 rr in register rr): the four k slots of step s are the
+  // real (s < 2) or imaginary parts of Z at rows r0 + (l >> 4) + 4 (s & 1).
+  const int lj = tid & 15, lk = (tid >> 4) & 3, k1w = 16 * (tid >> 6);
+  const bool one_block = nrows <= 8;
+  double2 za0 = make_double2(0.0, 0.0), za1 = make_double2(0.0, 0.0);
+  if (one_block) {  // the block's Z operands stay in registers
+    if (lk < nrows) za0 = zp[lk * kM1 + k1w + lj];
+    if (lk + 4 < nrows) za1 = zp[(lk + 4) * kM1 + k1w + lj];
+  }
   for (int c = c_lo; c < c_hi; ++c) {
     {
-      const int k1 = row4;
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc) {
-        const int n2 = kChunk * c + 4 * q + cc;
-        double2 b = make_double2(0.0, 0.0);
-        if (small_rows) {
-#pragma unroll
-          for (int r = 0; r < kRowsPerPass; ++r)
-            if (r == 0) {  // row k2 = 0: the twiddle is 1
-              b = cadd(b, zr[0]);
-            } else if (r < nrows) {
-              const double2 w = sm.wm2[(n2 * sm.rowk2[r]) & (M2 - 1)];
-              b = cadd(b, cmulc(zr[r], w));
-            }
-        } else {
-          for (int r = 0; r < nrows; ++r) {
-            const double2 w = sm.wm2[(n2 * sm.rowk2[r]) & (M2 - 1)];
-            b = cadd(b, cmulc(zp[r * kM1 + k1], w));
-          }
+      const int n2 = kChunk * c + lj;
+      f64x4 dre = {0.0, 0.0, 0.0, 0.0}, dim = {0.0, 0.0, 0.0, 0.0};
+      for (int r0 = 0; r0 < nrows; r0 += 8) {
+        const int ra = r0 + lk, rb = r0 + lk + 4;
+        double2 z0 = za0, z1 = za1;
+        if (!one_block) {
+          z0 = (ra < nrows) ? zp[ra * kM1 + k1w + lj] : make_double2(0.0, 0.0);
+          z1 = (rb < nrows) ? zp[rb * kM1 + k1w + lj] : make_double2(0.0, 0.0);
         }
+        const double2 v0 = (ra < nrows) ? sm.wm2[(n2 * sm.rowk2[ra]) & (M2 - 1)]
+                                        : make_double2(0.0, 0.0);
+        const double2 v1 = (rb < nrows) ? sm.wm2[(n2 * sm.rowk2[rb]) & (M2 - 1)]
+                                        : make_double2(0.0, 0.0);
+        dre = __builtin_amdgcn_mfma_f64_16x16x4f64(z0.x, v0.x, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f64_16x16x4f64(z0.x, -v0.y, dim, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f64_16x16x4f64(z1.x, v1.x, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f64_16x16x4f64(z1.x, -v1.y, dim, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f64_16x16x4f64(z0.y, v0.y, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f64_16x16x4f64(z0.y, v0.x, dim, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f64_16x16x4f64(z1.y, v1.y, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f64_16x16x4f64(z1.y, v1.x, dim, 0, 0, 0);
+      }
+      // the four-step twiddle, then column n2 of the chunk into LDS
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int k1 = k1w + lk + 4 * rr;
         const int t = k1 * n2;
-        b = cmulc(b, cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
-        sm.bufa[cpos(4 * q + cc, k1)] = b;
+        const double2 b = cmulc(make_double2(dre[rr], dim[rr]),
+                                cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
+        sm.bufa[cpos(lj, k1)] = b;
       }
     }
     __syncthreads();
