@@ -832,6 +832,38 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       double xk = bs.xnode[k];
       double4 cf = *reinterpret_cast<const double4*>(coef + 4 * k);
       double rprev = 0.0;
+      // the common case: all 8 points in the positive fill range and below
+      // wrapRidx - 1, where R needs no range test and the moments take the
+      // plain terms (the same arithmetic as the general loop below)
+      const int wr = bs.wrapRidx;
+      const bool plain = j0 >= 1 && j0 >= bs.minit && j0 + 7 <= bs.maxit &&
+                         (j0 + 7 < bs.minitn || j0 > bs.maxitn) && j0 + 7 < wr - 1;
+      if (plain) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int j = j0 + e;
+          const int ke = (kw[e >> 1] >> (16 * (e & 1))) & 0xffff;
+          if (ke != k) {
+            k = ke;
+            xk = bs.xnode[k];
+            cf = *reinterpret_cast<const double4*>(coef + 4 * k);
+          }
+          const double cflux = static_cast<double>(j) * dflux;
+          const double delx = cflux - xk;
+          const double sv = cf.x + delx * (cf.y + delx * (cf.z + delx * cf.w));
+          const double rv = exp2_dflux(sv, sm.ex2);
+          if (p0 == 0) {
+            const double prod = cflux * rv;
+            mn_part += prod;
+            var_part += cflux * prod;
+          }
+          if (e & 1)
+            dst[cpos(4 * q + (e >> 1), row4)] = hz ? make_double2(0.0, 0.0)
+                                                   : make_double2(rprev, rv);
+          rprev = rv;
+        }
+        return;
+      }
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int j = j0 + e;
@@ -853,7 +885,6 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
         }
         if (p0 == 0 && j >= 1) {
           // getMeanVarFromR (src/PDFactory.cc:625-679)
-          const int wr = bs.wrapRidx;
           double prod, cfl;
           if (j <= wr) {
             cfl = static_cast<double>(j) * dflux;
