@@ -211,6 +211,15 @@ __device__ void block_argmin(double& v, int& idx, double* red) {
   idx = bi;
 }
 
+// The 32-value block statistics of the P(D): block b = j / kBlk (kBlk = 2
+// kChunk, so row n1 = b / nbc and chunk c = b % nbc of the inverse transform,
+// nbc = M2 / kChunk) is stored at c * kM1 + n1, so that the 16 rows of a
+// wavefront's epilogue store whole 128-byte lines
+static_assert(kBlk == 2 * kChunk, "a block is one row of one chunk");
+__device__ __forceinline__ int blk_slot(const int b, const int nbc) {
+  return (b % nbc) * kM1 + b / nbc;
+}
+
 // ---- 256-point transforms of 16 swizzled LDS columns ----------------------
 // Radix-4 Stockham between two buffers: 64 threads (one wavefront) per
 // column, 4 points each; after the four stages the result is in `a`.
@@ -1376,8 +1385,9 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
       bsm += __shfl_xor(bsm, 1);
       bsm += __shfl_xor(bsm, 2);
       if (q == 0) {
-        bmin[j0 / kBlk] = bm;
-        bsum[j0 / kBlk] = bsm;
+        // block j0 / kBlk = n1 * (M2 / kChunk) + c, stored at blk_slot
+        bmin[c * kM1 + n1] = bm;
+        bsum[c * kM1 + n1] = bsm;
       }
     }
     __syncthreads();
@@ -1433,6 +1443,7 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
   double* pofd = wk.pofd + wf * wk.pofd_stride;
   const double* bmin = wk.bmin + wf * wk.blk_stride;
   const double* bsum = wk.bsum + wf * wk.blk_stride;
+  const int nbc = bs.M2 / kChunk;  // P(D) blocks per transform row
   double* dbg = wk.dbg + wf * kDbg;
   if (parts > 1) {
     // the inverse transform's parts: peak and argmin in part (= index)
@@ -1580,7 +1591,7 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
       for (int top = bh - 1; top >= bl + 1 && found < 0; top -= kThreads) {
         const int b = top - tid;
         int cb = -1;
-        if (b >= bl + 1 && bmin[b] < jitter) cb = b;
+        if (b >= bl + 1 && bmin[blk_slot(b, nbc)] < jitter) cb = b;
         const int bstar = block_max_int(cb, red);
         if (bstar >= 0) {
           int cand = -1;
@@ -1619,7 +1630,7 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
     // sum of the P(D) at indices >= split: whole blocks + the partial one
     const int bs0 = split / kBlk;
     double part = 0.0;
-    for (int b = bs0 + 1 + tid; b < N / kBlk; b += kThreads) part += bsum[b];
+    for (int b = bs0 + 1 + tid; b < N / kBlk; b += kThreads) part += bsum[blk_slot(b, nbc)];
     if (tid < kBlk) {
       const int i = bs0 * kBlk + tid;
       if (i >= split && i < N) part += pofd[i];
