@@ -1349,11 +1349,14 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
     fft256_cols<+1>(sm.bufa, sm.bufb, sm.wst);
     {
       const int n1 = row4;
-      const int j0 = 2 * M2 * n1 + 2 * kChunk * c + 8 * q;
+      // lane q of the row's quad takes columns 4 cc + q, so that each of
+      // its four 16-byte stores and the quad's other lanes' fill 64
+      // contiguous bytes of the row
+      const int jrow = 2 * M2 * n1 + 2 * kChunk * c;
       double v8[8];
 #pragma unroll
       for (int cc = 0; cc < 4; ++cc) {
-        const double2 z = sm.bufa[cpos(4 * q + cc, n1)];
+        const double2 z = sm.bufa[cpos(4 * cc + q, n1)];
         v8[2 * cc] = z.x;
         v8[2 * cc + 1] = z.y;
       }
@@ -1363,7 +1366,7 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
         double v = v8[e];
         if (v < 0) v = 0.0;  // unwrapAndNormalizePD positivity
         v8[e] = v;
-        const int j = j0 + e;
+        const int j = jrow + 8 * (e >> 1) + 2 * q + (e & 1);
         pk = (v > pk) ? v : pk;
         if (j >= botidx && j <= topidx && (mni < 0 || v < mnv)) {
           mnv = v;
@@ -1374,9 +1377,10 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
         bm = (e == 0 || v < bm) ? v : bm;
         bsm += v;
       }
-      double4* dst = reinterpret_cast<double4*>(pofd + j0);
-      dst[0] = make_double4(v8[0], v8[1], v8[2], v8[3]);
-      dst[1] = make_double4(v8[4], v8[5], v8[6], v8[7]);
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc)
+        *reinterpret_cast<double2*>(pofd + jrow + 8 * cc + 2 * q) =
+            make_double2(v8[2 * cc], v8[2 * cc + 1]);
       // 32-value block statistics over the 4 lanes q = 0..3
       double o = __shfl_xor(bm, 1);
       bm = (o < bm) ? o : bm;
@@ -1385,7 +1389,7 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
       bsm += __shfl_xor(bsm, 1);
       bsm += __shfl_xor(bsm, 2);
       if (q == 0) {
-        // block j0 / kBlk = n1 * (M2 / kChunk) + c, stored at blk_slot
+        // block jrow / kBlk = n1 * (M2 / kChunk) + c, stored at blk_slot
         bmin[c * kM1 + n1] = bm;
         bsum[c * kM1 + n1] = bsm;
       }
