@@ -405,14 +405,96 @@ __device__ __forceinline__ double exp2_dflux(double s, const double* tab) {
 // ===========================================================================
 // prep: knot spline, R at the nodes, node spline, interval coefficients
 // ===========================================================================
+// The natural spline over the nodes of segment seg from log2 R in sm.yn and
+// the segment's theta-independent factors in sm.lg / sm.la / sm.gi, into the
+// walker's per-interval coefficients.  The tridiagonal matrix is fixed
+// (LDL^T factors from the host); forward and backward substitution run as
+// truncated recurrences: the multipliers are < 1/2, so starting kWarm = 64
+// steps early changes no significant bit.  Called by every thread.
+__device__ void node_spline(const BeamsetDev& bs, const WorkDev& wk, const int walker,
+                            const int seg, PrepSmem& sm) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x;
+  const int ninterp = bs.ninterp, sys = ninterp - 2;
+  const double* xnode = bs.xnode + seg * ninterp;
+  for (int i = tid; i < sys; i += kThreads)
+    sm.gR[i] = 3.0 * ((sm.yn[i + 2] - sm.yn[i + 1]) * sm.gi[i + 1] -
+                      (sm.yn[i + 1] - sm.yn[i]) * sm.gi[i]);
+  __syncthreads();
+  const int per = (sys + kThreads - 1) / kThreads;
+  const int lo = tid * per, hi = min(lo + per, sys);
+  if (lo < hi) {
+    int i0 = lo - kWarm;
+    double z;
+    if (i0 <= 0) {
+      i0 = 0;
+      z = sm.gR[0];
+    } else {
+      z = sm.gR[i0];
+    }
+    if (i0 >= lo) sm.zz[i0] = z;
+    for (int i = i0 + 1; i < hi; ++i) {
+      z = sm.gR[i] - sm.lg[i - 1] * z;
+      if (i >= lo) sm.zz[i] = z;
+    }
+  }
+  __syncthreads();
+  for (int i = lo; i < hi; ++i) sm.zz[i] = sm.zz[i] / sm.la[i];
+  __syncthreads();
+  if (lo < hi) {
+    int i1 = hi - 1 + kWarm;
+    if (i1 > sys - 1) i1 = sys - 1;
+    double x = sm.zz[i1];
+    if (i1 < hi) sm.xs[i1] = x;
+    for (int i = i1 - 1; i >= lo; --i) {
+      x = sm.zz[i] - sm.lg[i] * x;
+      if (i < hi) sm.xs[i] = x;
+    }
+  }
+  __syncthreads();
+  // per-interval coefficients (GSL coeff_calc), node c = xs shifted
+  double* coef = wk.coef + walker * wk.coef_stride + 4 * seg * ninterp;
+  for (int k = tid; k < ninterp - 1; k += kThreads) {
+    const double c_i = (k == 0) ? 0.0 : sm.xs[k - 1];
+    const double c_ip1 = (k + 1 == ninterp - 1) ? 0.0 : sm.xs[k];
+    const double dx = xnode[k + 1] - xnode[k];
+    const double dy = sm.yn[k + 1] - sm.yn[k];
+    double* cf = coef + 4 * k;
+    cf[0] = sm.yn[k];
+    cf[1] = (dy / dx) - dx * (c_ip1 + 2.0 * c_i) / 3.0;
+    cf[2] = c_i;
+    cf[3] = (c_ip1 - c_i) / (3.0 * dx);
+  }
+  __syncthreads();  // LDS reused by the next segment
+}
+
+// the segment's theta-independent node-spline factors into LDS
+__device__ void load_node_factors(const BeamsetDev& bs, const int seg, PrepSmem& sm) {
+  const int ninterp = bs.ninterp, sys = ninterp - 2;
+  for (int i = threadIdx.x; i < sys; i += kThreads) {
+    sm.lg[i] = bs.lgamma[seg * sys + i];
+    sm.la[i] = bs.lalpha[seg * sys + i];
+  }
+  for (int i = threadIdx.x; i < ninterp - 1; i += kThreads)
+    sm.gi[i] = bs.ginv[seg * (ninterp - 1) + i];
+}
+
+// A small batch spreads each walker over `parts` workgroups (the split
+// transforms' count): part p takes the nodes [p n / parts, (p + 1) n /
+// parts) of each segment and writes their log2 R to the workspace;
+// prepspline1d_kernel then makes the node splines.  With one part the
+// workgroup does both.
 __global__ __launch_bounds__(kThreads) void prep1d_kernel(
     const BeamsetDev bs, const WorkDev wk, const float* __restrict__ theta,
     const uint32_t stride, const int32_t* __restrict__ prev_status,
     const int accumulate) {
   __shared__ PrepSmem sm;
   const int tid = threadIdx.x;
-  const int walker = blockIdx.x;
-  if (walker >= batch_n(wk)) return;
+  const int n = batch_n(wk);
+  const int parts = batch_parts(n, wk.force_parts, wk.nslots, wk.parts_tab);
+  const int walker = blockIdx.x / parts;
+  const int part = blockIdx.x - walker * parts;
+  if (walker >= n) return;
   const int ninterp = bs.ninterp, nk = bs.nknots;
   double* st = walker_state(wk, walker);
   const unsigned long long t0 = wall_clock64();
@@ -436,8 +518,10 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
       else if (!valid)
         status = POFD_E_INVALID_MODEL;  // PDFactory.cc:1046-1047
     }
-    st[ST_STATUS] = status;
-    st[ST_SIGMULT] = sigmult;
+    if (part == 0) {
+      st[ST_STATUS] = status;
+      st[ST_SIGMULT] = sigmult;
+    }
     sm.isc[0] = status;
     sm.isc[1] = valid ? 1 : 0;
     if (valid) {
@@ -491,7 +575,7 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
     }
   }
   __syncthreads();
-  if (bs.bonus) {
+  if (bs.bonus && part == 0) {
     // getNS / getFluxPerArea / getFluxSqPerArea (numberCountsKnotsSpline::
     // splineInt, src/numberCountsKnotsSpline.cc:300-343): int S^a n(S) dS
     // over the knot range, a = 0, 1, 2, as Gauss-Legendre sums in
@@ -530,15 +614,9 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
   if (sm.isc[0] != POFD_OK) return;  // uniform across the workgroup
 
   // one pass per node segment: the positive beam, then the negative beam
-  const int sys = ninterp - 2;
+  const int node_lo = part * ninterp / parts, node_hi = (part + 1) * ninterp / parts;
   for (int seg = bs.haspos ? 0 : 1; seg < bs.nseg; ++seg) {
-    const double* xnode = bs.xnode + seg * ninterp;
-    // theta-independent factors of the segment's node system into LDS
-    for (int i = tid; i < sys; i += kThreads) {
-      sm.lg[i] = bs.lgamma[seg * sys + i];
-      sm.la[i] = bs.lalpha[seg * sys + i];
-    }
-    for (int i = tid; i < ninterp - 1; i += kThreads) sm.gi[i] = bs.ginv[seg * (ninterp - 1) + i];
+    if (parts == 1) load_node_factors(bs, seg, sm);
 
     // ---- R at the interpolation nodes (initRInterp + array getR) ---------
     {
@@ -546,34 +624,43 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
       const int j0 = (seg == 0) ? 0 : bs.nbeam;
       const int j1 = (seg == 0) ? bs.nbeam : bs.nbeam + bs.nbeamn;
       double* ynode_g = wk.ynode + walker * wk.ynode_stride + seg * ninterp;
-      for (int i = tid; i < ninterp; i += kThreads) {
+      for (int i = node_lo + tid; i < node_hi; i += kThreads) {
         double workval = 0.0;
-        // beam bins in groups of 8: the loads of a group are issued
-        // together, then its terms are added in bin order
-        for (int jg = j0; jg < j1; jg += 8) {
-          int kq[8];
-          double dq[8], wq[8], eq[8];
+        // beam bins in groups of 8, the table loads of the next group
+        // issued before the current group's terms are added (in bin order)
+        int kq[8], kn8[8];
+        double dq[8], dn8[8];
+        auto load_group = [&](const int jg, int* kk, double* dd) {
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const int j = jg + u;
-            kq[u] = -1;
+            kk[u] = -1;
+            dd[u] = 0.0;
             if (j < j1) {  // uniform across the workgroup
               const size_t e = static_cast<size_t>(j) * ninterp + i;
-              kq[u] = bs.rk[e];
-              dq[u] = bs.rdelx[e];
-              wq[u] = bs.bw[j];
-              eq[u] = bs.bieta[j];
+              kk[u] = bs.rk[e];
+              dd[u] = bs.rdelx[e];
             }
           }
+        };
+        if (j0 < j1) load_group(j0, kq, dq);
+        for (int jg = j0; jg < j1; jg += 8) {
+          if (jg + 8 < j1) load_group(jg + 8, kn8, dn8);
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const int k = kq[u];
             if (k < 0) continue;
+            const int j = jg + u;
             const double dx = dq[u];
             const double s =
                 sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
             const double cts = exp2(s);
-            if (cts > 0) workval += wq[u] * cts * eq[u];
+            if (cts > 0) workval += bs.bw[j] * cts * bs.bieta[j];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            kq[u] = kn8[u];
+            dq[u] = dn8[u];
           }
         }
         const double R = bs.prefac * workval;
@@ -582,65 +669,28 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
       }
     }
     __syncthreads();
-
-    // ---- natural spline over the nodes.  The tridiagonal matrix is fixed
-    // (theta-independent LDL^T factors from the host); forward and backward
-    // substitution run as truncated recurrences: the multipliers are < 1/2,
-    // so starting kWarm = 64 steps early changes no significant bit.
-    {
-#pragma clang fp contract(off)
-      for (int i = tid; i < sys; i += kThreads)
-        sm.gR[i] = 3.0 * ((sm.yn[i + 2] - sm.yn[i + 1]) * sm.gi[i + 1] -
-                          (sm.yn[i + 1] - sm.yn[i]) * sm.gi[i]);
-      __syncthreads();
-      const int per = (sys + kThreads - 1) / kThreads;
-      const int lo = tid * per, hi = min(lo + per, sys);
-      if (lo < hi) {
-        int i0 = lo - kWarm;
-        double z;
-        if (i0 <= 0) {
-          i0 = 0;
-          z = sm.gR[0];
-        } else {
-          z = sm.gR[i0];
-        }
-        if (i0 >= lo) sm.zz[i0] = z;
-        for (int i = i0 + 1; i < hi; ++i) {
-          z = sm.gR[i] - sm.lg[i - 1] * z;
-          if (i >= lo) sm.zz[i] = z;
-        }
-      }
-      __syncthreads();
-      for (int i = lo; i < hi; ++i) sm.zz[i] = sm.zz[i] / sm.la[i];
-      __syncthreads();
-      if (lo < hi) {
-        int i1 = hi - 1 + kWarm;
-        if (i1 > sys - 1) i1 = sys - 1;
-        double x = sm.zz[i1];
-        if (i1 < hi) sm.xs[i1] = x;
-        for (int i = i1 - 1; i >= lo; --i) {
-          x = sm.zz[i] - sm.lg[i] * x;
-          if (i < hi) sm.xs[i] = x;
-        }
-      }
-      __syncthreads();
-      // per-interval coefficients (GSL coeff_calc), node c = xs shifted
-      double* coef = wk.coef + walker * wk.coef_stride + 4 * seg * ninterp;
-      for (int k = tid; k < ninterp - 1; k += kThreads) {
-        const double c_i = (k == 0) ? 0.0 : sm.xs[k - 1];
-        const double c_ip1 = (k + 1 == ninterp - 1) ? 0.0 : sm.xs[k];
-        const double dx = xnode[k + 1] - xnode[k];
-        const double dy = sm.yn[k + 1] - sm.yn[k];
-        double* cf = coef + 4 * k;
-        cf[0] = sm.yn[k];
-        cf[1] = (dy / dx) - dx * (c_ip1 + 2.0 * c_i) / 3.0;
-        cf[2] = c_i;
-        cf[3] = (c_ip1 - c_i) / (3.0 * dx);
-      }
-    }
-    __syncthreads();  // LDS reused by the next segment
+    if (parts == 1) node_spline(bs, wk, walker, seg, sm);
   }
-  if (tid == 0) st[ST_TPREP] = static_cast<double>(wall_clock64() - t0);
+  if (tid == 0 && part == 0) st[ST_TPREP] = static_cast<double>(wall_clock64() - t0);
+}
+
+// the node splines of a split batch (one workgroup per walker), from the
+// log2 R the parts wrote
+__global__ __launch_bounds__(kThreads) void prepspline1d_kernel(const BeamsetDev bs,
+                                                                 const WorkDev wk) {
+  __shared__ PrepSmem sm;
+  const int n = batch_n(wk);
+  const int walker = blockIdx.x;
+  if (walker >= n || batch_parts(n, wk.force_parts, wk.nslots, wk.parts_tab) == 1) return;
+  if (walker_state(wk, walker)[ST_STATUS] != POFD_OK) return;  // uniform
+  const int ninterp = bs.ninterp;
+  for (int seg = bs.haspos ? 0 : 1; seg < bs.nseg; ++seg) {
+    load_node_factors(bs, seg, sm);
+    const double* ynode_g = wk.ynode + walker * wk.ynode_stride + seg * ninterp;
+    for (int i = threadIdx.x; i < ninterp; i += kThreads) sm.yn[i] = ynode_g[i];
+    __syncthreads();
+    node_spline(bs, wk, walker, seg, sm);
+  }
 }
 
 // ===========================================================================
@@ -1871,8 +1921,10 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk_in, int beamset_idx
     if (ev) (void)hipEventRecord(ev[i], s);
   };
   mark(POFD_STAGE_PREP);
-  hipLaunchKernelGGL(prep1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk,
+  hipLaunchKernelGGL(prep1d_kernel, dim3(split ? nxf : nwalk), dim3(kThreads), 0, s, bs, wk,
                      d_theta, stride, d_status, accumulate);
+  if (split)
+    hipLaunchKernelGGL(prepspline1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk);
   mark(POFD_STAGE_FORWARD);
   hipLaunchKernelGGL(forward1d_kernel, dim3(nxf), dim3(kThreads), 0, s, bs, wk);
   if (split)
