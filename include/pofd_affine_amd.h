@@ -126,8 +126,10 @@ int pofd_data_mean_subtract(double *data, uint64_t n, double *mean_out);
         (src/calcLike.cc:688-758), calcLikeSingle::setSigmaBase /
         setLikeNorm / setExpConf (src/calcLike.cc:241-308) and
         calcLike::setKnotPositions / setFFTSize / setNInterp. ---- */
-/* knots in Jy (double, strictly increasing, > 0); nflux in
-   {32768, 65536, 131072, 262144}; ninterp >= 16 */
+/* knots in Jy (double, strictly increasing, > 0); nflux a power of two
+   from 2^13 to 2^18 (the reference plans FFTW for any size; the pruned
+   four-step transform here needs 256 x M2 complex points, M2 a power of
+   two from 16 to 512); ninterp in [16, 2048] */
 int pofd1d_configure(pofd_ctx *ctx, uint32_t nflux, uint32_t ninterp,
                      const double *knot_pos, uint32_t nknots,
                      double exp_conf);

@@ -682,9 +682,9 @@ extern "C" int pofd1d_configure(pofd_ctx* c, uint32_t nflux, uint32_t ninterp,
                                 const double* knot_pos, uint32_t nknots,
                                 double exp_conf) {
   if (!c) return POFD_E_ARG;
-  if (nflux != 32768 && nflux != 65536 && nflux != 131072 && nflux != 262144)
+  if (nflux < 8192 || nflux > 262144 || (nflux & (nflux - 1)) != 0)
     return fail(c, POFD_E_ARG,
-                "pofd1d_configure: nflux must be 2^15, 2^16, 2^17 or 2^18");
+                "pofd1d_configure: nflux must be a power of two from 2^13 to 2^18");
   if (ninterp < 16 || ninterp > 2048)
     return fail(c, POFD_E_ARG, "pofd1d_configure: ninterp must be in [16, 2048]");
   if (!knot_pos || nknots < 2 || nknots > static_cast<uint32_t>(pofd::kMaxKnots))

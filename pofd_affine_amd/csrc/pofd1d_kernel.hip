@@ -837,9 +837,9 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     // column transforms and V[n2][r] = W_M2^(n2 k2_r).  Wave wv takes rows
     // k1 in [16 wv, 16 wv + 16) in v_mfma_f64_16x16x4_f64 steps (lane l
     // holds A[l & 15][l >> 4], B[l >> 4][l & 15], D[i][l & 15] for
-    // i = (l >> 4) + 4 rr in register rr): step s takes the real (s < 4) or
-    // imaginary parts of Yt at columns 4 (s & 3) + (l >> 4); output column
-    // j is Re (j < 8) or Im (j >= 8) of kept row p0 + (j & 7).
+    // i = (l >> 4) + 4 rr in register rr): steps 2m and 2m + 1 take the
+    // real and the imaginary parts of Yt at columns 4 m + (l >> 4); output
+    // column j is Re (j < 8) or Im (j >= 8) of kept row p0 + (j & 7).
     f64x4 acc = {0.0, 0.0, 0.0, 0.0};
     const int lj = tid & 15, lk = (tid >> 4) & 3, k1w = 16 * (tid >> 6);
     const int lr = lj & 7;  // this lane's output row (of the pass)
@@ -972,25 +972,21 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     auto accum_chunk = [&](const int c, const double2* src) {
       // Yt at rows k1w + lj, columns 4 m + lk of the chunk, and V of those
       // columns at this lane's output row lr (zero past the pass's rows)
+      // (one column block's real and imaginary steps after each other, so
+      // that only its operands are live)
       const int k2 = (lr < np) ? sm.rowk2[p0 + lr] : 0;
-      double2 y[4], v[4];
+      const bool im_col = lj >= 8;
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int n2l = 4 * m + lk;
         const int n2 = kChunk * c + n2l;
         const int t = (k1w + lj) * n2;
-        y[m] = cmul(src[cpos(n2l, k1w + lj)], cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
-        v[m] = (lr < np) ? sm.wm2[(n2 * k2) & (M2 - 1)] : make_double2(0.0, 0.0);
+        const double2 y =
+            cmul(src[cpos(n2l, k1w + lj)], cmul(sm.wlo[t & (kM1 - 1)], sm.wm2[t >> 8]));
+        const double2 v = (lr < np) ? sm.wm2[(n2 * k2) & (M2 - 1)] : make_double2(0.0, 0.0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(y.x, im_col ? v.y : v.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(y.y, im_col ? v.x : -v.y, acc, 0, 0, 0);
       }
-      const bool im_col = lj >= 8;
-#pragma unroll
-      for (int m = 0; m < 4; ++m)  // real parts of Yt
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(y[m].x, im_col ? v[m].y : v[m].x, acc, 0,
-                                                   0, 0);
-#pragma unroll
-      for (int m = 0; m < 4; ++m)  // imaginary parts of Yt
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(y[m].y, im_col ? v[m].x : -v[m].y, acc, 0,
-                                                   0, 0);
     };
     // the two buffers alternate by chunk: a chunk's column transforms end
     // in the buffer it was filled into, so its row accumulation reads that

@@ -194,3 +194,26 @@ def test_config1d_full_size(binned):
     ctx, L, th, ll, dbl, ext = _full_size(binned, smax=synth.SMAX_SRC)
     off = check_exact_parity(ctx, L, th, ll)
     _say("walkers at an ambiguous split: %s" % off.tolist())
+
+
+@pytest.mark.parametrize("nflux", [8192, 16384])
+def test_small_transform_sizes(nflux):
+    """The smallest transforms (nflux 2^13 and 2^14: 16 and 32 columns of
+    256, a single or two column chunks) against the oracle, binned and
+    unbinned."""
+    inp = workload.make_inputs(npix_side=150, seed=13)
+    th = synth.walker_thetas(8, seed=14)
+    for binned in (True, False):
+        _say("nflux %d binned=%s" % (nflux, binned))
+        ctx = workload.build_context(inp, nflux, 256, binned=binned)
+        ll, st = ctx.loglike_batch(th)
+        L = build_oracle(inp, nflux, 256, binned, th[0])
+        ref, rst = L.loglike_batch(th, 8)
+        assert np.all(st == 0) and np.all(rst == 0), (st, rst)
+        np.testing.assert_allclose(ll, ref, rtol=0, atol=1e-6)
+        for parts in (2, 4):  # the one- and two-chunk splits
+            ctx.set_parts(parts)
+            llp, stp = ctx.loglike_batch(th)
+            ctx.set_parts(0)
+            np.testing.assert_array_equal(llp, ll)
+
