@@ -1804,19 +1804,29 @@ __global__ __launch_bounds__(kPixThreads) void pixels1d_kernel(const BeamsetDev 
     const uint64_t per = (fd.npix + kPixBlocks - 1) / kPixBlocks;
     const uint64_t p0 = per * blk;
     const uint64_t p1 = (p0 + per < fd.npix) ? p0 + per : fd.npix;
-    for (uint64_t i = p0 + threadIdx.x; i < p1; i += kPixThreads) {
-      const double cflux = fd.pix[i];
-      if (cflux <= minflux) {
-        part += Lr(0);
-      } else if (cflux >= maxfluxval) {
-        part += Lr(N - 1);
-      } else {
-        const double delt = (cflux - minflux) * idflux;
-        const int idx = static_cast<int>(delt);
-        const double t = delt - static_cast<double>(idx);
-        part += (1.0 - t) * Lr(idx) + t * Lr(idx + 1);
-      }
+    // the term of pixel flux cflux, as PD::getLogLike takes it
+    auto term = [&](const double cflux) {
+      if (cflux <= minflux) return Lr(0);
+      if (cflux >= maxfluxval) return Lr(N - 1);
+      const double delt = (cflux - minflux) * idflux;
+      const int idx = static_cast<int>(delt);
+      const double t = delt - static_cast<double>(idx);
+      return (1.0 - t) * Lr(idx) + t * Lr(idx + 1);
+    };
+    // four pixels a step: their loads and gathers are issued together,
+    // their terms added in pixel order as a one-pixel loop would
+    constexpr int kU = 4;
+    uint64_t i = p0 + threadIdx.x;
+    for (; i + (kU - 1) * kPixThreads < p1; i += kU * kPixThreads) {
+      double cf[kU], tv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) cf[u] = fd.pix[i + u * kPixThreads];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) tv[u] = term(cf[u]);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) part += tv[u];
     }
+    for (; i < p1; i += kPixThreads) part += term(fd.pix[i]);
   }
   const double s = block_sum(part, red);
   if (threadIdx.x == 0) *out = s;
