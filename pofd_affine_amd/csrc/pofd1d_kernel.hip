@@ -1733,7 +1733,15 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
       curr = 0.69314718055994530942 * block_sum(part, red);  // log2toe
     } else {
       __syncthreads();  // every load of the split search precedes the rewrite
-      for (int j = tid; j < N; j += kThreads) pofd[j] = log2_or_small(pofd[j] * itot);
+      // four values a step per thread (N is a multiple of 4 x kThreads):
+      // their loads issued together
+      for (int j = tid; j < N; j += 4 * kThreads) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = pofd[j + u * kThreads];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pofd[j + u * kThreads] = log2_or_small(v[u] * itot);
+      }
     }
   }
   if (tid == 0) {
