@@ -25,9 +25,22 @@ HBM_PEAK_GBS = 8000.0
 # written = 1.0 x WRITE_SIZE KiB); None where not profiled or for another
 # workload.
 TRAFFIC = {
-    "forward": (2.0 * 5844.8 + 9612.0) * 1024,
-    "inverse": (2.0 * 2297.5 + 333267.3) * 1024,
-    "finalize": (2.0 * 9927.7 + 64.0) * 1024,
+    "prep": (2.0 * 2098.1 + 20512.0) * 1024,
+    "forward": (2.0 * 19755.7 + 35308.2) * 1024,
+    "inverse": (2.0 * 2774.5 + 279619.0) * 1024,
+    "finalize": (2.0 * 10061.2 + 64.0) * 1024,
+}
+FP64_PEAK_TFLOPS = 78.6
+# FP64 FLOP per launch of each kernel at the default workload, from
+# rocprofv3 --pmc SQ_INSTS_VALU_{FMA,ADD,MUL}_F64 and
+# SQ_INSTS_VALU_MFMA_MOPS_F64: 64 x (2 FMA + ADD + MUL) + 512 x MOPS (a
+# 16x16x4 f64 MFMA is 2048 FLOP, counted as 4 MOPS).  Divided by the live
+# kernel time for the FP64 roofline.
+FP64_FLOP = {
+    "prep": 3.156e8,
+    "forward": 2.924e9,
+    "inverse": 1.943e9,
+    "finalize": 6.39e7,
 }
 
 
@@ -215,6 +228,20 @@ def main():
             },
         },
     }
+    if default_workload:
+        # the same live kernel times against the FP64 peak, with the FLOP
+        # per launch the SQ counters give for this workload
+        fp = {k: FP64_FLOP[k] / (stage_ms[k] * 1e-3) / 1e12
+              for k in FP64_FLOP if stage_ms.get(k, 0) > 0}
+        out["roofline"]["fp64"] = {
+            "kernel": dom + "1d_kernel", "achieved": fp[dom],
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": fp[dom] / FP64_PEAK_TFLOPS,
+            "flop_per_launch": FP64_FLOP[dom],
+            "per_kernel": {k + "1d_kernel": {"achieved": v,
+                                             "frac": v / FP64_PEAK_TFLOPS}
+                           for k, v in fp.items()},
+        }
     if args.unbinned_steps > 0:
         u = run(False, args.unbinned_steps, 2)
         out["unbinned"] = {
@@ -642,7 +669,10 @@ def bench_da(args, inp):
                       "epochs": args.da_epochs},
         "autotune": {"sizes": tune_sizes,
                      "parts": [int(v) for v in tuned],
-                     "size_rule": [min(4, max(1, 256 // n)) for n in tune_sizes],
+                     # the untuned rule: 256 // n clamped to 1..4, 3 -> 2
+                     "size_rule": [{3: 2}.get(v, v) for v in
+                                   (min(4, max(1, 256 // n))
+                                    for n in tune_sizes)],
                      "seconds": t_tune},
         "burn_and_collect_s": t_burn,
         "plain": {"wall_s": plain["wall"], "ms_per_step":
