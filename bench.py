@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--da-burn", type=int, default=500)
     ap.add_argument("--da-train-steps", type=int, default=100)
     ap.add_argument("--da-epochs", type=int, default=60)
+    ap.add_argument("--st-steps", type=int, default=8,
+                    help="surrogate steps per half-step of the surrogate-"
+                         "transition chain of the delayed-acceptance leg "
+                         "(0 or 1 skips that chain)")
     ap.add_argument("--only-da", action="store_true",
                     help="run the delayed-acceptance leg only")
     ap.add_argument("--da2d-steps", type=int, default=500,
@@ -576,7 +580,10 @@ def bench_da(args, inp):
     on effective samples per second (walkers x steps / tau of the
     walker-averaged series, per knot) and on the knot posterior medians; a
     second plain chain with another seed gives the Monte Carlo spread the
-    median shift is to be read against."""
+    median shift is to be read against.  A surrogate-transition chain
+    (--st-steps surrogate stretch moves per walker and half-step, one exact
+    correction at the end) runs from the same state and is compared the
+    same way."""
     from pofd_affine_amd import _lib, synth, workload
     N, W = args.nflux, args.da_walkers
     ctx = workload.build_context(inp, N, args.ninterp, binned=True)
@@ -616,9 +623,11 @@ def bench_da(args, inp):
                        holdout_frac=0.1, seed=6)
     t_train = time.perf_counter() - t0
 
-    def chain(use_da, seed=41):
+    def chain(use_da, seed=41, inner=1):
         e = make(start, seed)
         on = e.set_surrogate(net, knots) if use_da else False
+        if on and inner > 1:
+            e.set_surrogate_steps(inner)
         e.run(5, thin=0)  # warm the launch path
         n0 = e.counts()
         e.clear_chain()
@@ -642,6 +651,7 @@ def bench_da(args, inp):
     plain = chain(False)
     da = chain(True)
     control = chain(False, seed=43)  # the Monte Carlo spread of the medians
+    st = chain(True, inner=args.st_steps) if args.st_steps > 1 else None
     S = args.da_steps
 
     def ess_per_s(r):
@@ -656,7 +666,7 @@ def bench_da(args, inp):
     sig = flat_p.std(axis=0)
     dmed = (np.median(flat_d, axis=0) - np.median(flat_p, axis=0)) / sig
     dctl = (np.median(flat_c, axis=0) - np.median(flat_p, axis=0)) / sig
-    return {
+    res = {
         "workload": "1D P(D) stretch move, nflux=%d, %d walkers, %d steps "
                     "per chain, binned %dx%d map; surrogate MLP %d -> "
                     "3x256 GELU -> 1 (bf16, fp32 accumulate) trained on "
@@ -699,6 +709,21 @@ def bench_da(args, inp):
         "control_max_abs_median_shift_sigma": float(np.max(np.abs(dctl))),
         "control_tau_mean": float(np.nanmean(control["taus"])),
     }
+    if st is not None:
+        es, ts = ess_per_s(st)
+        flat_s = st["chain"].reshape(-1, nk).astype(np.float64)
+        dst = (np.median(flat_s, axis=0) - np.median(flat_p, axis=0)) / sig
+        res["surrogate_transition"] = {
+            "inner_steps": args.st_steps,
+            "wall_s": st["wall"], "ms_per_step": 1e3 * st["wall"] / S,
+            "tau_mean": ts, "tau": st["taus"].tolist(), "ess_per_s": es,
+            "exact_evals": st["nexact"], "proposals": st["nprop"],
+            "acceptance": st["acc"],
+            "ess_per_s_ratio": es / ep,
+            "median_shift_sigma": dst.tolist(),
+            "max_abs_median_shift_sigma": float(np.max(np.abs(dst))),
+        }
+    return res
 
 
 def bench_2d(args, hiprt):

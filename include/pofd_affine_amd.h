@@ -588,6 +588,16 @@ int pofd_mlp_train(pofd_mlp *mlp, const float *x, const double *y,
 int pofd_ens_set_surrogate(pofd_ens *ens, pofd_mlp *mlp,
                            const uint32_t *param_index, uint32_t nidx,
                            double gate, double holdout_rms, int *enabled);
+/* Surrogate steps per half-step while a surrogate is on (no counterpart in
+   the reference; default 1).  1 is the delayed-acceptance step above.  With
+   nsteps > 1 each walker of a half first runs nsteps stretch moves against
+   the other half accepted on the surrogate alone, then its end point is
+   accepted with exp(((L(y) - L(x)) - (S(y) - S(x)))/T) on the exact log
+   likelihood L (Liu's surrogate transition method): the chain still
+   samples the exact posterior, every stored log likelihood is exact, one
+   exact evaluation per walker whose inner chain moved, and a half-step
+   moves each walker up to nsteps times.  nsteps in [1, 64]. */
+int pofd_ens_set_surrogate_steps(pofd_ens *ens, uint32_t nsteps);
 /* zero the per-walker acceptance and rejection counters (the end of a
    burn-in) */
 int pofd_ens_reset_counts(pofd_ens *ens);

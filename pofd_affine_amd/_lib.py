@@ -107,6 +107,7 @@ _SIGS = {
     "pofd_acor": (C.c_int, [FP, C.c_uint32, DP, DP, DP]),
     "pofd_ens_set_surrogate": (C.c_int, [VP, VP, U32P, C.c_uint32, D, D,
                                          C.POINTER(C.c_int)]),
+    "pofd_ens_set_surrogate_steps": (C.c_int, [VP, C.c_uint32]),
     "pofd_ens_counts": (C.c_int, [VP, C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]),
     "pofd_mlp_create": (C.c_int, [C.POINTER(VP), C.c_uint32, C.c_uint32,
@@ -862,6 +863,14 @@ class Ensemble:
             self.ptr, mlp.ptr, idx.ctypes.data_as(U32P), idx.size, gate,
             holdout_rms, C.byref(on)), "pofd_ens_set_surrogate")
         return bool(on.value)
+
+    def set_surrogate_steps(self, nsteps):
+        """Surrogate stretch moves per walker and half-step while the
+        surrogate is on: 1 is delayed acceptance; more runs an inner chain
+        on the surrogate with one exact correction at its end (the
+        surrogate transition).  1 to 64."""
+        self._check(lib().pofd_ens_set_surrogate_steps(self.ptr, int(nsteps)),
+                    "pofd_ens_set_surrogate_steps")
 
     def counts(self):
         """(proposals, exact likelihood evaluations) since set_positions."""

@@ -36,6 +36,10 @@ struct pofd_ens {
   bool use_da = false;
   int* sidx = nullptr;         // parameter index of each surrogate input
   void* dabuf[9] = {};
+  // surrogate steps per half-step: 1 is delayed acceptance, more is the
+  // surrogate transition's inner chain
+  uint32_t st_steps = 1;
+  void* stbuf[3] = {};
   uint64_t n_exact = 0, n_proposed = 0;
   // host likelihood (POFD_TARGET_HOST): the caller's function, host staging
   // and the device copy of the bonus values it returns
@@ -55,6 +59,8 @@ struct pofd_ens {
     if (e.chainll) (void)hipFree(e.chainll);
     for (void* p : dabuf)
       if (p) (void)hipFree(p);
+    for (void* p : stbuf)
+      if (p) (void)hipFree(p);
     if (sidx) (void)hipFree(sidx);
     if (host_bonus_dev) (void)hipFree(host_bonus_dev);
     if (stream) (void)hipStreamDestroy(stream);
@@ -62,6 +68,14 @@ struct pofd_ens {
 };
 
 namespace {
+
+// random-draw numbers of the surrogate transition, apart from those of the
+// plain and delayed-acceptance steps (0-41, 100-102): inner step i
+// proposes with draws kInnerDraw0 + kInnerDrawStride i + (0 .. 41) and
+// accepts with + kInnerAcceptDraw; the exact correction uses kCorrectDraw
+constexpr uint32_t kInnerDraw0 = 1000, kInnerDrawStride = 64, kInnerAcceptDraw = 50;
+constexpr uint32_t kCorrectDraw = 999;
+constexpr uint32_t kMaxInnerSteps = 64;
 
 int efail(pofd_ens* s, int code, const std::string& msg) {
   if (s) s->err = msg;
@@ -195,6 +209,53 @@ int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int stor
       return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
   }
   if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
+  return POFD_OK;
+}
+
+// one half-step of the surrogate transition: st_steps stretch moves per
+// walker on the surrogate (the other half fixed), then one exact
+// correction of the walkers whose inner chain moved
+int st_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int store) {
+  const uint32_t hw = s->nwalk / 2;
+  EnsDev& e = s->e;
+  if (pofd_ens_dev::launch_st_init(e, half, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: inner chain launch failed");
+  for (uint32_t i = 0; i < s->st_steps; ++i) {
+    const uint32_t d0 = kInnerDraw0 + kInnerDrawStride * i;
+    if (pofd_ens_dev::launch_propose(e, half, step, s->stream, e.yin, d0))
+      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
+    if (pofd_mlp_predict_device(s->mlp, e.prop, static_cast<int>(s->npar), s->sidx,
+                                static_cast<int>(hw), e.llsp, s->stream))
+      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: surrogate prediction failed");
+    if (pofd_ens_dev::launch_st_inner(e, half, step, inv_temp, d0 + kInnerAcceptDraw, s->stream))
+      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: inner chain launch failed");
+  }
+  if (pofd_ens_dev::launch_stage1(e, half, step, inv_temp, s->stream, 1) ||
+      pofd_ens_dev::launch_compact(e, s->stream, e.yin))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: compaction launch failed");
+  int32_t npass = static_cast<int32_t>(hw);
+  if (s->target == POFD_TARGET_1D) {
+    int rc = pofd1d_loglike_batch_device_count(s->ctx, e.cprop, hw, e.npass, s->npar, e.cll,
+                                               e.cst, s->stream);
+    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+    e.bonus_src = pofd1d_bonus_device(s->ctx);
+    e.bonus_stride = 4;
+    e.bonus_count = 2;
+  } else {
+    hipError_t he = hipMemcpyAsync(&npass, e.npass, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   s->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
+    if (he != hipSuccess) return ehip(s, he, "hipMemcpy(npass)");
+    if (npass > 0) {
+      int rc = evaluate(s, e.cprop, static_cast<uint32_t>(npass), e.cll, e.cst);
+      if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+    }
+  }
+  if (npass > 0 &&
+      pofd_ens_dev::launch_stage2(e, half, step, inv_temp, s->stream, e.yin, e.lsy, kCorrectDraw))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: correction launch failed");
+  if (store && pofd_ens_dev::launch_store_half(e, half, s->nstored, s->stream))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
   return POFD_OK;
 }
@@ -411,6 +472,26 @@ int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_ind
   return refresh_surrogate(s);
 }
 
+int pofd_ens_set_surrogate_steps(pofd_ens* s, uint32_t nsteps) {
+  if (!s) return POFD_E_ARG;
+  if (nsteps < 1 || nsteps > kMaxInnerSteps)
+    return efail(s, POFD_E_ARG, "pofd_ens_set_surrogate_steps: steps must be 1 to 64");
+  if (nsteps > 1 && !s->stbuf[0]) {
+    const size_t H = s->nwalk / 2, P = s->npar;
+    const size_t sizes[3] = {H * P * sizeof(float), H * sizeof(double), H * sizeof(int32_t)};
+    for (int i = 0; i < 3; ++i) {
+      hipError_t he = hipMalloc(&s->stbuf[i], sizes[i]);
+      if (he == hipSuccess) he = hipMemset(s->stbuf[i], 0, sizes[i]);
+      if (he != hipSuccess) return ehip(s, he, "hipMalloc(surrogate transition)");
+    }
+    s->e.yin = static_cast<float*>(s->stbuf[0]);
+    s->e.lsy = static_cast<double*>(s->stbuf[1]);
+    s->e.moved = static_cast<int32_t*>(s->stbuf[2]);
+  }
+  s->st_steps = nsteps;
+  return POFD_OK;
+}
+
 int pofd_ens_reset_counts(pofd_ens* s) {
   if (!s) return POFD_E_ARG;
   const size_t W = s->nwalk;
@@ -471,6 +552,12 @@ int pofd_ens_run(pofd_ens* s, uint32_t nsteps, double temperature, uint32_t thin
     const uint32_t step = s->step++;
     const int store = (thin != 0 && (k + 1) % thin == 0) ? 1 : 0;
     for (uint32_t half = 0; half < 2; ++half) {
+      if (s->use_da && s->st_steps > 1) {
+        s->n_proposed += hw;
+        int rc = st_half(s, half, step, inv_temp, store);
+        if (rc) return rc;
+        continue;
+      }
       if (pofd_ens_dev::launch_propose(s->e, half, step, s->stream))
         return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
       s->n_proposed += hw;

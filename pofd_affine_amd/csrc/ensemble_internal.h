@@ -42,6 +42,10 @@ struct EnsDev {
   double* cll;       // nwalk/2        their exact log likelihoods
   int32_t* cst;      // nwalk/2
   unsigned long long* nexact;  // 1: exact evaluations requested by stage 1
+  // surrogate transition (an inner chain of several surrogate steps)
+  float* yin;        // nwalk/2 x npar inner-chain positions
+  double* lsy;       // nwalk/2        their surrogate log likelihoods
+  int32_t* moved;    // nwalk/2        the inner chain has moved
   // counts integrals of the last likelihood batch, bonus_stride per batch
   // row, [0] the source count: the first bonus_count bonus parameters of
   // an accepted step take entries 1.. (1D: mean flux and flux^2 per area,
@@ -51,7 +55,9 @@ struct EnsDev {
   int bonus_stride, bonus_count;
 };
 
-int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st);
+// from: the walkers' positions (null: the current ones); draw0: first draw
+int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st,
+                   const float* from = nullptr, uint32_t draw0 = 0);
 int launch_gauss(const EnsDev& e, const float* x, int n, double* ll, int32_t* stv,
                  hipStream_t st);
 int launch_accept(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp, int store,
@@ -60,10 +66,16 @@ int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st);
 // delayed acceptance: first stage on the surrogate, compaction of the
 // passing proposals, second stage on their exact likelihoods, chain store
 int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st);
-int launch_compact(const EnsDev& e, hipStream_t st);
+                  hipStream_t st, int moved_only = 0);
+int launch_compact(const EnsDev& e, hipStream_t st, const float* ys = nullptr);
 int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st);
+                  hipStream_t st, const float* ys = nullptr, const double* lsn = nullptr,
+                  uint32_t draw = 102);
+// surrogate transition: inner positions from the walkers, one inner step's
+// acceptance on the surrogate
+int launch_st_init(const EnsDev& e, uint32_t half, hipStream_t st);
+int launch_st_inner(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
+                    uint32_t draw, hipStream_t st);
 int launch_store_half(const EnsDev& e, uint32_t half, uint32_t slot, hipStream_t st);
 // bonus parameters of all walkers from the batch that evaluated them in order
 int launch_bonus_init(const EnsDev& e, hipStream_t st);

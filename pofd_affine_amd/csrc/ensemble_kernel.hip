@@ -100,24 +100,29 @@ __device__ __forceinline__ void fill_bonus(const EnsDev& e, float* x, int r) {
 
 }  // namespace
 
-// one thread per walker of half `half`
+// one thread per walker of half `half`; the walker's position is its
+// current one, or row t of `from` (the surrogate-transition inner chain);
+// its draws are numbered from draw0
 __global__ __launch_bounds__(64) void propose_kernel(const EnsDev e, const uint32_t half,
-                                                     const uint32_t step) {
+                                                     const uint32_t step, const float* from,
+                                                     const uint32_t draw0) {
   const int hw = e.nwalk / 2;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= hw) return;
   const int k = static_cast<int>(half) * hw + t;       // walker updated
   const int other = (half == 0) ? hw : 0;              // partner half
-  const Draws d0 = draws(e, step, half, k, 0);
+  const Draws d0 = draws(e, step, half, k, draw0);
   int j = other + static_cast<int>(static_cast<double>(hw) * d0.u0);
   if (j >= other + hw) j = other + hw - 1;
-  const float* xo = e.pos + static_cast<size_t>(k) * e.npar;
+  const float* xo = from ? from + static_cast<size_t>(t) * e.npar
+                         : e.pos + static_cast<size_t>(k) * e.npar;
   const float* xp = e.pos + static_cast<size_t>(j) * e.npar;
   float* y = e.prop + static_cast<size_t>(t) * e.npar;
   float z = gen_z(e.scalefac, d0.u1);
   bool ok = false;
   for (int it = 0; it <= 41 && !ok; ++it) {  // first try + maxiters redraws
-    if (it > 0) z = gen_z(e.scalefac, draws(e, step, half, k, static_cast<uint32_t>(it)).u0);
+    if (it > 0)
+      z = gen_z(e.scalefac, draws(e, step, half, k, draw0 + static_cast<uint32_t>(it)).u0);
     const float omz = 1.0f - z;
     for (int i = 0; i < e.npar; ++i) {
       const int st = e.state[i];
@@ -206,10 +211,13 @@ __global__ __launch_bounds__(64) void accept_kernel(const EnsDev e, const uint32
 // lists the survivors in cidx and counts them in npass.
 constexpr int kCompactThreads = 1024;
 
+// moved_only: no first stage; the walkers whose surrogate-transition inner
+// chain moved are the survivors
 __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
                                                                  const uint32_t half,
                                                                  const uint32_t step,
-                                                                 const double inv_temp) {
+                                                                 const double inv_temp,
+                                                                 const int moved_only) {
   __shared__ int woff[kCompactThreads / 64 + 1];
   const int hw = e.nwalk / 2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -217,7 +225,9 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
   for (int t0 = 0; t0 < hw; t0 += kCompactThreads) {
     const int t = t0 + tid;
     int pass = 0;
-    if (t < hw) {
+    if (t < hw && moved_only) {
+      pass = e.moved[t];
+    } else if (t < hw) {
       const int k = static_cast<int>(half) * hw + t;
       if (!e.pvalid[t]) {
         e.nerr[k] += 1;
@@ -229,8 +239,8 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
           pass = (a1 >= 1.0) || (draws(e, step, half, k, 101).u0 < a1);
         }
       }
-      e.pass[t] = pass;
     }
+    if (t < hw) e.pass[t] = pass;
     const unsigned long long bal = __ballot(pass);
     if (lane == 0) woff[wave] = __popcll(bal);
     __syncthreads();
@@ -257,17 +267,21 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
   }
 }
 
-__global__ __launch_bounds__(64) void gather_pass_kernel(const EnsDev e) {
+// the survivors' rows of ys (the proposals, or the inner chains' ends)
+__global__ __launch_bounds__(64) void gather_pass_kernel(const EnsDev e, const float* ys) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= *e.npass) return;
   const int t = e.cidx[c];
-  const float* src = e.prop + static_cast<size_t>(t) * e.npar;
+  const float* src = ys + static_cast<size_t>(t) * e.npar;
   float* dst = e.cprop + static_cast<size_t>(c) * e.npar;
   for (int i = 0; i < e.npar; ++i) dst[i] = src[i];
 }
 
+// the candidate of proposal slot t is row t of ys with surrogate lsn[t]
 __global__ __launch_bounds__(64) void stage2_kernel(const EnsDev e, const uint32_t half,
-                                                    const uint32_t step, const double inv_temp) {
+                                                    const uint32_t step, const double inv_temp,
+                                                    const float* ys, const double* lsn,
+                                                    const uint32_t draw) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= *e.npass) return;
   const int hw = e.nwalk / 2;
@@ -279,16 +293,58 @@ __global__ __launch_bounds__(64) void stage2_kernel(const EnsDev e, const uint32
     e.nerr[k] += 1;
     return;
   }
-  const double a2 = exp(inv_temp * ((lnew - e.ll[k]) - (e.llsp[t] - e.llsur[k])));
-  const bool accept = (a2 >= 1.0) || (draws(e, step, half, k, 102).u0 < a2);
+  const double a2 = exp(inv_temp * ((lnew - e.ll[k]) - (lsn[t] - e.llsur[k])));
+  const bool accept = (a2 >= 1.0) || (draws(e, step, half, k, draw).u0 < a2);
   if (!accept) return;
   float* xo = e.pos + static_cast<size_t>(k) * e.npar;
-  const float* y = e.prop + static_cast<size_t>(t) * e.npar;
+  const float* y = ys + static_cast<size_t>(t) * e.npar;
   for (int i = 0; i < e.npar; ++i) xo[i] = y[i];
   fill_bonus(e, xo, c);
   e.ll[k] = lnew;
-  e.llsur[k] = e.llsp[t];
+  e.llsur[k] = lsn[t];
   e.naccept[k] += 1;
+}
+
+// ---- surrogate transition (Liu, Monte Carlo Strategies in Scientific
+// Computing, 2001, sec. 9.4.3): with the other half fixed, each walker of
+// the half runs `nsteps` stretch moves accepted on the surrogate alone
+// (z^d exp((L~new - L~old)/T)), a chain reversible with respect to
+// exp(L~/T); its end y is then accepted with
+// exp(((L(y) - L(x)) - (L~(y) - L~(x)))/T) (stage2_kernel), which makes
+// the whole move reversible with respect to the exact posterior.  One inner
+// step is the delayed-acceptance first stage.  Walkers whose inner chain
+// never moved need no exact evaluation.
+__global__ __launch_bounds__(64) void st_init_kernel(const EnsDev e, const uint32_t half) {
+  const int hw = e.nwalk / 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hw) return;
+  const int k = static_cast<int>(half) * hw + t;
+  const float* x = e.pos + static_cast<size_t>(k) * e.npar;
+  float* y = e.yin + static_cast<size_t>(t) * e.npar;
+  for (int i = 0; i < e.npar; ++i) y[i] = x[i];
+  e.lsy[t] = e.llsur[k];
+  e.moved[t] = 0;
+}
+
+// inner step: the proposals (from the inner positions) against them on the
+// surrogate predictions llsp; its uniform is draw `draw`
+__global__ __launch_bounds__(64) void st_inner_kernel(const EnsDev e, const uint32_t half,
+                                                      const uint32_t step, const double inv_temp,
+                                                      const uint32_t draw) {
+  const int hw = e.nwalk / 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hw || !e.pvalid[t]) return;
+  const int k = static_cast<int>(half) * hw + t;
+  const double ls = e.llsp[t];
+  if (isnan(ls) || isinf(ls)) return;
+  const double a = exp(static_cast<double>(e.dexp) * log(static_cast<double>(e.z[t])) +
+                       inv_temp * (ls - e.lsy[t]));
+  if (!((a >= 1.0) || (draws(e, step, half, k, draw).u0 < a))) return;
+  const float* p = e.prop + static_cast<size_t>(t) * e.npar;
+  float* y = e.yin + static_cast<size_t>(t) * e.npar;
+  for (int i = 0; i < e.npar; ++i) y[i] = p[i];
+  e.lsy[t] = ls;
+  e.moved[t] = 1;
 }
 
 __global__ __launch_bounds__(64) void bonus_init_kernel(const EnsDev e) {
@@ -319,9 +375,11 @@ __global__ __launch_bounds__(256) void zsample_kernel(const EnsDev e, const uint
   out[t] = v * v / a;
 }
 
-int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st) {
+int launch_propose(const EnsDev& e, uint32_t half, uint32_t step, hipStream_t st,
+                   const float* from, uint32_t draw0) {
   const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(propose_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step);
+  hipLaunchKernelGGL(propose_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step, from,
+                     draw0);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -346,23 +404,38 @@ int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st) {
 }
 
 int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st) {
+                  hipStream_t st, int moved_only) {
   hipLaunchKernelGGL(stage1_kernel, dim3(1), dim3(kCompactThreads), 0, st, e, half, step,
-                     inv_temp);
+                     inv_temp, moved_only);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_compact(const EnsDev& e, hipStream_t st) {
+int launch_compact(const EnsDev& e, hipStream_t st, const float* ys) {
   const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(gather_pass_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e);
+  hipLaunchKernelGGL(gather_pass_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e,
+                     ys ? ys : e.prop);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st) {
+                  hipStream_t st, const float* ys, const double* lsn, uint32_t draw) {
   const int hw = e.nwalk / 2;
   hipLaunchKernelGGL(stage2_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
-                     inv_temp);
+                     inv_temp, ys ? ys : e.prop, lsn ? lsn : e.llsp, draw);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_st_init(const EnsDev& e, uint32_t half, hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(st_init_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_st_inner(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
+                    uint32_t draw, hipStream_t st) {
+  const int hw = e.nwalk / 2;
+  hipLaunchKernelGGL(st_inner_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
+                     inv_temp, draw);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -179,6 +179,59 @@ def test_delayed_acceptance_keeps_exact_target(shift, width):
         assert np.std(ch[:, :, i]) == pytest.approx(sig[i], rel=0.06), i
 
 
+@pytest.mark.parametrize("shift,width", [(0.0, 1.0), (0.7, 1.5)])
+def test_surrogate_transition_keeps_exact_target(shift, width):
+    """The surrogate transition (8 surrogate stretch moves per walker and
+    half-step, one exact correction at the end) samples the exact Gaussian
+    whether the surrogate is right or 0.7 sigma and 50% in width wrong; the
+    stored log likelihoods are the exact ones, and with a right surrogate
+    the chain decorrelates faster than the plain one per step."""
+    _say("surrogate transition Gaussian shift=%.1f width=%.1f" % (shift,
+                                                                   width))
+    ens, net, mean, sig = _gauss_da(23, shift, width)
+    assert ens.set_surrogate(net, [0, 1, 2, 3], gate=np.inf,
+                             holdout_rms=0.0)
+    ens.set_surrogate_steps(8)
+    ens.run(500, thin=0)
+    n0 = ens.counts()
+    ens.run(4000, thin=1)
+    n1 = ens.counts()
+    ch, ll = ens.chain()
+    _, _, nacc, nerr = ens.state()
+    _say("proposals %d exact %d" % (n1[0] - n0[0], n1[1] - n0[1]))
+    assert n1[0] - n0[0] == 64 * 4000
+    assert n1[1] - n0[1] <= n1[0] - n0[0]
+    assert np.all(ch[:, :, 4] == 7.0) and np.all(np.isnan(ch[:, :, 5]))
+    r = (ch[..., :4].astype(np.float64) - mean[:4]) / sig[:4]
+    np.testing.assert_allclose(ll, -0.5 * np.sum(r * r, axis=-1),
+                               rtol=1e-12, atol=1e-12)
+    for i in range(4):
+        bm = ch[:, :, i].astype(np.float64).mean(axis=1).reshape(20, 200)
+        bm = bm.mean(axis=1)
+        se = bm.std(ddof=1) / np.sqrt(20)
+        m = bm.mean()
+        assert abs(m - mean[i]) < 5 * se, (i, m, se)
+        assert np.std(ch[:, :, i]) == pytest.approx(sig[i], rel=0.06), i
+    if shift == 0.0:
+        tau_st = np.mean([ens.acor(i)[0] for i in range(4)])
+        ref, _, _, _ = _gauss_da(23, shift, width)
+        ref.run(500, thin=0)
+        ref.run(4000, thin=1)
+        tau_plain = np.mean([ref.acor(i)[0] for i in range(4)])
+        _say("tau plain %.1f surrogate transition %.1f" % (tau_plain,
+                                                           tau_st))
+        assert tau_st < 0.5 * tau_plain
+
+
+def test_surrogate_transition_steps_limits():
+    ens, net, _, _ = _gauss_da(7, 0.0, 1.0)
+    for bad in (0, 65):
+        with pytest.raises(_lib.PofdError):
+            ens.set_surrogate_steps(bad)
+    ens.set_surrogate_steps(64)
+    ens.set_surrogate_steps(1)
+
+
 def test_good_surrogate_saves_exact_evaluations():
     _say("DA savings")
     ens, net, mean, sig = _gauss_da(5, 0.0, 1.0)
@@ -245,6 +298,49 @@ def test_delayed_acceptance_on_pofd1d():
     assert n1[1] - n0[1] < n1[0] - n0[0]
     ch2, cll2 = ens.chain()
     assert ch2.shape[0] == 250 and np.all(np.isfinite(cll2))
+
+
+def test_surrogate_transition_on_pofd1d():
+    """The surrogate transition on the 1D P(D) likelihood: every walker's
+    stored log likelihood is the exact one at its position, and at most one
+    exact evaluation per walker and half-step."""
+    _say("surrogate transition on the 1D P(D) likelihood")
+    inp = workload.make_inputs(npix_side=200, seed=7)
+    ctx = workload.build_context(inp, 32768, 512, binned=True)
+    nw = 64
+    th = synth.walker_thetas(nw, seed=31)
+    npar = th.shape[1]
+    ens = _lib.Ensemble(ctx, _lib.TARGET_1D, nw, npar, seed=17)
+    state = np.zeros(npar, dtype=np.uint8)
+    state[8] = 1
+    state[9:] = 2
+    ens.set_param_state(state)
+    ens.set_exponent_mode(True)
+    ens.set_positions(th)
+    ens.run(150, thin=1)
+    ch, cll = ens.chain()
+    net = _lib.Mlp(8, 128, seed=2)
+    tr, ho = net.train(ch[50:, :, :8].reshape(-1, 8), cll[50:].reshape(-1),
+                       epochs=100, batch=256, lr=1e-3, holdout_frac=0.1,
+                       seed=3)
+    _say("train RMS %.3f holdout RMS %.3f" % (tr, ho))
+    assert ens.set_surrogate(net, list(range(8)), gate=np.inf,
+                             holdout_rms=0.0)
+    ens.set_surrogate_steps(6)
+    n0 = ens.counts()
+    _, _, a0, _ = ens.state()
+    ens.run(60, thin=1)
+    n1 = ens.counts()
+    cur, ll, a1, nerr = ens.state()
+    ref, st = ctx.loglike_batch(cur)
+    assert np.all(st == 0)
+    np.testing.assert_array_equal(ll, ref)
+    _say("exact %d of %d walker half-steps, accepted %d" % (
+        n1[1] - n0[1], n1[0] - n0[0], (a1 - a0).sum()))
+    assert 0 < n1[1] - n0[1] <= n1[0] - n0[0]
+    assert (a1 - a0).sum() > 0
+    _, cll2 = ens.chain()
+    assert np.all(np.isfinite(cll2))
 
 
 def test_delayed_acceptance_on_pofd2d():
