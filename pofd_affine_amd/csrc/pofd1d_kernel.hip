@@ -1472,8 +1472,58 @@ __global__ __launch_bounds__(kThreads) void inverse1d_kernel(const BeamsetDev bs
 // normalisation, mean, binned log likelihood (an unbinned dataset gets
 // log2 of its normalised P(D) in place for pixels1d_kernel)
 // ===========================================================================
+// combine for one walker: (curr - like_offset)/like_norm summed over the
+// datasets of the beam set in order, added to the batch outputs (one
+// thread; the walker's dataset results already written)
+__device__ void combine_walker(const BeamsetDev& bs, const WorkDev& wk, const int walker,
+                               const int accumulate, double* __restrict__ loglike,
+                               int32_t* __restrict__ status) {
+  const double* st = walker_state(wk, walker);
+  const int s = static_cast<int>(st[ST_STATUS]);
+  const double prev = accumulate ? loglike[walker] : 0.0;
+  if (s == POFD_E_BADLIKE) {
+    status[walker] = s;
+    loglike[walker] = prev + 1e25;  // calcLikeSingle::bad_like per beam set
+    return;
+  }
+  if (s != POFD_OK) {
+    status[walker] = s;
+    loglike[walker] = qnan();
+    return;
+  }
+  double total = 0.0;
+  for (int f = 0; f < bs.nfields; ++f) {
+    const FieldDev& fd = bs.fields[f];
+    const size_t wf = static_cast<size_t>(walker) * bs.nfields + f;
+    double* dbg = wk.dbg + wf * kDbg;
+    const int fs = static_cast<int>(dbg[15]);
+    if (fs != POFD_OK) {  // the reference stops at the first failing dataset
+      status[walker] = fs;
+      loglike[walker] = qnan();
+      return;
+    }
+    double curr = dbg[14];
+    if (!fd.binned) {
+      const double* up = wk.upart + wf * kPixBlocks;
+      double s2 = 0.0;
+      for (int b = 0; b < kPixBlocks; ++b) s2 += up[b];
+      curr = 0.69314718055994530942 * s2;  // log2toe
+      dbg[14] = curr;
+    }
+    total += (curr - fd.like_offset) / fd.like_norm;
+  }
+  status[walker] = POFD_OK;
+  loglike[walker] = prev + total;
+}
+
+// fuse: the beam set has one dataset and it is binned, so the combine needs
+// nothing from a later launch: thread 0 does it for the walker at the end
+// (combine1d_kernel is then not launched)
 __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev bs,
-                                                               const WorkDev wk) {
+                                                               const WorkDev wk, const int fuse,
+                                                               const int accumulate,
+                                                               double* __restrict__ loglike,
+                                                               int32_t* __restrict__ status_out) {
   __shared__ double red[kWaves * 2];
   __shared__ int isc[4];
   __shared__ double sc[2];
@@ -1484,7 +1534,10 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
   const int parts = batch_parts(n, wk.force_parts, wk.nslots, wk.parts_tab);
   if (walker >= n) return;
   const double* st = walker_state(wk, walker);
-  if (st[ST_STATUS] != POFD_OK) return;  // uniform across the workgroup
+  if (st[ST_STATUS] != POFD_OK) {  // uniform across the workgroup
+    if (fuse && tid == 0) combine_walker(bs, wk, walker, accumulate, loglike, status_out);
+    return;
+  }
   unsigned long long tmark = wall_clock64();
   const int N = bs.N;
   const double dflux = bs.dflux;
@@ -1767,6 +1820,7 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
     dbg[29] = static_cast<double>(t_split);
     dbg[30] = static_cast<double>(wall_clock64() - tmark);
     dbg[31] = dbg[24] + dbg[26] + dbg[27] + dbg[28] + dbg[29] + dbg[30];
+    if (fuse) combine_walker(bs, wk, walker, accumulate, loglike, status_out);
   }
 }
 
@@ -1849,42 +1903,7 @@ __global__ void combine1d_kernel(const BeamsetDev bs, const WorkDev wk,
                                  int32_t* __restrict__ status) {
   const int walker = blockIdx.x * blockDim.x + threadIdx.x;
   if (walker >= batch_n(wk)) return;
-  const double* st = walker_state(wk, walker);
-  const int s = static_cast<int>(st[ST_STATUS]);
-  const double prev = accumulate ? loglike[walker] : 0.0;
-  if (s == POFD_E_BADLIKE) {
-    status[walker] = s;
-    loglike[walker] = prev + 1e25;  // calcLikeSingle::bad_like per beam set
-    return;
-  }
-  if (s != POFD_OK) {
-    status[walker] = s;
-    loglike[walker] = qnan();
-    return;
-  }
-  double total = 0.0;
-  for (int f = 0; f < bs.nfields; ++f) {
-    const FieldDev& fd = bs.fields[f];
-    const size_t wf = static_cast<size_t>(walker) * bs.nfields + f;
-    double* dbg = wk.dbg + wf * kDbg;
-    const int fs = static_cast<int>(dbg[15]);
-    if (fs != POFD_OK) {  // the reference stops at the first failing dataset
-      status[walker] = fs;
-      loglike[walker] = qnan();
-      return;
-    }
-    double curr = dbg[14];
-    if (!fd.binned) {
-      const double* up = wk.upart + wf * kPixBlocks;
-      double s2 = 0.0;
-      for (int b = 0; b < kPixBlocks; ++b) s2 += up[b];
-      curr = 0.69314718055994530942 * s2;  // log2toe
-      dbg[14] = curr;
-    }
-    total += (curr - fd.like_offset) / fd.like_norm;
-  }
-  status[walker] = POFD_OK;
-  loglike[walker] = prev + total;
+  combine_walker(bs, wk, walker, accumulate, loglike, status);
 }
 
 // Writes the P(D) of (walker, field) as PDFactory::getPD leaves PD::pd_:
@@ -1945,18 +1964,21 @@ int launch_loglike1d(const BeamsetDev& bs, const WorkDev& wk_in, int beamset_idx
     hipLaunchKernelGGL(spectrum1d_kernel, dim3(nwalk), dim3(kThreads), 0, s, bs, wk);
   mark(POFD_STAGE_INVERSE);
   hipLaunchKernelGGL(inverse1d_kernel, dim3(nxf, bs.nfields), dim3(kThreads), 0, s, bs, wk);
-  mark(POFD_STAGE_FINALIZE);
-  hipLaunchKernelGGL(finalize1d_kernel, dim3(nwalk, bs.nfields), dim3(kThreads), 0,
-                     s, bs, wk);
-  mark(POFD_STAGE_PIXELS);
   bool any_unbinned = false;
   for (int f = 0; f < bs.nfields; ++f) any_unbinned |= !bs.fields[f].binned;
+  // one binned dataset: finalize does the combine too
+  const int fuse = (bs.nfields == 1 && !any_unbinned) ? 1 : 0;
+  mark(POFD_STAGE_FINALIZE);
+  hipLaunchKernelGGL(finalize1d_kernel, dim3(nwalk, bs.nfields), dim3(kThreads), 0,
+                     s, bs, wk, fuse, accumulate, d_loglike, d_status);
+  mark(POFD_STAGE_PIXELS);
   if (any_unbinned)
     hipLaunchKernelGGL(pixels1d_kernel, dim3(kPixBlocks, nwalk, bs.nfields),
                        dim3(kPixThreads), 0, s, bs, wk);
   mark(POFD_STAGE_COMBINE);
-  hipLaunchKernelGGL(combine1d_kernel, dim3((nwalk + 63) / 64), dim3(64), 0, s,
-                     bs, wk, accumulate, d_loglike, d_status);
+  if (!fuse)
+    hipLaunchKernelGGL(combine1d_kernel, dim3((nwalk + 63) / 64), dim3(64), 0, s,
+                       bs, wk, accumulate, d_loglike, d_status);
   mark(POFD_NSTAGES);
   return hipGetLastError() == hipSuccess ? POFD_OK : POFD_E_INTERNAL;
 }
