@@ -25,6 +25,8 @@ struct pofd_mlp {
   size_t npar = 0;
   float* w = nullptr;   // float master parameters
   bf16* wb = nullptr;   // bf16 copy (weights used by the GEMMs)
+  bf16* wpk = nullptr;  // the hidden layers' bf16 weights packed for inference
+  size_t pk_off[3] = {};
   float* g = nullptr;   // gradient
   float* m = nullptr;   // Adam moments
   float* v = nullptr;
@@ -50,8 +52,8 @@ struct pofd_mlp {
   std::string err;
   ~pofd_mlp() {
     if (stream) (void)hipStreamSynchronize(stream);
-    void* ps[] = {w,   wb,  g, m, v, xmean, xw, idx, X, Z[0], Z[1], Z[2], A[0], A[1], A[2],
-                  out, y,   dout, se, dA, dA2, raw, dpred};
+    void* ps[] = {w,   wb,  wpk, g, m, v, xmean, xw, idx, X, Z[0], Z[1], Z[2], A[0], A[1],
+                  A[2], out, y,   dout, se, dA, dA2, raw, dpred};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -158,6 +160,15 @@ int mfail(pofd_mlp* s, int code, const std::string& msg) {
   return code;
 }
 
+// the packed copy of the hidden layers' weights, after every change of wb
+int repack(pofd_mlp* s) {
+  int rc = 0;
+  for (int l = 0; l < 3; ++l)
+    rc |= md::pack_weights(s->wb + s->off[2 * l], s->H, l == 0 ? s->nin : s->H,
+                           s->wpk + s->pk_off[l], s->stream);
+  return rc;
+}
+
 }  // namespace
 
 // used by the ensemble sampler: predictions (log likelihood units) of the
@@ -169,8 +180,9 @@ int pofd_mlp_predict_device(pofd_mlp* s, const float* src, int stride, const int
     lay.w[l] = s->off[2 * l];
     lay.b[l] = s->off[2 * l + 1];
   }
-  return md::infer(src, stride, idx, s->xmean, s->xw, s->nin, s->H, rows, s->wb, s->w, lay,
-                   s->ymean, s->ystd, out, st);
+  for (int l = 0; l < 3; ++l) lay.p[l] = s->pk_off[l];
+  return md::infer(src, stride, idx, s->xmean, s->xw, s->nin, s->H, rows, s->wb, s->wpk, s->w,
+                   lay, s->ymean, s->ystd, out, st);
 }
 
 extern "C" {
@@ -193,8 +205,13 @@ int pofd_mlp_create(pofd_mlp** out, uint32_t nin, uint32_t hidden, uint64_t seed
     o += (sz[k] + 7) & ~size_t(7);
   }
   s->npar = o;
+  const size_t Hp = (H + 15) & ~size_t(15), Ip = (I + 15) & ~size_t(15);
+  s->pk_off[0] = 0;
+  s->pk_off[1] = Hp * Ip;
+  s->pk_off[2] = Hp * Ip + Hp * Hp;
   if (hipStreamCreate(&s->stream) != hipSuccess ||
-      (dalloc(&s->w, o) | dalloc(&s->wb, o) | dalloc(&s->g, o) | dalloc(&s->m, o) |
+      (dalloc(&s->w, o) | dalloc(&s->wb, o) | dalloc(&s->wpk, Hp * Ip + 2 * Hp * Hp) |
+       dalloc(&s->g, o) | dalloc(&s->m, o) |
        dalloc(&s->v, o) | dalloc(&s->xmean, I) | dalloc(&s->xw, I * I) | dalloc(&s->idx, I))) {
     delete s;
     return POFD_E_INTERNAL;
@@ -219,7 +236,7 @@ int pofd_mlp_create(pofd_mlp** out, uint32_t nin, uint32_t hidden, uint64_t seed
   if (he == hipSuccess) he = hipMemcpy(s->xmean, mean.data(), I * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(s->xw, eye.data(), I * I * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(s->idx, idx.data(), I * sizeof(int), hipMemcpyHostToDevice);
-  if (he != hipSuccess || md::to_bf16(s->w, s->wb, o, s->stream) ||
+  if (he != hipSuccess || md::to_bf16(s->w, s->wb, o, s->stream) || repack(s) ||
       hipStreamSynchronize(s->stream) != hipSuccess) {
     delete s;
     return POFD_E_INTERNAL;
@@ -261,7 +278,7 @@ int pofd_mlp_get_params(pofd_mlp* s, float* w) {
 int pofd_mlp_set_params(pofd_mlp* s, const float* w) {
   if (!s || !w) return POFD_E_ARG;
   if (hipMemcpy(s->w, w, s->npar * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-      md::to_bf16(s->w, s->wb, s->npar, s->stream) ||
+      md::to_bf16(s->w, s->wb, s->npar, s->stream) || repack(s) ||
       hipStreamSynchronize(s->stream) != hipSuccess)
     return POFD_E_INTERNAL;
   return POFD_OK;
@@ -479,6 +496,7 @@ int pofd_mlp_train(pofd_mlp* s, const float* x, const double* y, uint32_t n, uin
     }
   }
   release();
+  if (repack(s)) return mfail(s, POFD_E_INTERNAL, "pofd_mlp_train: packing failed");
   // errors in log-likelihood units
   auto rms = [&](uint32_t first, uint32_t count, double* res) -> int {
     if (!res) return 0;

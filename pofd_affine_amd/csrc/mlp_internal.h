@@ -40,15 +40,24 @@ int mse_grad(const float* pred, const float* y, int M, float* d, float* se, hipS
 int adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float lr, float b1,
          float b2, float eps, float c1, float c2, hipStream_t st);
 
-// offsets (elements) of the weight and bias blocks in the parameter vector
+// offsets (elements) of the weight and bias blocks in the parameter vector,
+// and of the three hidden layers' weights in the packed copy (pack_weights)
 struct InferLayout {
-  size_t w[4], b[4];
+  size_t w[4], b[4], p[3];
 };
+// the packed copy of a hidden layer's bf16 weights W[j][k] (H outputs, K
+// inputs, row-major at wb): the v_mfma_f32_16x16x16_bf16 B fragments in
+// the order the inference kernel reads them, 16-column tile t, 16-input
+// step s, lane l, element q at ((t * (Kp/16) + s) * 64 + l) * 4 + q =
+// W[16 t + l % 16][16 s + 4 (l / 16) + q], zero past H and K (Hp x Kp
+// values, Hp and Kp the 16-multiples at or above H and K)
+int pack_weights(const bf16* wb, int H, int K, bf16* out, hipStream_t st);
 // fused prediction of rows inputs gathered from src (stride floats per row,
-// inputs at idx): out[r] = net(whitened x_r) * ys + ym
+// inputs at idx): out[r] = net(whitened x_r) * ys + ym; the hidden layers'
+// weights from the packed copy wpk, the output layer's from wb
 int infer(const float* src, int stride, const int* idx, const float* mean, const float* wz,
-          int nin, int H, int rows, const bf16* wb, const float* w, const InferLayout& lay,
-          double ym, double ys, double* out, hipStream_t st);
+          int nin, int H, int rows, const bf16* wb, const bf16* wpk, const float* w,
+          const InferLayout& lay, double ym, double ys, double* out, hipStream_t st);
 
 // out[i] = pred[i] * ys + ym (double)
 int unstandardize(const float* pred, int n, double ym, double ys, double* out, hipStream_t st);

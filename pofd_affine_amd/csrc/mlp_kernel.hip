@@ -234,23 +234,23 @@ constexpr int kMaxIn = 64;  // network inputs (pofd_mlp_create's limit)
 constexpr int kInfThreads = 1024;
 constexpr int kInfWaves = kInfThreads / 64;
 
-__device__ __forceinline__ short4v wfrag(const bf16* W, int j, int k, int H, int K) {
-  short4v bv;
-  if (j < H && k + 3 < K && (K & 3) == 0) {
-    bv = *reinterpret_cast<const short4v*>(W + static_cast<size_t>(j) * K + k);
-  } else {
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      bv[q] = (j < H && k + q < K) ? static_cast<short>(W[static_cast<size_t>(j) * K + k + q])
-                                   : static_cast<short>(0);
+__global__ __launch_bounds__(256) void pack_kernel(const bf16* wb, int H, int K, bf16* out) {
+  const int Hp = (H + 15) & ~15, Kp = (K + 15) & ~15, nks = Kp / 16;
+  const size_t n = static_cast<size_t>(Hp) * Kp;
+  for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int q = static_cast<int>(e & 3), l = static_cast<int>((e >> 2) & 63);
+    const size_t ts = e >> 8;
+    const int sk = static_cast<int>(ts % nks), t = static_cast<int>(ts / nks);
+    const int j = 16 * t + (l & 15), k = 16 * sk + 4 * (l >> 4) + q;
+    out[e] = (j < H && k < K) ? wb[static_cast<size_t>(j) * K + k] : static_cast<bf16>(0);
   }
-  return bv;
 }
 
 __global__ __launch_bounds__(kInfThreads) void infer_kernel(
     const float* src, int stride, const int* idx, const float* mean, const float* wz, int nin,
-    int H, int rows, const bf16* wb, const float* w, InferLayout lay, double ym, double ys,
-    double* out) {
+    int H, int rows, const bf16* wb, const bf16* wpk, const float* w, InferLayout lay,
+    double ym, double ys, double* out) {
   extern __shared__ bf16 act[];  // two buffers of kInfRows x ld bf16
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = blockIdx.x * kInfRows;
@@ -287,62 +287,31 @@ __global__ __launch_bounds__(kInfThreads) void infer_kernel(
   const int li = lane & 15, lk = (lane >> 4) * 4;
   const int ntile = Hp / 16;
   for (int l = 0; l < 3; ++l) {
-    const int K = (l == 0) ? nin : H;
     const int Kp = (l == 0) ? Kin : Hp;
-    const bf16* W = wb + lay.w[l];
+    const int nks = Kp / 16;
     const float* b = w + lay.b[l];
     for (int t = wave; t < ntile; t += kInfWaves) {
       const int j = t * 16 + li;  // output column this lane feeds B for
       float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      int k0 = 0;
-      if ((H & 15) == 0 && (K & 15) == 0) {
-        // every lane in range (a uniform test): unguarded loads, so that the
-        // sixteen fragments of 256 inputs are all in flight before one wait
-        const bf16* wr = W + static_cast<size_t>(j) * K + lk;
-        for (; k0 + 256 <= Kp; k0 += 256) {
-          short4v bq[16];
-#pragma unroll
-          for (int u = 0; u < 16; ++u)
-            bq[u] = *reinterpret_cast<const short4v*>(wr + k0 + 16 * u);
-#pragma unroll
-          for (int u = 0; u < 16; ++u) {
-            const short4v a =
-                *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + 16 * u + lk);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u], acc, 0, 0, 0);
-          }
-        }
-        for (; k0 < Kp; k0 += 16) {
-          const short4v bq = *reinterpret_cast<const short4v*>(wr + k0);
-          const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + lk);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
-        }
-      }
-      // 256 inputs at a time: sixteen fragments in flight, one wait
-      for (; k0 + 256 <= Kp; k0 += 256) {
+      // this tile's fragments: 64 lanes x 8 contiguous bytes per k-step
+      const short4v* wp =
+          reinterpret_cast<const short4v*>(wpk + lay.p[l]) + static_cast<size_t>(t) * nks * 64 + lane;
+      int s0 = 0;
+      // sixteen k-steps at a time: their fragments all in flight, one wait
+      for (; s0 + 16 <= nks; s0 += 16) {
         short4v bq[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) bq[u] = wfrag(W, j, k0 + 16 * u + lk, H, K);
+        for (int u = 0; u < 16; ++u) bq[u] = wp[(s0 + u) * 64];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
           const short4v a =
-              *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + 16 * u + lk);
+              *reinterpret_cast<const short4v*>(bufA + li * ld + 16 * (s0 + u) + lk);
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u], acc, 0, 0, 0);
         }
       }
-      for (; k0 + 64 <= Kp; k0 += 64) {
-        short4v bq[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) bq[u] = wfrag(W, j, k0 + 16 * u + lk, H, K);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const short4v a =
-              *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + 16 * u + lk);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u], acc, 0, 0, 0);
-        }
-      }
-      for (; k0 < Kp; k0 += 16) {
-        const short4v bq = wfrag(W, j, k0 + lk, H, K);
-        const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + k0 + lk);
+      for (; s0 < nks; ++s0) {
+        const short4v bq = wp[s0 * 64];
+        const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + 16 * s0 + lk);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
       }
       // D rows lk..lk+3, column j
@@ -445,15 +414,22 @@ int adam(float* w, const float* g, float* m, float* v, bf16* wb, size_t n, float
   return ok();
 }
 
+int pack_weights(const bf16* wb, int H, int K, bf16* out, hipStream_t st) {
+  const size_t n = static_cast<size_t>((H + 15) & ~15) * ((K + 15) & ~15);
+  const unsigned g = nb(n, 256);
+  hipLaunchKernelGGL(pack_kernel, dim3(g < 1024 ? g : 1024), dim3(256), 0, st, wb, H, K, out);
+  return ok();
+}
+
 int infer(const float* src, int stride, const int* idx, const float* mean, const float* wz,
-          int nin, int H, int rows, const bf16* wb, const float* w, const InferLayout& lay,
-          double ym, double ys, double* out, hipStream_t st) {
+          int nin, int H, int rows, const bf16* wb, const bf16* wpk, const float* w,
+          const InferLayout& lay, double ym, double ys, double* out, hipStream_t st) {
   if (rows <= 0) return 0;
   const int Hp = (H + 15) & ~15, Kin = (nin + 15) & ~15;
   const int ld = (Hp > Kin ? Hp : Kin) + 4;
   const size_t lds = 2 * kInfRows * ld * sizeof(bf16);
   hipLaunchKernelGGL(infer_kernel, dim3((rows + kInfRows - 1) / kInfRows), dim3(kInfThreads), lds, st,
-                     src, stride, idx, mean, wz, nin, H, rows, wb, w, lay, ym, ys, out);
+                     src, stride, idx, mean, wz, nin, H, rows, wb, wpk, w, lay, ym, ys, out);
   return ok();
 }
 
