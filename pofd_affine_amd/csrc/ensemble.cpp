@@ -174,42 +174,45 @@ int refresh_surrogate(pofd_ens* s) {
   return POFD_OK;
 }
 
+// the exact log likelihoods of the first stage's survivors (cprop, *npass
+// rows) into cll / cst; the 1D batch reads the count on the device
+int exact_survivors(pofd_ens* s) {
+  const uint32_t hw = s->nwalk / 2;
+  EnsDev& e = s->e;
+  if (s->target == POFD_TARGET_1D) {
+    // the survivors' count stays on the device: no host round trip
+    int rc = pofd1d_loglike_batch_device_count(s->ctx, e.cprop, hw, e.npass, s->npar, e.cll,
+                                               e.cst, s->stream);
+    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+    e.bonus_src = pofd1d_bonus_device(s->ctx);
+    e.bonus_stride = 4;
+    e.bonus_count = 2;
+    return POFD_OK;
+  }
+  int32_t npass = 0;
+  hipError_t he = hipMemcpyAsync(&npass, e.npass, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 s->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
+  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(npass)");
+  if (npass > 0) {
+    int rc = evaluate(s, e.cprop, static_cast<uint32_t>(npass), e.cll, e.cst);
+    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
+  }
+  return POFD_OK;
+}
+
 // one half-step with the delayed-acceptance first stage
 int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int store) {
   const uint32_t hw = s->nwalk / 2;
   if (pofd_mlp_predict_device(s->mlp, s->e.prop, static_cast<int>(s->npar), s->sidx,
                               static_cast<int>(hw), s->e.llsp, s->stream))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: surrogate prediction failed");
-  if (pofd_ens_dev::launch_stage1(s->e, half, step, inv_temp, s->stream) ||
-      pofd_ens_dev::launch_compact(s->e, s->stream))
+  if (pofd_ens_dev::launch_stage1(s->e, half, step, inv_temp, s->stream))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: first stage launch failed");
-  if (s->target == POFD_TARGET_1D) {
-    // the survivors' count stays on the device: no host round trip
-    int rc = pofd1d_loglike_batch_device_count(s->ctx, s->e.cprop, hw, s->e.npass,
-                                               s->npar, s->e.cll, s->e.cst, s->stream);
-    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
-    s->e.bonus_src = pofd1d_bonus_device(s->ctx);
-    s->e.bonus_stride = 4;
-    s->e.bonus_count = 2;
-    if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, s->stream))
-      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
-    if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))
-      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
-    return POFD_OK;
-  }
-  int32_t npass = 0;
-  hipError_t he = hipMemcpyAsync(&npass, s->e.npass, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                 s->stream);
-  if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
-  if (he != hipSuccess) return ehip(s, he, "hipMemcpy(npass)");
-  if (npass > 0) {
-    int rc = evaluate(s, s->e.cprop, static_cast<uint32_t>(npass), s->e.cll, s->e.cst);
-    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
-    if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, s->stream))
-      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
-  }
-  if (store && pofd_ens_dev::launch_store_half(s->e, half, s->nstored, s->stream))
-    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
+  int rc = exact_survivors(s);
+  if (rc) return rc;
+  if (pofd_ens_dev::launch_stage2(s->e, half, step, inv_temp, store, s->nstored, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: second stage launch failed");
   return POFD_OK;
 }
 
@@ -231,32 +234,13 @@ int st_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int stor
     if (pofd_ens_dev::launch_st_inner(e, half, step, inv_temp, d0 + kInnerAcceptDraw, s->stream))
       return efail(s, POFD_E_INTERNAL, "pofd_ens_run: inner chain launch failed");
   }
-  if (pofd_ens_dev::launch_stage1(e, half, step, inv_temp, s->stream, 1) ||
-      pofd_ens_dev::launch_compact(e, s->stream, e.yin))
+  if (pofd_ens_dev::launch_stage1(e, half, step, inv_temp, s->stream, 1, e.yin))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: compaction launch failed");
-  int32_t npass = static_cast<int32_t>(hw);
-  if (s->target == POFD_TARGET_1D) {
-    int rc = pofd1d_loglike_batch_device_count(s->ctx, e.cprop, hw, e.npass, s->npar, e.cll,
-                                               e.cst, s->stream);
-    if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
-    e.bonus_src = pofd1d_bonus_device(s->ctx);
-    e.bonus_stride = 4;
-    e.bonus_count = 2;
-  } else {
-    hipError_t he = hipMemcpyAsync(&npass, e.npass, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                   s->stream);
-    if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
-    if (he != hipSuccess) return ehip(s, he, "hipMemcpy(npass)");
-    if (npass > 0) {
-      int rc = evaluate(s, e.cprop, static_cast<uint32_t>(npass), e.cll, e.cst);
-      if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");
-    }
-  }
-  if (npass > 0 &&
-      pofd_ens_dev::launch_stage2(e, half, step, inv_temp, s->stream, e.yin, e.lsy, kCorrectDraw))
+  int rc = exact_survivors(s);
+  if (rc) return rc;
+  if (pofd_ens_dev::launch_stage2(e, half, step, inv_temp, store, s->nstored, s->stream, e.yin,
+                                  e.lsy, kCorrectDraw))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: correction launch failed");
-  if (store && pofd_ens_dev::launch_store_half(e, half, s->nstored, s->stream))
-    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: store launch failed");
   return POFD_OK;
 }
 

@@ -35,7 +35,7 @@ struct EnsDev {
   // delayed acceptance (surrogate first stage)
   double* llsur;     // nwalk          surrogate log likelihood of the walkers
   double* llsp;      // nwalk/2        surrogate of the proposals
-  int32_t* pass;     // nwalk/2        passed the first stage
+  int32_t* pass;     // nwalk/2        compact index of a survivor, -1 if rejected
   int32_t* cidx;     // nwalk/2        compact list of the passing proposals
   int32_t* npass;    // 1
   float* cprop;      // nwalk/2 x npar compact proposals
@@ -65,18 +65,19 @@ int launch_accept(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp
 int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st);
 // delayed acceptance: first stage on the surrogate, compaction of the
 // passing proposals, second stage on their exact likelihoods, chain store
+// (ys: the candidate rows, the proposals when null; the survivors' rows
+// are gathered into cprop by the first stage, and the second stage stores
+// the half's walkers into chain slot `slot` when `store`)
 int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st, int moved_only = 0);
-int launch_compact(const EnsDev& e, hipStream_t st, const float* ys = nullptr);
-int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st, const float* ys = nullptr, const double* lsn = nullptr,
-                  uint32_t draw = 102);
+                  hipStream_t st, int moved_only = 0, const float* ys = nullptr);
+int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp, int store,
+                  uint32_t slot, hipStream_t st, const float* ys = nullptr,
+                  const double* lsn = nullptr, uint32_t draw = 102);
 // surrogate transition: inner positions from the walkers, one inner step's
 // acceptance on the surrogate
 int launch_st_init(const EnsDev& e, uint32_t half, hipStream_t st);
 int launch_st_inner(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
                     uint32_t draw, hipStream_t st);
-int launch_store_half(const EnsDev& e, uint32_t half, uint32_t slot, hipStream_t st);
 // bonus parameters of all walkers from the batch that evaluated them in order
 int launch_bonus_init(const EnsDev& e, hipStream_t st);
 

@@ -208,7 +208,9 @@ __global__ __launch_bounds__(64) void accept_kernel(const EnsDev e, const uint32
 //   stage 2: a2 = exp(((Lnew - Lold) - (L~new - L~old))/T)
 // First stage and compaction in one workgroup: pass/fail per proposal,
 // then an order-preserving scan (wave ballots, wave offsets in LDS) that
-// lists the survivors in cidx and counts them in npass.
+// lists the survivors in cidx, counts them in npass, records each
+// proposal's compact index in pass (-1: rejected) and copies the
+// survivors' rows of ys into cprop (each by the thread that placed it).
 constexpr int kCompactThreads = 1024;
 
 // moved_only: no first stage; the walkers whose surrogate-transition inner
@@ -217,7 +219,8 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
                                                                  const uint32_t half,
                                                                  const uint32_t step,
                                                                  const double inv_temp,
-                                                                 const int moved_only) {
+                                                                 const int moved_only,
+                                                                 const float* ys) {
   __shared__ int woff[kCompactThreads / 64 + 1];
   const int hw = e.nwalk / 2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -240,7 +243,6 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
         }
       }
     }
-    if (t < hw) e.pass[t] = pass;
     const unsigned long long bal = __ballot(pass);
     if (lane == 0) woff[wave] = __popcll(bal);
     __syncthreads();
@@ -254,9 +256,17 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
       woff[kCompactThreads / 64] = acc;
     }
     __syncthreads();
-    if (pass) {
-      const unsigned long long below = (lane == 0) ? 0ull : (bal & (~0ull >> (64 - lane)));
-      e.cidx[base + woff[wave] + __popcll(below)] = t;
+    if (t < hw) {
+      int c = -1;
+      if (pass) {
+        const unsigned long long below = (lane == 0) ? 0ull : (bal & (~0ull >> (64 - lane)));
+        c = base + woff[wave] + __popcll(below);
+        e.cidx[c] = t;
+        const float* src = ys + static_cast<size_t>(t) * e.npar;
+        float* dst = e.cprop + static_cast<size_t>(c) * e.npar;
+        for (int i = 0; i < e.npar; ++i) dst[i] = src[i];
+      }
+      e.pass[t] = c;
     }
     base += woff[kCompactThreads / 64];
     __syncthreads();
@@ -267,42 +277,43 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
   }
 }
 
-// the survivors' rows of ys (the proposals, or the inner chains' ends)
-__global__ __launch_bounds__(64) void gather_pass_kernel(const EnsDev e, const float* ys) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= *e.npass) return;
-  const int t = e.cidx[c];
-  const float* src = ys + static_cast<size_t>(t) * e.npar;
-  float* dst = e.cprop + static_cast<size_t>(c) * e.npar;
-  for (int i = 0; i < e.npar; ++i) dst[i] = src[i];
-}
-
-// the candidate of proposal slot t is row t of ys with surrogate lsn[t]
+// one thread per walker t of the half: a survivor (compact index pass[t])
+// is accepted or not on its exact log likelihood (row pass[t] of cll),
+// its candidate row t of ys with surrogate lsn[t]; then, when storing, the
+// walker's state goes into chain slot `slot`
 __global__ __launch_bounds__(64) void stage2_kernel(const EnsDev e, const uint32_t half,
                                                     const uint32_t step, const double inv_temp,
                                                     const float* ys, const double* lsn,
-                                                    const uint32_t draw) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= *e.npass) return;
+                                                    const uint32_t draw, const int store,
+                                                    const uint32_t slot) {
   const int hw = e.nwalk / 2;
-  const int t = e.cidx[c];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hw) return;
   const int k = static_cast<int>(half) * hw + t;
-  const int st = e.cst[c];
-  const double lnew = e.cll[c];
-  if ((st != 0 && st != kBadLike) || isnan(lnew) || isinf(lnew)) {
-    e.nerr[k] += 1;
-    return;
-  }
-  const double a2 = exp(inv_temp * ((lnew - e.ll[k]) - (lsn[t] - e.llsur[k])));
-  const bool accept = (a2 >= 1.0) || (draws(e, step, half, k, draw).u0 < a2);
-  if (!accept) return;
   float* xo = e.pos + static_cast<size_t>(k) * e.npar;
-  const float* y = ys + static_cast<size_t>(t) * e.npar;
-  for (int i = 0; i < e.npar; ++i) xo[i] = y[i];
-  fill_bonus(e, xo, c);
-  e.ll[k] = lnew;
-  e.llsur[k] = lsn[t];
-  e.naccept[k] += 1;
+  const int c = e.pass[t];
+  if (c >= 0) {
+    const int st = e.cst[c];
+    const double lnew = e.cll[c];
+    if ((st != 0 && st != kBadLike) || isnan(lnew) || isinf(lnew)) {
+      e.nerr[k] += 1;
+    } else {
+      const double a2 = exp(inv_temp * ((lnew - e.ll[k]) - (lsn[t] - e.llsur[k])));
+      if ((a2 >= 1.0) || (draws(e, step, half, k, draw).u0 < a2)) {
+        const float* y = ys + static_cast<size_t>(t) * e.npar;
+        for (int i = 0; i < e.npar; ++i) xo[i] = y[i];
+        fill_bonus(e, xo, c);
+        e.ll[k] = lnew;
+        e.llsur[k] = lsn[t];
+        e.naccept[k] += 1;
+      }
+    }
+  }
+  if (store) {
+    float* ch = e.chain + (static_cast<size_t>(slot) * e.nwalk + k) * e.npar;
+    for (int i = 0; i < e.npar; ++i) ch[i] = xo[i];
+    e.chainll[static_cast<size_t>(slot) * e.nwalk + k] = e.ll[k];
+  }
 }
 
 // ---- surrogate transition (Liu, Monte Carlo Strategies in Scientific
@@ -352,18 +363,6 @@ __global__ __launch_bounds__(64) void bonus_init_kernel(const EnsDev e) {
   if (k < e.nwalk) fill_bonus(e, e.pos + static_cast<size_t>(k) * e.npar, k);
 }
 
-__global__ __launch_bounds__(64) void store_half_kernel(const EnsDev e, const uint32_t half,
-                                                        const uint32_t slot) {
-  const int hw = e.nwalk / 2;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= hw) return;
-  const int k = static_cast<int>(half) * hw + t;
-  const float* xo = e.pos + static_cast<size_t>(k) * e.npar;
-  float* c = e.chain + (static_cast<size_t>(slot) * e.nwalk + k) * e.npar;
-  for (int i = 0; i < e.npar; ++i) c[i] = xo[i];
-  e.chainll[static_cast<size_t>(slot) * e.nwalk + k] = e.ll[k];
-}
-
 // z draws for tests of generateZ's distribution
 __global__ __launch_bounds__(256) void zsample_kernel(const EnsDev e, const uint32_t n,
                                                       double* out) {
@@ -404,24 +403,18 @@ int launch_zsample(const EnsDev& e, uint32_t n, double* out, hipStream_t st) {
 }
 
 int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st, int moved_only) {
+                  hipStream_t st, int moved_only, const float* ys) {
   hipLaunchKernelGGL(stage1_kernel, dim3(1), dim3(kCompactThreads), 0, st, e, half, step,
-                     inv_temp, moved_only);
+                     inv_temp, moved_only, ys ? ys : e.prop);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_compact(const EnsDev& e, hipStream_t st, const float* ys) {
-  const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(gather_pass_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e,
-                     ys ? ys : e.prop);
-  return hipGetLastError() == hipSuccess ? 0 : 1;
-}
-
-int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                  hipStream_t st, const float* ys, const double* lsn, uint32_t draw) {
+int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp, int store,
+                  uint32_t slot, hipStream_t st, const float* ys, const double* lsn,
+                  uint32_t draw) {
   const int hw = e.nwalk / 2;
   hipLaunchKernelGGL(stage2_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
-                     inv_temp, ys ? ys : e.prop, lsn ? lsn : e.llsp, draw);
+                     inv_temp, ys ? ys : e.prop, lsn ? lsn : e.llsp, draw, store, slot);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -444,10 +437,6 @@ int launch_bonus_init(const EnsDev& e, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_store_half(const EnsDev& e, uint32_t half, uint32_t slot, hipStream_t st) {
-  const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(store_half_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, slot);
-  return hipGetLastError() == hipSuccess ? 0 : 1;
-}
+
 
 }  // namespace pofd_ens_dev
