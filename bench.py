@@ -598,7 +598,7 @@ def bench_da(args, inp):
     npar, nk = th0.shape[1], 8
     # workgroups per walker measured for the batch sizes a half-step meets
     # (the plain chain's 256 proposals, the delayed-acceptance survivors)
-    tune_sizes = [16, 32, 48, 64, 80, 96, 128, 192, 256]
+    tune_sizes = [16, 32, 48, 56, 64, 72, 80, 96, 128, 192, 256]
     t0 = time.perf_counter()
     tuned = ctx.autotune_parts(th0, tune_sizes, reps=5)
     t_tune = time.perf_counter() - t0
