@@ -96,6 +96,56 @@ def test_two_beamsets_two_datasets_1d():
     ctx.set_parts(0)
 
 
+def test_two_beamsets_one_binned_dataset_each_1d():
+    """A beam set with a single binned dataset does its dataset sum in the
+    finalize kernel (no combine launch); the second beam set adds to the
+    first's log likelihood.  Against the oracle at 1e-6, with a walker
+    whose knot value is NaN (invalid model in both beam sets) and one with
+    a negative sigma multiplier (bad_like, 1e25 per beam set,
+    src/calcLike.cc:403), and the same bits for every walker split."""
+    gb, gps = synth.gaussian_beam()
+    nb, nps = fitsio.read_beam(os.path.join(REFDATA,
+                                            "band12_beam_f100.fits[0]"))
+    d0, _ = synth.make_map(200, 0.0025, seed=11)
+    d1, _ = synth.make_map(160, 0.003, seed=12)
+    knots = synth.knot_positions()
+    th = synth.walker_thetas(8, seed=23)
+    th[3, 2] = np.nan
+    th[5, 8] = -1.0
+    ctx = _lib.Context(0)
+    ctx.configure(NF, NI, knots, 0.0)
+    L = O.Like(NF, NI, knots)
+    keep = []
+    for beam, ps, d, sig, norm in ((gb.ravel(), gps, d0, 0.0025, 1.0),
+                                   (nb, nps, d1, 0.003, 2.0)):
+        gbm = _lib.Beam(beam, ps, 1e-6, 120)
+        obm = O.Beam(beam, ps, 1e-6, 120)
+        keep += [gbm, obm]
+        bg = ctx.add_beamset(gbm)
+        bo = L.add_beamset(obm)
+        bins, c0, dl = _lib.data_bin(d, 1000)
+        ctx.add_field(bg, bins=bins, bincent0=c0, bindelta=dl,
+                      sigma_base=sig, like_norm=norm, npix=d.size)
+        L.add_field(bo, bins=bins, bincent0=c0, bindelta=dl,
+                    sigma_base=sig, like_norm=norm, npix=d.size)
+    ctx.prepare()
+    L.finalize(th[0])
+    _say("1D: 2 beam sets x 1 binned dataset, 8 walkers")
+    ll, st = ctx.loglike_batch(th)
+    ref, rst = L.loglike_batch(th, 8)
+    ok = [w for w in range(8) if w not in (3, 5)]
+    assert np.all(st[ok] == 0) and np.all(rst[ok] == 0), (st, rst)
+    np.testing.assert_allclose(ll[ok], ref[ok], rtol=0, atol=1e-6)
+    assert st[3] == 2 and rst[3] == 2 and np.isnan(ll[3])  # invalid model
+    assert st[5] == 4 and ll[5] == 2e25 and ref[5] == 2e25  # bad_like x 2
+    for parts in (1, 2, 4):
+        ctx.set_parts(parts)
+        llp, stp = ctx.loglike_batch(th)
+        np.testing.assert_array_equal(stp, st)
+        np.testing.assert_array_equal(llp[ok], ll[ok])
+    ctx.set_parts(0)
+
+
 def test_two_beamsets_two_datasets_2d():
     n, nedge = 512, 64
     th0 = synth2d.true_theta()
