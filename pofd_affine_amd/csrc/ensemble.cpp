@@ -204,6 +204,8 @@ int exact_survivors(pofd_ens* s) {
 // one half-step with the delayed-acceptance first stage
 int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int store) {
   const uint32_t hw = s->nwalk / 2;
+  if (pofd_ens_dev::launch_propose(s->e, half, step, s->stream))
+    return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
   if (pofd_mlp_predict_device(s->mlp, s->e.prop, static_cast<int>(s->npar), s->sidx,
                               static_cast<int>(hw), s->e.llsp, s->stream))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: surrogate prediction failed");
@@ -220,18 +222,16 @@ int da_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int stor
 // walker on the surrogate (the other half fixed), then one exact
 // correction of the walkers whose inner chain moved
 int st_half(pofd_ens* s, uint32_t half, uint32_t step, double inv_temp, int store) {
-  const uint32_t hw = s->nwalk / 2;
   EnsDev& e = s->e;
   if (pofd_ens_dev::launch_st_init(e, half, s->stream))
     return efail(s, POFD_E_INTERNAL, "pofd_ens_run: inner chain launch failed");
+  pofd_mlp_dev::InferArgs ia;
+  pofd_mlp_infer_args(s->mlp, s->sidx, &ia);
   for (uint32_t i = 0; i < s->st_steps; ++i) {
     const uint32_t d0 = kInnerDraw0 + kInnerDrawStride * i;
-    if (pofd_ens_dev::launch_propose(e, half, step, s->stream, e.yin, d0))
-      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
-    if (pofd_mlp_predict_device(s->mlp, e.prop, static_cast<int>(s->npar), s->sidx,
-                                static_cast<int>(hw), e.llsp, s->stream))
-      return efail(s, POFD_E_INTERNAL, "pofd_ens_run: surrogate prediction failed");
-    if (pofd_ens_dev::launch_st_inner(e, half, step, inv_temp, d0 + kInnerAcceptDraw, s->stream))
+    // the inner step's proposals, predictions and acceptance in one launch
+    if (pofd_ens_dev::launch_inner_step(e, half, step, e.yin, d0, ia, inv_temp,
+                                        d0 + kInnerAcceptDraw, s->stream))
       return efail(s, POFD_E_INTERNAL, "pofd_ens_run: inner chain launch failed");
   }
   if (pofd_ens_dev::launch_stage1(e, half, step, inv_temp, s->stream, 1, e.yin))
@@ -445,6 +445,7 @@ int pofd_ens_set_surrogate(pofd_ens* s, pofd_mlp* mlp, const uint32_t* param_ind
     s->e.cst = static_cast<int32_t*>(s->dabuf[7]);
     s->e.nexact = static_cast<unsigned long long*>(s->dabuf[8]);
   }
+
   std::vector<int> idx(param_index, param_index + nidx);
   if (s->sidx) (void)hipFree(s->sidx);
   s->sidx = nullptr;
@@ -542,14 +543,15 @@ int pofd_ens_run(pofd_ens* s, uint32_t nsteps, double temperature, uint32_t thin
         if (rc) return rc;
         continue;
       }
-      if (pofd_ens_dev::launch_propose(s->e, half, step, s->stream))
-        return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
-      s->n_proposed += hw;
-      if (s->use_da) {
+      if (s->use_da) {  // proposals, surrogate, both stages
+        s->n_proposed += hw;
         int rc = da_half(s, half, step, inv_temp, store);
         if (rc) return rc;
         continue;
       }
+      if (pofd_ens_dev::launch_propose(s->e, half, step, s->stream))
+        return efail(s, POFD_E_INTERNAL, "pofd_ens_run: propose launch failed");
+      s->n_proposed += hw;
       s->n_exact += hw;
       int rc = evaluate(s, s->e.prop, hw, s->e.llp, s->e.stp);
       if (rc) return efail(s, rc, "pofd_ens_run: likelihood launch failed");

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mlp_internal.h"
+
 namespace pofd_ens_dev {
 
 constexpr int kMaxPar = 64;
@@ -73,11 +75,14 @@ int launch_stage1(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp
 int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp, int store,
                   uint32_t slot, hipStream_t st, const float* ys = nullptr,
                   const double* lsn = nullptr, uint32_t draw = 102);
-// surrogate transition: inner positions from the walkers, one inner step's
-// acceptance on the surrogate
+// one inner step of the surrogate transition in one launch: the proposals
+// from the inner positions `from` (draws from draw0), their surrogate
+// predictions and their inner acceptance (uniform draw adraw)
+int launch_inner_step(const EnsDev& e, uint32_t half, uint32_t step, const float* from,
+                      uint32_t draw0, const pofd_mlp_dev::InferArgs& a, double inv_temp,
+                      uint32_t adraw, hipStream_t st);
+// surrogate transition: the inner positions from the walkers
 int launch_st_init(const EnsDev& e, uint32_t half, hipStream_t st);
-int launch_st_inner(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                    uint32_t draw, hipStream_t st);
 // bonus parameters of all walkers from the batch that evaluated them in order
 int launch_bonus_init(const EnsDev& e, hipStream_t st);
 

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ensemble_internal.h"
+#include "mlp_infer.h"
 
 namespace pofd_ens_dev {
 
@@ -100,15 +101,12 @@ __device__ __forceinline__ void fill_bonus(const EnsDev& e, float* x, int r) {
 
 }  // namespace
 
-// one thread per walker of half `half`; the walker's position is its
-// current one, or row t of `from` (the surrogate-transition inner chain);
-// its draws are numbered from draw0
-__global__ __launch_bounds__(64) void propose_kernel(const EnsDev e, const uint32_t half,
-                                                     const uint32_t step, const float* from,
-                                                     const uint32_t draw0) {
+// the proposal of walker t of half `half`: its position is its current
+// one, or row t of `from` (the surrogate-transition inner chain); its
+// draws are numbered from draw0
+__device__ void propose_one(const EnsDev& e, const uint32_t half, const uint32_t step,
+                            const int t, const float* from, const uint32_t draw0) {
   const int hw = e.nwalk / 2;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= hw) return;
   const int k = static_cast<int>(half) * hw + t;       // walker updated
   const int other = (half == 0) ? hw : 0;              // partner half
   const Draws d0 = draws(e, step, half, k, draw0);
@@ -137,6 +135,14 @@ __global__ __launch_bounds__(64) void propose_kernel(const EnsDev e, const uint3
   }
   e.z[t] = z;
   e.pvalid[t] = ok ? 1 : 0;
+}
+
+// one thread per walker of half `half`
+__global__ __launch_bounds__(64) void propose_kernel(const EnsDev e, const uint32_t half,
+                                                     const uint32_t step, const float* from,
+                                                     const uint32_t draw0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < e.nwalk / 2) propose_one(e, half, step, t, from, draw0);
 }
 
 // diagonal Gaussian test target: L = -1/2 sum ((x - mu)/sigma)^2 over the
@@ -214,13 +220,10 @@ __global__ __launch_bounds__(64) void accept_kernel(const EnsDev e, const uint32
 constexpr int kCompactThreads = 1024;
 
 // moved_only: no first stage; the walkers whose surrogate-transition inner
-// chain moved are the survivors
-__global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
-                                                                 const uint32_t half,
-                                                                 const uint32_t step,
-                                                                 const double inv_temp,
-                                                                 const int moved_only,
-                                                                 const float* ys) {
+// chain moved are the survivors.  Run by all kCompactThreads threads of
+// one workgroup.
+__device__ void stage1_body(const EnsDev& e, const uint32_t half, const uint32_t step,
+                            const double inv_temp, const int moved_only, const float* ys) {
   __shared__ int woff[kCompactThreads / 64 + 1];
   const int hw = e.nwalk / 2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -275,6 +278,15 @@ __global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
     *e.npass = base;
     *e.nexact += static_cast<unsigned long long>(base);
   }
+}
+
+__global__ __launch_bounds__(kCompactThreads) void stage1_kernel(const EnsDev e,
+                                                                 const uint32_t half,
+                                                                 const uint32_t step,
+                                                                 const double inv_temp,
+                                                                 const int moved_only,
+                                                                 const float* ys) {
+  stage1_body(e, half, step, inv_temp, moved_only, ys);
 }
 
 // one thread per walker t of the half: a survivor (compact index pass[t])
@@ -337,14 +349,12 @@ __global__ __launch_bounds__(64) void st_init_kernel(const EnsDev e, const uint3
   e.moved[t] = 0;
 }
 
-// inner step: the proposals (from the inner positions) against them on the
-// surrogate predictions llsp; its uniform is draw `draw`
-__global__ __launch_bounds__(64) void st_inner_kernel(const EnsDev e, const uint32_t half,
-                                                      const uint32_t step, const double inv_temp,
-                                                      const uint32_t draw) {
+// inner step of walker t: its proposal (from the inner position) against
+// it on the surrogate predictions llsp; the uniform is draw `draw`
+__device__ void st_inner_one(const EnsDev& e, const uint32_t half, const uint32_t step,
+                             const double inv_temp, const int t, const uint32_t draw) {
   const int hw = e.nwalk / 2;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= hw || !e.pvalid[t]) return;
+  if (!e.pvalid[t]) return;
   const int k = static_cast<int>(half) * hw + t;
   const double ls = e.llsp[t];
   if (isnan(ls) || isinf(ls)) return;
@@ -356,6 +366,30 @@ __global__ __launch_bounds__(64) void st_inner_kernel(const EnsDev e, const uint
   for (int i = 0; i < e.npar; ++i) y[i] = p[i];
   e.lsy[t] = ls;
   e.moved[t] = 1;
+}
+
+// ---- one inner step of the surrogate transition in one launch: workgroup
+// g proposes rows 16 g .. 16 g + 15 of the half from the inner positions
+// `from`, predicts them with the surrogate (the network's fused inference,
+// mlp_infer.h) and runs their inner acceptance.  (Delayed acceptance keeps
+// separate launches: its first stage needs every workgroup's predictions,
+// and a device-wide handoff inside one launch costs more than the launch
+// boundary.)
+__global__ __launch_bounds__(pofd_mlp_dev::kInfThreads) void inner_step_kernel(
+    const EnsDev e, const uint32_t half, const uint32_t step, const float* from,
+    const uint32_t draw0, const pofd_mlp_dev::InferArgs a, const double inv_temp,
+    const uint32_t adraw) {
+  constexpr int R = pofd_mlp_dev::kInfRows;
+  const int hw = e.nwalk / 2;
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * R;
+  if (tid < R && r0 + tid < hw) propose_one(e, half, step, r0 + tid, from, draw0);
+  __syncthreads();  // the rows, written to global memory, visible to the workgroup
+  pofd_mlp_dev::infer_rows(
+      a, r0, hw, [&](int r, int c) { return e.prop[static_cast<size_t>(r) * e.npar + c]; },
+      e.llsp);
+  __syncthreads();
+  if (tid < R && r0 + tid < hw) st_inner_one(e, half, step, inv_temp, r0 + tid, adraw);
 }
 
 __global__ __launch_bounds__(64) void bonus_init_kernel(const EnsDev e) {
@@ -418,17 +452,20 @@ int launch_stage2(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_st_init(const EnsDev& e, uint32_t half, hipStream_t st) {
+int launch_inner_step(const EnsDev& e, uint32_t half, uint32_t step, const float* from,
+                      uint32_t draw0, const pofd_mlp_dev::InferArgs& a, double inv_temp,
+                      uint32_t adraw, hipStream_t st) {
   const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(st_init_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half);
+  const int R = pofd_mlp_dev::kInfRows;
+  hipLaunchKernelGGL(inner_step_kernel, dim3((hw + R - 1) / R), dim3(pofd_mlp_dev::kInfThreads),
+                     pofd_mlp_dev::infer_lds_bytes(a.nin, a.H), st, e, half, step, from, draw0,
+                     a, inv_temp, adraw);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_st_inner(const EnsDev& e, uint32_t half, uint32_t step, double inv_temp,
-                    uint32_t draw, hipStream_t st) {
+int launch_st_init(const EnsDev& e, uint32_t half, hipStream_t st) {
   const int hw = e.nwalk / 2;
-  hipLaunchKernelGGL(st_inner_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half, step,
-                     inv_temp, draw);
+  hipLaunchKernelGGL(st_init_kernel, dim3((hw + 63) / 64), dim3(64), 0, st, e, half);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

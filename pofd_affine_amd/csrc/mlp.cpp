@@ -173,16 +173,31 @@ int repack(pofd_mlp* s) {
 
 // used by the ensemble sampler: predictions (log likelihood units) of the
 // rows of src (stride floats per row, inputs at idx) into out, on stream st
+void pofd_mlp_infer_args(const pofd_mlp* s, const int* idx, md::InferArgs* a) {
+  a->idx = idx;
+  a->mean = s->xmean;
+  a->wz = s->xw;
+  a->nin = s->nin;
+  a->H = s->H;
+  a->wb = s->wb;
+  a->wpk = s->wpk;
+  a->w = s->w;
+  for (int l = 0; l < 4; ++l) {
+    a->lay.w[l] = s->off[2 * l];
+    a->lay.b[l] = s->off[2 * l + 1];
+  }
+  for (int l = 0; l < 3; ++l) a->lay.p[l] = s->pk_off[l];
+  a->ym = s->ymean;
+  a->ys = s->ystd;
+}
+
+// used by the ensemble sampler: predictions (log likelihood units) of the
+// rows of src (stride floats per row, inputs at idx) into out, on stream st
 int pofd_mlp_predict_device(pofd_mlp* s, const float* src, int stride, const int* idx, int rows,
                             double* out, hipStream_t st) {
-  md::InferLayout lay;
-  for (int l = 0; l < 4; ++l) {
-    lay.w[l] = s->off[2 * l];
-    lay.b[l] = s->off[2 * l + 1];
-  }
-  for (int l = 0; l < 3; ++l) lay.p[l] = s->pk_off[l];
-  return md::infer(src, stride, idx, s->xmean, s->xw, s->nin, s->H, rows, s->wb, s->wpk, s->w,
-                   lay, s->ymean, s->ystd, out, st);
+  md::InferArgs a;
+  pofd_mlp_infer_args(s, idx, &a);
+  return md::infer(src, stride, a, rows, out, st);
 }
 
 extern "C" {

@@ -52,14 +52,31 @@ struct InferLayout {
 // W[16 t + l % 16][16 s + 4 (l / 16) + q], zero past H and K (Hp x Kp
 // values, Hp and Kp the 16-multiples at or above H and K)
 int pack_weights(const bf16* wb, int H, int K, bf16* out, hipStream_t st);
+// what the fused inference reads: the input columns idx, the whitening
+// (mean, wz), the weights (hidden layers from the packed copy wpk, the
+// output layer from wb, biases from w) and the target standardisation
+struct InferArgs {
+  const int* idx;
+  const float* mean;
+  const float* wz;
+  int nin, H;
+  const bf16* wb;
+  const bf16* wpk;
+  const float* w;
+  InferLayout lay;
+  double ym, ys;
+};
 // fused prediction of rows inputs gathered from src (stride floats per row,
-// inputs at idx): out[r] = net(whitened x_r) * ys + ym; the hidden layers'
-// weights from the packed copy wpk, the output layer's from wb
-int infer(const float* src, int stride, const int* idx, const float* mean, const float* wz,
-          int nin, int H, int rows, const bf16* wb, const bf16* wpk, const float* w,
-          const InferLayout& lay, double ym, double ys, double* out, hipStream_t st);
+// inputs at a.idx): out[r] = net(whitened x_r) * ys + ym
+int infer(const float* src, int stride, const InferArgs& a, int rows, double* out,
+          hipStream_t st);
 
 // out[i] = pred[i] * ys + ym (double)
 int unstandardize(const float* pred, int n, double ym, double ys, double* out, hipStream_t st);
 
 }  // namespace pofd_mlp_dev
+
+// the inference arguments of a network (mlp.cpp), its inputs the columns idx
+// (device) of the rows it is given
+struct pofd_mlp;
+void pofd_mlp_infer_args(const pofd_mlp* s, const int* idx, pofd_mlp_dev::InferArgs* a);

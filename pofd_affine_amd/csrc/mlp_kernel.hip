@@ -4,32 +4,12 @@
 // fused inference of the whole network.
 #include <hip/hip_runtime.h>
 
+#include "mlp_infer.h"
 #include "mlp_internal.h"
 
 namespace pofd_mlp_dev {
 
 namespace {
-
-__device__ __forceinline__ float bf2f(bf16 v) {
-  return __uint_as_float(static_cast<uint32_t>(v) << 16);
-}
-
-__device__ __forceinline__ bf16 f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<bf16>((u >> 16) | 0x40);  // NaN
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-  return static_cast<bf16>(u >> 16);
-}
-
-// whitened input c of one row: sum_j wz[c][j] (x_j - mean_j), float, j in
-// order (a diagonal wz gives wz[c][c] (x_c - mean_c) exactly)
-template <typename F>
-__device__ __forceinline__ float whiten(F xj, const float* mean, const float* wz, int nin,
-                                        int c) {
-  float acc = 0.0f;
-  for (int j = 0; j < nin; ++j) acc = fmaf(wz[c * nin + j], xj(j) - mean[j], acc);
-  return acc;
-}
 
 template <int T>
 __device__ __forceinline__ float ld(const void* p, size_t i) {
@@ -217,23 +197,6 @@ __global__ void unstd_kernel(const float* pred, int n, double ym, double ys, dou
   if (i < n) out[i] = static_cast<double>(pred[i]) * ys + ym;
 }
 
-// ---- fused inference: gather + standardise, three GELU layers on the
-// matrix cores, output layer, de-standardisation, in one launch.  A
-// workgroup of sixteen waves takes 16 rows; the activations stay in LDS as
-// bfloat16 and the weights stream from L2.  Each wave owns 16-column tiles
-// of the layer output (tiles w, w+16, ...) and accumulates them with
-// v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)..+3] and
-// B[4(l/16)..+3][l%16] and receives D[4(l/16)..+3][l%16].  The weight
-// fragments of sixteen k-steps are loaded before their products, so a wave
-// waits on memory once per 256 inputs (the weights are usually evicted
-// from L2 by the likelihood kernels between two predictions).
-typedef short short4v __attribute__((ext_vector_type(4)));
-typedef float float4v __attribute__((ext_vector_type(4)));
-constexpr int kInfRows = 16;
-constexpr int kMaxIn = 64;  // network inputs (pofd_mlp_create's limit)
-constexpr int kInfThreads = 1024;
-constexpr int kInfWaves = kInfThreads / 64;
-
 __global__ __launch_bounds__(256) void pack_kernel(const bf16* wb, int H, int K, bf16* out) {
   const int Hp = (H + 15) & ~15, Kp = (K + 15) & ~15, nks = Kp / 16;
   const size_t n = static_cast<size_t>(Hp) * Kp;
@@ -247,100 +210,11 @@ __global__ __launch_bounds__(256) void pack_kernel(const bf16* wb, int H, int K,
   }
 }
 
-__global__ __launch_bounds__(kInfThreads) void infer_kernel(
-    const float* src, int stride, const int* idx, const float* mean, const float* wz, int nin,
-    int H, int rows, const bf16* wb, const bf16* wpk, const float* w, InferLayout lay,
-    double ym, double ys, double* out) {
-  extern __shared__ bf16 act[];  // two buffers of kInfRows x ld bf16
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r0 = blockIdx.x * kInfRows;
-  const int Hp = (H + 15) & ~15;
-  const int Kin = (nin + 15) & ~15;
-  const int ld = (Hp > Kin ? Hp : Kin) + 4;  // row pitch (bf16), 8-byte aligned
-  bf16* bufA = act;
-  bf16* bufB = act + kInfRows * ld;
-  // inputs: the column map, the whitening and the rows' raw values staged
-  // in LDS first (two dependent global reads in all), then whitened there
-  __shared__ int sidx[kMaxIn];
-  __shared__ float smean[kMaxIn];
-  __shared__ float swz[kMaxIn * kMaxIn];
-  __shared__ float sx[kInfRows * kMaxIn];
-  for (int e = tid; e < nin * nin; e += kInfThreads) swz[e] = wz[e];
-  if (tid < nin) {
-    sidx[tid] = idx[tid];
-    smean[tid] = mean[tid];
-  }
-  __syncthreads();
-  for (int e = tid; e < kInfRows * nin; e += kInfThreads) {
-    const int i = e / nin, j = e % nin;
-    const int r = r0 + i;
-    sx[e] = (r < rows) ? src[static_cast<size_t>(r) * stride + sidx[j]] : 0.0f;
-  }
-  __syncthreads();
-  for (int e = tid; e < kInfRows * Kin; e += kInfThreads) {
-    const int i = e / Kin, c = e % Kin;
-    float v = 0.0f;
-    if (c < nin && r0 + i < rows) v = whiten([&](int j) { return sx[i * nin + j]; }, smean, swz, nin, c);
-    bufA[i * ld + c] = f2bf(v);
-  }
-  __syncthreads();
-  const int li = lane & 15, lk = (lane >> 4) * 4;
-  const int ntile = Hp / 16;
-  for (int l = 0; l < 3; ++l) {
-    const int Kp = (l == 0) ? Kin : Hp;
-    const int nks = Kp / 16;
-    const float* b = w + lay.b[l];
-    for (int t = wave; t < ntile; t += kInfWaves) {
-      const int j = t * 16 + li;  // output column this lane feeds B for
-      float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      // this tile's fragments: 64 lanes x 8 contiguous bytes per k-step
-      const short4v* wp =
-          reinterpret_cast<const short4v*>(wpk + lay.p[l]) + static_cast<size_t>(t) * nks * 64 + lane;
-      int s0 = 0;
-      // sixteen k-steps at a time: their fragments all in flight, one wait
-      for (; s0 + 16 <= nks; s0 += 16) {
-        short4v bq[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) bq[u] = wp[(s0 + u) * 64];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const short4v a =
-              *reinterpret_cast<const short4v*>(bufA + li * ld + 16 * (s0 + u) + lk);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u], acc, 0, 0, 0);
-        }
-      }
-      for (; s0 < nks; ++s0) {
-        const short4v bq = wp[s0 * 64];
-        const short4v a = *reinterpret_cast<const short4v*>(bufA + li * ld + 16 * s0 + lk);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
-      }
-      // D rows lk..lk+3, column j
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float v = 0.0f;
-        if (j < H) {
-          const float z = acc[q] + b[j];
-          v = 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
-        }
-        bufB[(lk + q) * ld + t * 16 + li] = f2bf(v);
-      }
-    }
-    __syncthreads();
-    bf16* tmp = bufA;
-    bufA = bufB;
-    bufB = tmp;
-  }
-  // output layer: 64 threads (one wave) per row
-  {
-    const int i = wave, s = lane;
-    const bf16* W4 = wb + lay.w[3];
-    float acc = 0.0f;
-    for (int j = s; j < H; j += 64) acc += bf2f(bufA[i * ld + j]) * bf2f(W4[j]);
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
-    const int r = r0 + i;
-    if (s == 0 && r < rows) out[r] = static_cast<double>(acc + w[lay.b[3]]) * ys + ym;
-  }
+__global__ __launch_bounds__(kInfThreads) void infer_kernel(const float* src, int stride,
+                                                             InferArgs a, int rows, double* out) {
+  infer_rows(
+      a, blockIdx.x * kInfRows, rows,
+      [&](int r, int c) { return src[static_cast<size_t>(r) * stride + c]; }, out);
 }
 
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
@@ -421,15 +295,11 @@ int pack_weights(const bf16* wb, int H, int K, bf16* out, hipStream_t st) {
   return ok();
 }
 
-int infer(const float* src, int stride, const int* idx, const float* mean, const float* wz,
-          int nin, int H, int rows, const bf16* wb, const bf16* wpk, const float* w,
-          const InferLayout& lay, double ym, double ys, double* out, hipStream_t st) {
+int infer(const float* src, int stride, const InferArgs& a, int rows, double* out,
+          hipStream_t st) {
   if (rows <= 0) return 0;
-  const int Hp = (H + 15) & ~15, Kin = (nin + 15) & ~15;
-  const int ld = (Hp > Kin ? Hp : Kin) + 4;
-  const size_t lds = 2 * kInfRows * ld * sizeof(bf16);
-  hipLaunchKernelGGL(infer_kernel, dim3((rows + kInfRows - 1) / kInfRows), dim3(kInfThreads), lds, st,
-                     src, stride, idx, mean, wz, nin, H, rows, wb, wpk, w, lay, ym, ys, out);
+  hipLaunchKernelGGL(infer_kernel, dim3((rows + kInfRows - 1) / kInfRows), dim3(kInfThreads),
+                     infer_lds_bytes(a.nin, a.H), st, src, stride, a, rows, out);
   return ok();
 }
 
