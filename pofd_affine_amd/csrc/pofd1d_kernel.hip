@@ -1786,14 +1786,14 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
       curr = 0.69314718055994530942 * block_sum(part, red);  // log2toe
     } else {
       __syncthreads();  // every load of the split search precedes the rewrite
-      // four values a step per thread (N is a multiple of 4 x kThreads):
+      // eight values a step per thread (N is a multiple of 8 x kThreads):
       // their loads issued together
-      for (int j = tid; j < N; j += 4 * kThreads) {
-        double v[4];
+      for (int j = tid; j < N; j += 8 * kThreads) {
+        double v[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = pofd[j + u * kThreads];
+        for (int u = 0; u < 8; ++u) v[u] = pofd[j + u * kThreads];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pofd[j + u * kThreads] = log2_or_small(v[u] * itot);
+        for (int u = 0; u < 8; ++u) pofd[j + u * kThreads] = log2_or_small(v[u] * itot);
       }
     }
   }
@@ -1828,8 +1828,9 @@ __global__ __launch_bounds__(kThreads) void finalize1d_kernel(const BeamsetDev b
 // pixels: unbinned log likelihood, kPixBlocks workgroups per (walker, field)
 // ===========================================================================
 constexpr int kPixThreads = 256;
+constexpr int kPixU = 8;  // pixels per thread per step
 
-__global__ __launch_bounds__(kPixThreads) void pixels1d_kernel(const BeamsetDev bs,
+__global__ __launch_bounds__(kPixThreads, 8) void pixels1d_kernel(const BeamsetDev bs,
                                                                 const WorkDev wk) {
   __shared__ double red[kPixThreads / 64];
   const int blk = blockIdx.x;
@@ -1856,39 +1857,60 @@ __global__ __launch_bounds__(kPixThreads) void pixels1d_kernel(const BeamsetDev 
     const double maxflux = minflux + static_cast<double>(N - 1) * dflux;
     const double idflux = 1.0 / dflux;
     const double maxfluxval = maxflux - dflux;
-    auto Lr = [&](int i) {
-      int jj = i + split;
-      if (jj >= N) jj -= N;
-      return L[jj];
+    // rotated index i -> i + split mod N (N is a power of two), as a 32-bit
+    // byte offset from the walker's table
+    const char* Lb = reinterpret_cast<const char*>(L);
+    const uint32_t nm1 = static_cast<uint32_t>(N - 1), usplit = static_cast<uint32_t>(split);
+    auto Lr = [&](const uint32_t i) {
+      return *reinterpret_cast<const double*>(Lb + (((i + usplit) & nm1) << 3));
     };
     // contiguous pixel ranges per workgroup: pixels are sorted, so
     // neighbouring lanes read neighbouring P(D) values
     const uint64_t per = (fd.npix + kPixBlocks - 1) / kPixBlocks;
     const uint64_t p0 = per * blk;
     const uint64_t p1 = (p0 + per < fd.npix) ? p0 + per : fd.npix;
-    // the term of pixel flux cflux, as PD::getLogLike takes it
+    const uint32_t cnt = (p1 > p0) ? static_cast<uint32_t>(p1 - p0) : 0u;
+    const char* pb = reinterpret_cast<const char*>(fd.pix + p0);
+    auto pix = [&](const uint32_t k) { return *reinterpret_cast<const double*>(pb + (k << 3)); };
+    // the term of pixel flux cflux, as PD::getLogLike takes it: Lr(0) at or
+    // below minflux, Lr(N - 1) at or above maxflux - dflux, else the linear
+    // interpolation; the end cases as weights (1, 0) of the same formula,
+    // which give the end value exactly (the table holds finite values)
     auto term = [&](const double cflux) {
-      if (cflux <= minflux) return Lr(0);
-      if (cflux >= maxfluxval) return Lr(N - 1);
-      const double delt = (cflux - minflux) * idflux;
-      const int idx = static_cast<int>(delt);
-      const double t = delt - static_cast<double>(idx);
+      const bool lo = cflux <= minflux, hi = cflux >= maxfluxval;
+      const double delt = (lo || hi) ? 0.0 : (cflux - minflux) * idflux;
+      const uint32_t id0 = static_cast<uint32_t>(static_cast<int>(delt));
+      const double t0 = delt - static_cast<double>(static_cast<int>(id0));
+      const uint32_t idx = hi ? nm1 : id0;
+      const double t = hi ? 0.0 : t0;
       return (1.0 - t) * Lr(idx) + t * Lr(idx + 1);
     };
-    // four pixels a step: their loads and gathers are issued together,
-    // their terms added in pixel order as a one-pixel loop would
-    constexpr int kU = 4;
-    uint64_t i = p0 + threadIdx.x;
-    for (; i + (kU - 1) * kPixThreads < p1; i += kU * kPixThreads) {
-      double cf[kU], tv[kU];
+    // kPixU pixels a step: their gathers are issued together, and the next
+    // step's pixel loads before them, so a step waits for one memory
+    // latency, not two; the terms are added in pixel order as a one-pixel
+    // loop would
+    constexpr uint32_t kStep = static_cast<uint32_t>(kPixU) * kPixThreads;
+    constexpr uint32_t kLast = static_cast<uint32_t>(kPixU - 1) * kPixThreads;
+    uint32_t i = threadIdx.x;
+    double cf[kPixU];
+    if (i + kLast < cnt) {
 #pragma unroll
-      for (int u = 0; u < kU; ++u) cf[u] = fd.pix[i + u * kPixThreads];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) tv[u] = term(cf[u]);
-#pragma unroll
-      for (int u = 0; u < kU; ++u) part += tv[u];
+      for (int u = 0; u < kPixU; ++u) cf[u] = pix(i + u * kPixThreads);
     }
-    for (; i < p1; i += kPixThreads) part += term(fd.pix[i]);
+    for (; i + kLast < cnt; i += kStep) {
+      double nx[kPixU];
+      const bool more = i + kStep + kLast < cnt;  // the loop's next condition
+#pragma unroll
+      for (int u = 0; u < kPixU; ++u) nx[u] = more ? pix(i + kStep + u * kPixThreads) : 0.0;
+      double tv[kPixU];
+#pragma unroll
+      for (int u = 0; u < kPixU; ++u) tv[u] = term(cf[u]);
+#pragma unroll
+      for (int u = 0; u < kPixU; ++u) part += tv[u];
+#pragma unroll
+      for (int u = 0; u < kPixU; ++u) cf[u] = nx[u];
+    }
+    for (; i < cnt; i += kPixThreads) part += term(pix(i));
   }
   const double s = block_sum(part, red);
   if (threadIdx.x == 0) *out = s;
