@@ -377,6 +377,7 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   // cspline_init + gsl_linalg_solve_symm_tridiag)
   const uint32_t sys = ninterp - 2;
   std::vector<double> ginv(nseg * (ninterp - 1)), lalpha(nseg * sys), lgamma(nseg * sys, 0.0);
+  int kwarm[2] = {pofd::kWarm, pofd::kWarm};
   for (int g = 0; g < nseg; ++g) {
     const double* xs = xnode.data() + g * ninterp;
     double* gi = ginv.data() + g * (ninterp - 1);
@@ -400,6 +401,44 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
       lg[i] = off[i] / la[i];
     }
     if (sys > 1) la[sys - 1] = diag[sys - 1] - off[sys - 2] * lg[sys - 2];
+    // warm-up of the kernel's truncated recurrences: a start k steps early
+    // is off by at most max(lg)^k of the skipped value, 2^-64 at this k
+    // (uniform nodes: max(lg) = 2 - sqrt(3) = 0.268, 34 steps)
+    double lgmax = 0.0;
+    for (uint32_t i = 0; i < sys; ++i) lgmax = std::max(lgmax, std::fabs(lg[i]));
+    int kw = pofd::kWarm;
+    if (lgmax < 0.5) {
+      kw = (lgmax > 0.0) ? static_cast<int>(std::ceil(64.0 / -std::log2(lgmax))) : 1;
+      kw = std::min(std::max(kw, 1), pofd::kWarm);
+    }
+    kwarm[g] = kw;
+  }
+  // fixed factors of the knot spline's system (the knot positions do not
+  // change with theta): h_i, 1/h_i, and the LDL^T diagonal and multipliers
+  // of GSL's cspline_init + gsl_linalg_solve_symm_tridiag, in its order
+  std::vector<double> kfac(4 * pofd::kMaxKnots, 0.0);
+  {
+    const int nkk = static_cast<int>(nk), ksys = nkk - 2;
+    const double* xa = c->logknots.data();
+    double* kh = kfac.data();
+    double* kgi = kh + pofd::kMaxKnots;
+    double* kal = kgi + pofd::kMaxKnots;
+    double* kgm = kal + pofd::kMaxKnots;
+    for (int i = 0; i + 1 < nkk; ++i) {
+      kh[i] = xa[i + 1] - xa[i];
+      kgi[i] = (kh[i] != 0.0) ? 1.0 / kh[i] : 0.0;
+    }
+    if (ksys >= 1) {
+      kal[0] = 2.0 * (kh[1] + kh[0]);
+      if (ksys > 1) {
+        kgm[0] = kh[1] / kal[0];
+        for (int i = 1; i < ksys - 1; ++i) {
+          kal[i] = 2.0 * (kh[i + 1] + kh[i]) - kh[i] * kgm[i - 1];
+          kgm[i] = kh[i + 1] / kal[i];
+        }
+        kal[ksys - 1] = 2.0 * (kh[ksys] + kh[ksys - 1]) - kh[ksys - 1] * kgm[ksys - 2];
+      }
+    }
   }
   // initR fill ranges (src/PDFactory.cc:560-613); empty ranges are [1, 0]
   const bool fillpos = haspos && (maxflux > 0);
@@ -446,6 +485,8 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   d.nbeam = static_cast<int>(nbeam);
   d.nbeamn = static_cast<int>(nbeamn);
   d.nseg = nseg;
+  d.kwarm[0] = kwarm[0];
+  d.kwarm[1] = kwarm[1];
   d.haspos = haspos ? 1 : 0;
   d.minit = static_cast<int>(minit);
   d.maxit = static_cast<int>(maxit);
@@ -469,6 +510,7 @@ int prepare_beamset(pofd_ctx* c, Beamset& s) {
   if ((rc = upload(c, s, ginv, &d.ginv))) return rc;
   if ((rc = upload(c, s, lalpha, &d.lalpha))) return rc;
   if ((rc = upload(c, s, lgamma, &d.lgamma))) return rc;
+  if ((rc = upload(c, s, kfac, &d.kfac))) return rc;
   if ((rc = upload(c, s, rk, &d.rk))) return rc;
   if ((rc = upload(c, s, rdelx, &d.rdelx))) return rc;
   if ((rc = upload(c, s, bw, &d.bw))) return rc;

@@ -91,6 +91,11 @@ struct BeamsetDev {
   const double* ginv;     // nseg*(ninterp-1): 1/h_i  (GSL cspline_init g_i)
   const double* lalpha;   // nseg*(ninterp-2): LDL^T diagonal of the node system
   const double* lgamma;   // nseg*(ninterp-2): LDL^T sub-diagonal multipliers
+  int kwarm[2];           // per segment: warm-up steps of the node system's
+                          // truncated recurrences, kwarm >= 64 / -log2(max
+                          // multiplier) (at most 64: the multipliers are < 1/2)
+  const double* kfac;     // 4 x kMaxKnots, the knot spline's fixed factors:
+                          // h_i, 1/h_i, LDL^T diagonal, LDL^T multipliers
   const int8_t* rk;       // (nbeam+nbeamn)*ninterp: knot interval or -1 (skip)
   const double* rdelx;    // (nbeam+nbeamn)*ninterp: log2|S| - logknots[k]
   const double* bw;       // nbeam+nbeamn weights (1.0 for a raw beam)

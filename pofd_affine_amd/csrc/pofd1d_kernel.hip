@@ -89,6 +89,16 @@ __device__ __forceinline__ double log2_or_small(double v) {
   return (v > 0.0) ? log2(v) : -30.0;  // pofd_mcmc::smalllogval
 }
 
+// lane `lane`'s v (lane uniform across the wavefront)
+__device__ __forceinline__ double readlane_d(const double v, const int lane) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = static_cast<unsigned>(
+      __builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b)), lane));
+  const unsigned hi = static_cast<unsigned>(
+      __builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b >> 32)), lane));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 __device__ __forceinline__ double qnan() {
   return __longlong_as_double(0x7ff8000000000000ULL);
 }
@@ -409,8 +419,10 @@ __device__ __forceinline__ double exp2_dflux(double s, const double* tab) {
 // the segment's theta-independent factors in sm.lg / sm.la / sm.gi, into the
 // walker's per-interval coefficients.  The tridiagonal matrix is fixed
 // (LDL^T factors from the host); forward and backward substitution run as
-// truncated recurrences: the multipliers are < 1/2, so starting kWarm = 64
-// steps early changes no significant bit.  Called by every thread.
+// truncated recurrences, each thread's started bs.kwarm[seg] steps early:
+// what the start leaves out is scaled by the product of that many
+// multipliers, at most 2^-64 (the multipliers are < 1/2; at most 64 steps).
+// Called by every thread.
 __device__ void node_spline(const BeamsetDev& bs, const WorkDev& wk, const int walker,
                             const int seg, PrepSmem& sm) {
 #pragma clang fp contract(off)
@@ -423,8 +435,12 @@ __device__ void node_spline(const BeamsetDev& bs, const WorkDev& wk, const int w
   __syncthreads();
   const int per = (sys + kThreads - 1) / kThreads;
   const int lo = tid * per, hi = min(lo + per, sys);
+  const int kw = bs.kwarm[seg];
+  // The recurrences' operands are read from LDS four steps ahead, so that
+  // a step waits only for the one before it (the steps and their order are
+  // those of the plain loops).
   if (lo < hi) {
-    int i0 = lo - kWarm;
+    int i0 = lo - kw;
     double z;
     if (i0 <= 0) {
       i0 = 0;
@@ -433,22 +449,46 @@ __device__ void node_spline(const BeamsetDev& bs, const WorkDev& wk, const int w
       z = sm.gR[i0];
     }
     if (i0 >= lo) sm.zz[i0] = z;
-    for (int i = i0 + 1; i < hi; ++i) {
+    int i = i0 + 1;
+    for (; i + 4 <= lo; i += 4) {  // warm-up: no stores
+      double g4[4], l4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        g4[u] = sm.gR[i + u];
+        l4[u] = sm.lg[i + u - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) z = g4[u] - l4[u] * z;
+    }
+    for (; i < lo; ++i) z = sm.gR[i] - sm.lg[i - 1] * z;
+    for (; i < hi; ++i) {
       z = sm.gR[i] - sm.lg[i - 1] * z;
-      if (i >= lo) sm.zz[i] = z;
+      sm.zz[i] = z;
     }
   }
   __syncthreads();
   for (int i = lo; i < hi; ++i) sm.zz[i] = sm.zz[i] / sm.la[i];
   __syncthreads();
   if (lo < hi) {
-    int i1 = hi - 1 + kWarm;
+    int i1 = hi - 1 + kw;
     if (i1 > sys - 1) i1 = sys - 1;
     double x = sm.zz[i1];
     if (i1 < hi) sm.xs[i1] = x;
-    for (int i = i1 - 1; i >= lo; --i) {
+    int i = i1 - 1;
+    for (; i - 3 >= hi; i -= 4) {  // warm-up: no stores
+      double z4[4], l4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        z4[u] = sm.zz[i - u];
+        l4[u] = sm.lg[i - u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x = z4[u] - l4[u] * x;
+    }
+    for (; i >= hi; --i) x = sm.zz[i] - sm.lg[i] * x;
+    for (; i >= lo; --i) {
       x = sm.zz[i] - sm.lg[i] * x;
-      if (i < hi) sm.xs[i] = x;
+      sm.xs[i] = x;
     }
   }
   __syncthreads();
@@ -469,14 +509,15 @@ __device__ void node_spline(const BeamsetDev& bs, const WorkDev& wk, const int w
 }
 
 // the segment's theta-independent node-spline factors into LDS
-__device__ void load_node_factors(const BeamsetDev& bs, const int seg, PrepSmem& sm) {
+// (by threads t0, t0 + nt, ...: the whole workgroup by default)
+__device__ void load_node_factors(const BeamsetDev& bs, const int seg, PrepSmem& sm,
+                                  const int t0 = threadIdx.x, const int nt = kThreads) {
   const int ninterp = bs.ninterp, sys = ninterp - 2;
-  for (int i = threadIdx.x; i < sys; i += kThreads) {
+  for (int i = t0; i < sys; i += nt) {
     sm.lg[i] = bs.lgamma[seg * sys + i];
     sm.la[i] = bs.lalpha[seg * sys + i];
   }
-  for (int i = threadIdx.x; i < ninterp - 1; i += kThreads)
-    sm.gi[i] = bs.ginv[seg * (ninterp - 1) + i];
+  for (int i = t0; i < ninterp - 1; i += nt) sm.gi[i] = bs.ginv[seg * (ninterp - 1) + i];
 }
 
 // A small batch spreads each walker over `parts` workgroups (the split
@@ -499,79 +540,75 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
   double* st = walker_state(wk, walker);
   const unsigned long long t0 = wall_clock64();
 
-  if (tid == 0) {
-    int status = accumulate ? prev_status[walker] : POFD_OK;
+  // the first node segment's fixed factors, loaded while wave 0 makes the
+  // knot spline
+  const int seg0 = bs.haspos ? 0 : 1;
+  if (parts == 1 && tid >= 64) load_node_factors(bs, seg0, sm, tid - 64, kThreads - 64);
+  if (tid < 64) {
+    // The knot spline on the 64 lanes of wave 0 (nknots <= 64), lane l for
+    // knot or interval l: GSL's natural cubic spline through (logknots,
+    // logknotvals), cspline_init + gsl_linalg_solve_symm_tridiag +
+    // coeff_calc, with the knot positions' factors (h, 1/h, LDL^T diagonal
+    // and multipliers) from the host, rounded as GSL's non-FMA build rounds
+    // them; the two substitutions run in order over broadcast lane values.
+    const int l = tid;
     const float* th = theta + static_cast<size_t>(walker) * stride;
-    double* lkv = sm.ks[6];
-    bool valid = true;
-    for (int i = 0; i < nk; ++i) {
-      // numberCountsKnotsSpline::setParams src/numberCountsKnotsSpline.cc:265-274
-      lkv[i] = mul_rn(bs.logfac, static_cast<double>(th[i]));
-      if (isnan(lkv[i])) valid = false;
+    // numberCountsKnotsSpline::setParams src/numberCountsKnotsSpline.cc:265-274
+    const double lk = (l < nk) ? mul_rn(bs.logfac, static_cast<double>(th[l])) : 0.0;
+    const bool valid = __ballot(l < nk && isnan(lk)) == 0;
+    const int sys = nk - 2;
+    const double* kf = bs.kfac;
+    const double hI = kf[l], giI = kf[kMaxKnots + l];
+    const double alI = kf[2 * kMaxKnots + l], gmI = kf[3 * kMaxKnots + l];
+    const double lk1 = __shfl_down(lk, 1), lk2 = __shfl_down(lk, 2);
+    const double giI1 = __shfl_down(giI, 1);
+    // right-hand side g_i (lanes i < sys)
+    const double ydI = sub_rn(lk1, lk), ydI1 = sub_rn(lk2, lk1);
+    const double gI = mul_rn(3.0, sub_rn(mul_rn(ydI1, giI1), mul_rn(ydI, giI)));
+    // z_0 = g_0, z_i = g_i - gamma_(i-1) z_(i-1); then z_i / alpha_i
+    double zl = 0.0, z = 0.0;
+    for (int i = 0; i < sys; ++i) {
+      const double gi = readlane_d(gI, i);
+      z = (i == 0) ? gi : sub_rn(gi, mul_rn(readlane_d(gmI, i - 1), z));
+      if (l == i) zl = z;
     }
-    const double sigmult = static_cast<double>(th[nk]);
-    if (status == POFD_OK) {
-      if (isnan(sigmult))
-        status = POFD_E_ARG;
-      else if (mul_rn(bs.maxsigma_base, sigmult) < 0.0)
-        status = POFD_E_BADLIKE;  // calcLikeSingle::bad_like, calcLike.cc:403
-      else if (!valid)
-        status = POFD_E_INVALID_MODEL;  // PDFactory.cc:1046-1047
+    if (l < sys) zl = zl / alI;
+    // c_sys = z_(sys-1), c_(i+1) = z_i - gamma_i c_(i+2); c_0 = c_(nk-1) = 0
+    double cl = 0.0;
+    if (sys >= 1) {
+      double cc = readlane_d(zl, sys - 1);
+      if (l == sys) cl = cc;
+      for (int i = sys - 2; i >= 0; --i) {
+        cc = sub_rn(readlane_d(zl, i), mul_rn(readlane_d(gmI, i), cc));
+        if (l == i + 1) cl = cc;
+      }
     }
-    if (part == 0) {
-      st[ST_STATUS] = status;
-      st[ST_SIGMULT] = sigmult;
+    const double cl1 = __shfl_down(cl, 1);
+    if (valid && l < nk - 1) {
+      const double dx = hI;
+      const double dy = sub_rn(lk1, lk);
+      sm.kn[0][l] = lk;
+      sm.kn[1][l] = sub_rn(dy / dx, mul_rn(dx, add_rn(cl1, mul_rn(2.0, cl))) / 3.0);
+      sm.kn[2][l] = cl;
+      sm.kn[3][l] = sub_rn(cl1, cl) / mul_rn(3.0, dx);
     }
-    sm.isc[0] = status;
-    sm.isc[1] = valid ? 1 : 0;
-    if (valid) {
-      // GSL natural cubic spline through (logknots, logknotvals):
-      // cspline_init + gsl_linalg_solve_symm_tridiag + coeff_calc
-      const double* xa = bs.logknots;
-      const int sys = nk - 2;
-      double* c = sm.ks[0];
-      double* diag = sm.ks[1];
-      double* off = sm.ks[2];
-      double* g = sm.ks[3];
-      for (int i = 0; i < nk; ++i) c[i] = 0.0;
-      for (int i = 0; i < sys; ++i) {
-        const double h_i = sub_rn(xa[i + 1], xa[i]);
-        const double h_ip1 = sub_rn(xa[i + 2], xa[i + 1]);
-        const double yd_i = sub_rn(lkv[i + 1], lkv[i]);
-        const double yd_ip1 = sub_rn(lkv[i + 2], lkv[i + 1]);
-        const double g_i = (h_i != 0.0) ? 1.0 / h_i : 0.0;
-        const double g_ip1 = (h_ip1 != 0.0) ? 1.0 / h_ip1 : 0.0;
-        off[i] = h_ip1;
-        diag[i] = mul_rn(2.0, add_rn(h_ip1, h_i));
-        g[i] = mul_rn(3.0, sub_rn(mul_rn(yd_ip1, g_ip1), mul_rn(yd_i, g_i)));
+    if (l == 0) {
+      int status = accumulate ? prev_status[walker] : POFD_OK;
+      const double sigmult = static_cast<double>(th[nk]);
+      if (status == POFD_OK) {
+        if (isnan(sigmult))
+          status = POFD_E_ARG;
+        else if (mul_rn(bs.maxsigma_base, sigmult) < 0.0)
+          status = POFD_E_BADLIKE;  // calcLikeSingle::bad_like, calcLike.cc:403
+        else if (!valid)
+          status = POFD_E_INVALID_MODEL;  // PDFactory.cc:1046-1047
       }
-      if (sys == 1) {
-        c[1] = g[0] / diag[0];
-      } else if (sys > 1) {
-        double* alpha = sm.ks[4];
-        double* gamma = sm.ks[5];
-        double* z = diag;  // diag[i] is dead once alpha[i] is formed
-        alpha[0] = diag[0];
-        gamma[0] = off[0] / alpha[0];
-        for (int i = 1; i < sys - 1; ++i) {
-          alpha[i] = sub_rn(diag[i], mul_rn(off[i - 1], gamma[i - 1]));
-          gamma[i] = off[i] / alpha[i];
-        }
-        alpha[sys - 1] = sub_rn(diag[sys - 1], mul_rn(off[sys - 2], gamma[sys - 2]));
-        z[0] = g[0];
-        for (int i = 1; i < sys; ++i) z[i] = sub_rn(g[i], mul_rn(gamma[i - 1], z[i - 1]));
-        for (int i = 0; i < sys; ++i) z[i] = z[i] / alpha[i];
-        c[sys] = z[sys - 1];
-        for (int i = sys - 2; i >= 0; --i) c[i + 1] = sub_rn(z[i], mul_rn(gamma[i], c[i + 2]));
+      if (part == 0) {
+        st[ST_STATUS] = status;
+        st[ST_SIGMULT] = sigmult;
       }
-      for (int i = 0; i < nk - 1; ++i) {
-        const double dx = sub_rn(xa[i + 1], xa[i]);
-        const double dy = sub_rn(lkv[i + 1], lkv[i]);
-        sm.kn[0][i] = lkv[i];
-        sm.kn[1][i] = sub_rn(dy / dx, mul_rn(dx, add_rn(c[i + 1], mul_rn(2.0, c[i]))) / 3.0);
-        sm.kn[2][i] = c[i];
-        sm.kn[3][i] = sub_rn(c[i + 1], c[i]) / mul_rn(3.0, dx);
-      }
+      sm.isc[0] = status;
+      sm.isc[1] = valid ? 1 : 0;
     }
   }
   __syncthreads();
@@ -615,8 +652,8 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
 
   // one pass per node segment: the positive beam, then the negative beam
   const int node_lo = part * ninterp / parts, node_hi = (part + 1) * ninterp / parts;
-  for (int seg = bs.haspos ? 0 : 1; seg < bs.nseg; ++seg) {
-    if (parts == 1) load_node_factors(bs, seg, sm);
+  for (int seg = seg0; seg < bs.nseg; ++seg) {
+    if (parts == 1 && seg != seg0) load_node_factors(bs, seg, sm);
 
     // ---- R at the interpolation nodes (initRInterp + array getR) ---------
     {
