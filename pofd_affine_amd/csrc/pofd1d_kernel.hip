@@ -35,6 +35,8 @@
 // per walker and dataset is the P(D) itself.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "internal.h"
 
 namespace pofd {
@@ -347,6 +349,7 @@ __device__ __forceinline__ double* walker_state(const WorkDev& wk, int w) {
 }
 
 // ---- LDS layouts -----------------------------------------------------------
+constexpr int kBeamStage = 1024;
 struct PrepSmem {
   double yn[2048];   // log2 R at the nodes
   double gR[2048];   // right-hand side of the node system
@@ -356,7 +359,8 @@ struct PrepSmem {
   double la[2048];   // LDL^T diagonal
   double gi[2048];   // 1/h
   double kn[4][kMaxKnots];
-  double ks[7][kMaxKnots];
+  double bw[kBeamStage];  // beam weights and inverse values (staged when
+  double be[kBeamStage];  // nbeam + nbeamn <= kBeamStage)
   double red[kWaves];
   int isc[4];
 };
@@ -544,6 +548,14 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
   // knot spline
   const int seg0 = bs.haspos ? 0 : 1;
   if (parts == 1 && tid >= 64) load_node_factors(bs, seg0, sm, tid - 64, kThreads - 64);
+  // and the beam weights and inverse values, when they fit in LDS
+  const bool staged = bs.nbeam + bs.nbeamn <= kBeamStage;
+  if (staged) {
+    for (int j = tid - 64; j >= 0 && j < bs.nbeam + bs.nbeamn; j += kThreads - 64) {
+      sm.bw[j] = bs.bw[j];
+      sm.be[j] = bs.bieta[j];
+    }
+  }
   if (tid < 64) {
     // The knot spline on the 64 lanes of wave 0 (nknots <= 64), lane l for
     // knot or interval l: GSL's natural cubic spline through (logknots,
@@ -656,53 +668,66 @@ __global__ __launch_bounds__(kThreads) void prep1d_kernel(
     if (parts == 1 && seg != seg0) load_node_factors(bs, seg, sm);
 
     // ---- R at the interpolation nodes (initRInterp + array getR) ---------
+    // (the beam weights and inverse values from LDS when they fit there:
+    // read from global memory inside a term they would stall the term, and
+    // with it the next group's table loads, on their latency)
     {
-#pragma clang fp contract(off)
       const int j0 = (seg == 0) ? 0 : bs.nbeam;
       const int j1 = (seg == 0) ? bs.nbeam : bs.nbeam + bs.nbeamn;
       double* ynode_g = wk.ynode + walker * wk.ynode_stride + seg * ninterp;
-      for (int i = node_lo + tid; i < node_hi; i += kThreads) {
-        double workval = 0.0;
-        // beam bins in groups of 8, the table loads of the next group
-        // issued before the current group's terms are added (in bin order)
-        int kq[8], kn8[8];
-        double dq[8], dn8[8];
-        auto load_group = [&](const int jg, int* kk, double* dd) {
+      auto rnodes = [&](auto staged) {
+#pragma clang fp contract(off)
+        constexpr bool kStaged = decltype(staged)::value;
+        for (int i = node_lo + tid; i < node_hi; i += kThreads) {
+          double workval = 0.0;
+          // beam bins in groups of 8, the table loads of the next group
+          // issued before the current group's terms are added (in bin order)
+          int kq[8], kn8[8];
+          double dq[8], dn8[8];
+          auto load_group = [&](const int jg, int* kk, double* dd) {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int j = jg + u;
-            kk[u] = -1;
-            dd[u] = 0.0;
-            if (j < j1) {  // uniform across the workgroup
-              const size_t e = static_cast<size_t>(j) * ninterp + i;
-              kk[u] = bs.rk[e];
-              dd[u] = bs.rdelx[e];
+            for (int u = 0; u < 8; ++u) {
+              const int j = jg + u;
+              kk[u] = -1;
+              dd[u] = 0.0;
+              if (j < j1) {  // uniform across the workgroup
+                const size_t e = static_cast<size_t>(j) * ninterp + i;
+                kk[u] = bs.rk[e];
+                dd[u] = bs.rdelx[e];
+              }
+            }
+          };
+          if (j0 < j1) load_group(j0, kq, dq);
+          for (int jg = j0; jg < j1; jg += 8) {
+            if (jg + 8 < j1) load_group(jg + 8, kn8, dn8);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int k = kq[u];
+              if (k < 0) continue;
+              const int j = jg + u;
+              const double dx = dq[u];
+              const double sv =
+                  sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
+              const double cts = exp2(sv);
+              const double wj = kStaged ? sm.bw[j] : bs.bw[j];
+              const double ej = kStaged ? sm.be[j] : bs.bieta[j];
+              if (cts > 0) workval += wj * cts * ej;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              kq[u] = kn8[u];
+              dq[u] = dn8[u];
             }
           }
-        };
-        if (j0 < j1) load_group(j0, kq, dq);
-        for (int jg = j0; jg < j1; jg += 8) {
-          if (jg + 8 < j1) load_group(jg + 8, kn8, dn8);
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = kq[u];
-            if (k < 0) continue;
-            const int j = jg + u;
-            const double dx = dq[u];
-            const double s =
-                sm.kn[0][k] + dx * (sm.kn[1][k] + dx * (sm.kn[2][k] + dx * sm.kn[3][k]));
-            const double cts = exp2(s);
-            if (cts > 0) workval += bs.bw[j] * cts * bs.bieta[j];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            kq[u] = kn8[u];
-            dq[u] = dn8[u];
-          }
+          const double R = bs.prefac * workval;
+          sm.yn[i] = log2_or_small(R);
+          ynode_g[i] = sm.yn[i];
         }
-        const double R = bs.prefac * workval;
-        sm.yn[i] = log2_or_small(R);
-        ynode_g[i] = sm.yn[i];
+      };
+      if (staged) {
+        rnodes(std::true_type{});
+      } else {
+        rnodes(std::false_type{});
       }
     }
     __syncthreads();
