@@ -37,7 +37,7 @@ FP64_PEAK_TFLOPS = 78.6
 # 16x16x4 f64 MFMA is 2048 FLOP, counted as 4 MOPS).  Divided by the live
 # kernel time for the FP64 roofline.
 FP64_FLOP = {
-    "prep": 3.156e8,
+    "prep": 2.738e8,
     "forward": 2.924e9,
     "inverse": 1.943e9,
     "finalize": 6.39e7,
