@@ -245,7 +245,12 @@ __device__ __forceinline__ int stw_base(int Ns) { return Ns == 4 ? 0 : (Ns == 16
 
 template <int SIGN>
 __device__ void fft256_cols(double2* a, double2* b, const double2* wst) {
-  const int tid = threadIdx.x;
+  // forward: the thread index through an empty asm, so that the LDS
+  // offsets below are recomputed in each call instead of being held (and
+  // spilled) across the chunk loop; the inverse has the registers to hold
+  // them
+  int tid = threadIdx.x;
+  if (SIGN < 0) asm volatile("" : "+v"(tid));
   const int col = tid >> 6, j = tid & 63;
   double2* src = a;
   double2* dst = b;
@@ -881,7 +886,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
   // this part's chunk groups (kGroups / parts of them) and their chunks
   const int g_lo = part * kGroups / parts, g_hi = (part + 1) * kGroups / parts;
   const int c_lo = group_chunk(g_lo, nchunks), c_hi = group_chunk(g_hi, nchunks);
-  const int row4 = tid >> 2, q = tid & 3;
+  const int row4 = tid >> 2;
   // moments: this thread's share of the open group; the group tree in
   // sm.gtree
   double mn_part = 0.0, var_part = 0.0;
@@ -942,6 +947,11 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
     // forward_tail adds its part directly
     const bool head_row = (row4 == 0) || (bs.headneg && row4 == kM1 - 1);
     auto fill_chunk = [&](const int c, double2* dst) {
+      // row and quad lane from the thread index through an empty asm, so
+      // that the LDS offsets are recomputed per chunk rather than held
+      int tf = threadIdx.x;
+      asm volatile("" : "+v"(tf));
+      const int row4 = tf >> 2, q = tf & 3;
       const int j0 = 2 * M2 * row4 + 2 * kChunk * c + 8 * q;
       // headn is a multiple of 8: a thread's 8 points are all in the head or
       // all outside it
@@ -1035,7 +1045,11 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       // Yt at rows k1w + lj, columns 4 m + lk of the chunk, and V of those
       // columns at this lane's output row lr (zero past the pass's rows)
       // (one column block's real and imaginary steps after each other, so
-      // that only its operands are live)
+      // that only its operands are live; the lane indices through an empty
+      // asm, so that the LDS offsets are recomputed per chunk)
+      int ta = threadIdx.x;
+      asm volatile("" : "+v"(ta));
+      const int lj = ta & 15, lk = (ta >> 4) & 3, k1w = 16 * (ta >> 6), lr = lj & 7;
       const int k2 = (lr < np) ? sm.rowk2[p0 + lr] : 0;
       const bool im_col = lj >= 8;
 #pragma unroll
