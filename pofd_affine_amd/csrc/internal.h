@@ -36,7 +36,7 @@ constexpr int kChunk = 16;       // columns per LDS chunk
 constexpr int kRowsPerPass = 8;  // spectral rows accumulated per forward pass
 constexpr int kMaxKnots = 64;
 constexpr int kBlk = 32;         // P(D) values per min/sum block
-constexpr int kWarm = 64;        // warm-up steps of the truncated recurrences
+constexpr int kWarm = 64;        // most warm-up steps of the truncated recurrences
 constexpr int kDbg = 32;         // debug scalars per (walker, field)
 constexpr int kState = 16;       // per-walker scalars handed between kernels
 constexpr int kPixBlocks = 64;   // workgroups per (walker, field), unbinned
