@@ -1,7 +1,7 @@
 """GPU parity of the fused 1D P(D) likelihood against the CPU oracle.
 
 Every call goes through the C ABI (include/pofd_affine_amd.h).  Tolerances
-are the north-star ones: log likelihood within 1e-6 absolute, P(D) within
+are the parity bars: log likelihood within 1e-6 absolute, P(D) within
 1e-10 of its peak; intermediate stages are held tighter where the
 arithmetic allows it.
 """
