@@ -809,6 +809,13 @@ int ensure_work2(pofd_ctx* c, Ctx2& C2, Set2& s, uint32_t nwalk) {
   const size_t o_lp = tot; tot += r8(pofd2::kLikeBlocks) * nf * W;
   const size_t o_dbg = tot; tot += r8(pofd2::kDbg2) * nf * W;
   const size_t o_bo = tot; tot += 8 * W;
+  // the R grids' per-row beam pieces (rows of the largest grid, n or nedge)
+  int maxc = 1;
+  for (int k = 0; k < 2; ++k)
+    maxc = std::max(maxc, s.dev.comp[2 * k].n + s.dev.comp[2 * k + 1].n);
+  w.rpieces_w = 3 * maxc;
+  w.rpieces_stride = r8(std::max(n, ne) * static_cast<size_t>(w.rpieces_w));
+  const size_t o_rp = tot; tot += w.rpieces_stride * W;
   s.wbuf.reset();
   int rc = alloc(c, s.wbuf, tot * sizeof(double));
   if (rc) return rc;
@@ -837,6 +844,7 @@ int ensure_work2(pofd_ctx* c, Ctx2& C2, Set2& s, uint32_t nwalk) {
   w.lpart = b + o_lp;
   w.dbg = b + o_dbg;
   w.bonus = b + o_bo;
+  w.rpieces = b + o_rp;
   s.gmain.out = w.R;
   s.gmain.wstride = w.R_stride;
   s.g00.out = w.e00;

@@ -128,8 +128,12 @@ struct Work2 {
   double* lpart;                  // W * nf * kLikeBlocks
   double* dbg;                    // W * nf * kDbg2
   double* bonus;                  // W * 8: NS, S1, S2, S1^2, S2^2 per area
+  double* rpieces;                // W * rpieces_stride: per row of the R grid
+                                  //   being filled, rpieces_w values (3 per
+                                  //   beam element: weight, centre, width)
+  int rpieces_w;
   size_t R_stride, ex_stride, edge_stride, Y_stride, YT_stride, PR_stride,
-      Q_stride, QT_stride, P_stride, colpart_stride;
+      Q_stride, QT_stride, P_stride, colpart_stride, rpieces_stride;
   int ystride;                    // n/2 + 1 (complex)
 };
 

@@ -371,9 +371,14 @@ __global__ __launch_bounds__(64) void bonus2d_kernel(const Set2Dev s, const Work
 // rgrid2d: R on (f1[i], f2[j]) for every walker; one workgroup per
 // (row block, walker).  out[i*ostride + j] = R * scale.
 // ===========================================================================
-__global__ __launch_bounds__(kT) void rgrid2d_kernel(const Set2Dev s, const Work2 wk,
-                                                     const RGrid g) {
-  extern __shared__ double rw[];  // 3 * (np + nn) per row
+// The pieces of R that depend on the row flux f1 but not on f2, per row
+// of the grid and beam element (RWork, :1868-1950): weight (0 where the
+// element adds nothing), log-flux centre and Gaussian width factor, into
+// wk.rpieces for rgrid2d_kernel.  One thread per (row, element), so that
+// their logs, exp2s and colour splines run in parallel rather than on a
+// few threads of each row's workgroup.
+__global__ __launch_bounds__(kT) void rpieces2d_kernel(const Set2Dev s, const Work2 wk,
+                                                       const RGrid g) {
   __shared__ double lk[kMaxK], lkcf[4 * kMaxK];
   __shared__ double sk[kMaxK], scf[4 * kMaxK], svl[kMaxK];
   __shared__ double ok[kMaxK], ocf[4 * kMaxK], ovl[kMaxK];
@@ -394,8 +399,58 @@ __global__ __launch_bounds__(kT) void rgrid2d_kernel(const Set2Dev s, const Work
     ocf[t] = m[kOffO + t];
   }
   __syncthreads();
+  const int maxc = wk.rpieces_w / 3;
+  const int t = blockIdx.x * kT + threadIdx.x;
+  const int i = t / maxc, e = t - i * maxc;
+  if (i >= g.n1) return;
+  double f1val = g.f1[i];
+  if ((f1val == 0) || (f1val < 0 && !s.hasneg1)) return;  // a row of zeros
+  int sgn1 = 0;
+  if (!(f1val > 0)) {
+    sgn1 = 2;
+    f1val = fabs(f1val);
+  }
+  const int np = s.comp[sgn1].n, nn = s.comp[sgn1 + 1].n;
+  if (e >= np + nn) return;
   const double minknot = s.knots[0], maxknot = s.knots[nk - 1];
   const double ilog2toe = 1.4426950408889634074;  // 1/ln 2
+  const double logf1 = log(f1val);
+  const int k = (e < np) ? 0 : 1;
+  const BeamComp& bc = s.comp[sgn1 + k];
+  const int jj = (k == 0) ? e : e - np;
+  double* c = wk.rpieces + static_cast<size_t>(w) * wk.rpieces_stride +
+              static_cast<size_t>(i) * wk.rpieces_w + 3 * e;
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+  if (f1val < bc.maxf1) {
+#pragma clang fp contract(off)
+    const double ieta1 = bc.ieta1[jj];
+    const double f1prod = f1val * ieta1;
+    if (f1prod >= minknot && f1prod < maxknot) {
+      const double cts = exp2(spline_eval(lk, lkcf, nk, log2(f1prod)));
+      if (cts > 0) {
+        const double isigma = 1.0 / colour_eval(sk, svl, scf, ns, f1prod, s.min_sigma, true);
+        c0 = (bc.w ? bc.w[jj] * ieta1 * isigma * cts * s.prefac
+                   : cts * ieta1 * isigma * s.prefac);
+        c1 = logf1 + colour_eval(ok, ovl, ocf, no, f1prod, 0.0, false) - bc.logratio[jj];
+        c2 = -0.5 * isigma * isigma * ilog2toe;
+      }
+    }
+  }
+  c[0] = c0;
+  c[1] = c1;
+  c[2] = c2;
+}
+
+// R on one flux grid, a row per workgroup step: the row's pieces (from
+// rpieces2d_kernel) into LDS, then every f2 of the row sums its beam
+// elements' Gaussian colour terms in element order.
+__global__ __launch_bounds__(kT) void rgrid2d_kernel(const Set2Dev s, const Work2 wk,
+                                                     const RGrid g) {
+  extern __shared__ double rw[];  // 3 * (np + nn) per row
+  const int w = blockIdx.y;
+  const double* m = model_ptr(wk, w);
+  if (m[0] != POFD_OK) return;  // uniform
+  const double* pieces = wk.rpieces + static_cast<size_t>(w) * wk.rpieces_stride;
   for (int i = blockIdx.x; i < g.n1; i += gridDim.x) {
     double f1val = g.f1[i];
     if ((f1val == 0) || (f1val < 0 && !s.hasneg1)) {
@@ -403,37 +458,10 @@ __global__ __launch_bounds__(kT) void rgrid2d_kernel(const Set2Dev s, const Work
         g.out[static_cast<size_t>(w) * g.wstride + static_cast<size_t>(i) * g.ostride + j] = 0.0;
       continue;
     }
-    int sgn1 = 0;
-    if (!(f1val > 0)) {
-      sgn1 = 2;
-      f1val = fabs(f1val);
-    }
-    const double logf1 = log(f1val);
+    const int sgn1 = (f1val > 0) ? 0 : 2;
     const int np = s.comp[sgn1].n, nn = s.comp[sgn1 + 1].n;
-    // pre-computed pieces per beam element (RWork, :1868-1950)
-    for (int e = threadIdx.x; e < np + nn; e += kT) {
-      const int k = (e < np) ? 0 : 1;
-      const BeamComp& bc = s.comp[sgn1 + k];
-      const int jj = (k == 0) ? e : e - np;
-      double* c = rw + 3 * e;
-      c[0] = 0.0;
-      if (f1val < bc.maxf1) {
-#pragma clang fp contract(off)
-        const double ieta1 = bc.ieta1[jj];
-        const double f1prod = f1val * ieta1;
-        if (f1prod >= minknot && f1prod < maxknot) {
-          const double cts = exp2(spline_eval(lk, lkcf, nk, log2(f1prod)));
-          if (cts > 0) {
-            const double isigma =
-                1.0 / colour_eval(sk, svl, scf, ns, f1prod, s.min_sigma, true);
-            c[0] = (bc.w ? bc.w[jj] * ieta1 * isigma * cts * s.prefac
-                         : cts * ieta1 * isigma * s.prefac);
-            c[1] = logf1 + colour_eval(ok, ovl, ocf, no, f1prod, 0.0, false) - bc.logratio[jj];
-            c[2] = -0.5 * isigma * isigma * ilog2toe;
-          }
-        }
-      }
-    }
+    const double* prow = pieces + static_cast<size_t>(i) * wk.rpieces_w;
+    for (int e = threadIdx.x; e < 3 * (np + nn); e += kT) rw[e] = prow[e];
     __syncthreads();
     double* orow = g.out + static_cast<size_t>(w) * g.wstride + static_cast<size_t>(i) * g.ostride;
     for (int j = threadIdx.x; j < g.n2; j += kT) {
@@ -1285,24 +1313,28 @@ int launch_loglike2d(const Set2Dev& s, const Work2& wk, const RGrid& gmain,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(rg_lds));
   const int rblocks = 64;  // row blocks per walker (grid-stride over rows)
+  // one R grid: its per-row beam pieces, then the grid
+  auto rgrid = [&](const RGrid& g, const int blocks) {
+    const int pb = static_cast<int>((static_cast<size_t>(g.n1) * (wk.rpieces_w / 3) + kT - 1) / kT);
+    hipLaunchKernelGGL(rpieces2d_kernel, dim3(pb, W), dim3(kT), 0, st, s, wk, g);
+    hipLaunchKernelGGL(rgrid2d_kernel, dim3(blocks, W), dim3(kT), rg_lds, st, s, wk, g);
+  };
   mark(POFD_STAGE_PREP);
   hipLaunchKernelGGL(prep2d_kernel, dim3(W), dim3(64), 0, st, s, wk, d_theta, stride, d_st,
                      accumulate);
   if (s.bonus) hipLaunchKernelGGL(bonus2d_kernel, dim3(W), dim3(64), 0, st, s, wk);
   mark(POFD_STAGE_FORWARD);
   if (s.edge_integ) {
-    hipLaunchKernelGGL(rgrid2d_kernel, dim3(std::min(rblocks, ne), W), dim3(kT), rg_lds, st, s,
-                       wk, g00);
-    hipLaunchKernelGGL(rgrid2d_kernel, dim3(std::min(rblocks, ne), W), dim3(kT), rg_lds, st, s,
-                       wk, gx);
-    hipLaunchKernelGGL(rgrid2d_kernel, dim3(rblocks, W), dim3(kT), rg_lds, st, s, wk, gy);
+    rgrid(g00, std::min(rblocks, ne));
+    rgrid(gx, std::min(rblocks, ne));
+    rgrid(gy, rblocks);
     hipLaunchKernelGGL(edge2d_rows_kernel, dim3((ne + 255) / 256, W), dim3(256), 0, st, s, wk,
                        ne, wk.e00, g00.wstride, wk.r00rows, wk.edge_stride, 0.0);
     hipLaunchKernelGGL(edge2d_rows_kernel, dim3((n + 255) / 256, W), dim3(256), 0, st, s, wk,
                        n, wk.ey, gy.wstride, wk.ry, wk.edge_stride, 0.0);
     hipLaunchKernelGGL(edge2d_cols_kernel, dim3((n + 255) / 256, W), dim3(256), 0, st, s, wk);
   }
-  hipLaunchKernelGGL(rgrid2d_kernel, dim3(rblocks * 4, W), dim3(kT), rg_lds, st, s, wk, gmain);
+  rgrid(gmain, rblocks * 4);
   if (s.edge_integ)
     hipLaunchKernelGGL(edge2d_patch_kernel, dim3((n + 255) / 256, W), dim3(256), 0, st, s, wk);
   const int nparts = 64;
