@@ -243,22 +243,60 @@ __device__ __forceinline__ int blk_slot(const int b, const int nbc) {
 constexpr int kStageTw = 4 + 16 + 64;
 __device__ __forceinline__ int stw_base(int Ns) { return Ns == 4 ? 0 : (Ns == 16 ? 4 : 20); }
 
+// The 20 LDS element offsets of one thread's column transforms (the four
+// reads e = j + 64 m, the same in every stage, then four writes per stage),
+// two 16-bit offsets per word (they are below 16 x 257): held across the
+// forward's chunk loop in 10 registers instead of 20.
+struct ColOffsets {
+  uint32_t w[10];
+  __device__ __forceinline__ int get(const int k) const {
+    return static_cast<int>((w[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+  }
+};
+
+__device__ ColOffsets col_offsets() {
+  const int tid = threadIdx.x, col = tid >> 6, j = tid & 63;
+  int o[20];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) o[m] = cpos(col, j + 64 * m);
+  int k = 4;
+#pragma unroll
+  for (int Ns = 1; Ns < kM1; Ns *= 4) {
+    const int d = (j / Ns) * Ns * 4 + (j & (Ns - 1));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[k++] = cpos(col, d + q * Ns);
+  }
+  ColOffsets c;
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+    c.w[i] = static_cast<uint32_t>(o[2 * i]) | (static_cast<uint32_t>(o[2 * i + 1]) << 16);
+  return c;
+}
+
+// co: the packed offsets (forward), or null to compute them here (inverse)
 template <int SIGN>
-__device__ void fft256_cols(double2* a, double2* b, const double2* wst) {
-  // forward: the thread index through an empty asm, so that the LDS
-  // offsets below are recomputed in each call instead of being held (and
-  // spilled) across the chunk loop; the inverse has the registers to hold
-  // them
-  int tid = threadIdx.x;
-  if (SIGN < 0) asm volatile("" : "+v"(tid));
+__device__ void fft256_cols(double2* a, double2* b, const double2* wst,
+                            const ColOffsets* co = nullptr) {
+  const int tid = threadIdx.x;
   const int col = tid >> 6, j = tid & 63;
   double2* src = a;
   double2* dst = b;
+  int stage = 0;
 #pragma unroll
-  for (int Ns = 1; Ns < kM1; Ns *= 4) {
-    double2 v0 = src[cpos(col, j)], v1 = src[cpos(col, j + 64)];
-    double2 v2 = src[cpos(col, j + 128)], v3 = src[cpos(col, j + 192)];
-    const int jm = j & (Ns - 1);
+  for (int Ns = 1; Ns < kM1; Ns *= 4, ++stage) {
+    double2 v0, v1, v2, v3;
+    if (co) {
+      v0 = src[co->get(0)];
+      v1 = src[co->get(1)];
+      v2 = src[co->get(2)];
+      v3 = src[co->get(3)];
+    } else {
+      v0 = src[cpos(col, j)];
+      v1 = src[cpos(col, j + 64)];
+      v2 = src[cpos(col, j + 128)];
+      v3 = src[cpos(col, j + 192)];
+    }
+  const int jm = j & (Ns - 1);
     if (Ns > 1) {
       const int e = stw_base(Ns) + jm;
       double2 t1 = wst[e], t2 = wst[kStageTw + e], t3 = wst[2 * kStageTw + e];
@@ -281,11 +319,18 @@ __device__ void fft256_cols(double2* a, double2* b, const double2* wst) {
       y1 = make_double2(a1.x - a3.y, a1.y + a3.x);
       y3 = make_double2(a1.x + a3.y, a1.y - a3.x);
     }
-    const int d = (j / Ns) * Ns * 4 + jm;
-    dst[cpos(col, d)] = cadd(a0, a2);
-    dst[cpos(col, d + Ns)] = y1;
-    dst[cpos(col, d + 2 * Ns)] = csub(a0, a2);
-    dst[cpos(col, d + 3 * Ns)] = y3;
+    if (co) {
+      dst[co->get(4 + 4 * stage)] = cadd(a0, a2);
+      dst[co->get(5 + 4 * stage)] = y1;
+      dst[co->get(6 + 4 * stage)] = csub(a0, a2);
+      dst[co->get(7 + 4 * stage)] = y3;
+    } else {
+      const int d = (j / Ns) * Ns * 4 + jm;
+      dst[cpos(col, d)] = cadd(a0, a2);
+      dst[cpos(col, d + Ns)] = y1;
+      dst[cpos(col, d + 2 * Ns)] = csub(a0, a2);
+      dst[cpos(col, d + 3 * Ns)] = y3;
+    }
     // a column belongs to one wavefront (64 threads) in every stage, so
     // between stages only that wavefront's LDS accesses need ordering; the
     // last stage ends with a workgroup barrier for the callers, which read
@@ -896,6 +941,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
   // forward_tail fills only later
   double2* zhi = (parts == 1) ? zrows + bs.M : zdst;
 
+  const ColOffsets co = col_offsets();
   for (int p0 = 0; p0 < nrows; p0 += kRowsPerPass) {
     const int np = min(kRowsPerPass, nrows - p0);
     // The pass's kept rows accumulate on the FP64 matrix cores:
@@ -1072,7 +1118,7 @@ __global__ __launch_bounds__(kThreads) void forward1d_kernel(const BeamsetDev bs
       double2* cur = ((c - c_lo) & 1) ? sm.bufb : sm.bufa;
       double2* oth = ((c - c_lo) & 1) ? sm.bufa : sm.bufb;
       if (c >= c_lo) {
-        fft256_cols<-1>(cur, oth, sm.wst);
+        fft256_cols<-1>(cur, oth, sm.wst, &co);
         accum_chunk(c, cur);
         close_groups(c + 1);
       }
